@@ -1,0 +1,203 @@
+// Cone-beam geometry shared by the host entry points and the gfx950 kernels.
+//
+// Every geometry is reduced to per-view FP64 vectors in the voxel frame (the
+// frame where voxel (ix, iy, iz) has its sample centre at the integer point
+// (ix, iy, iz)): source S, pixel (0,0) position D0, per-pixel steps U and V.
+// Rays are normalised; samples sit one voxel apart at
+// beta_k = beta_start + k, beta_start = min(alpha0, alpha1) - margin, where
+// alpha0/1 are the distances to the box planes of the ray's dominant axis.
+// This reproduces the sample phase of
+//   Standard/plug/kernel/forwardKernel.cu:30-80 (axis among x,y; ties -> x)
+//   Beijing/plug/kernel/forwardKernel.cu:24-73  (axis among x,y,z; ties x->y->z)
+// Per-ray set-up runs in FP64 with FP contraction off so that it follows the
+// same rounding as the host-side table builder; sample positions are then
+// advanced in 32.32 fixed point, which keeps them exact to ~1e-7 voxel over
+// any ray length (FP32 stepping would drift by ~1e-5 voxel at 256^3).
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#ifndef DTV_HD
+#if defined(__HIPCC__)
+#define DTV_HD __host__ __device__
+#else
+#define DTV_HD
+#endif
+#endif
+
+namespace dtv {
+
+// ``#define PI 3.14159265359`` of the reference kernels (forwardKernel.cu:14).
+constexpr double kPiRef = 3.14159265359;
+constexpr double kFix = 4294967296.0;            // 2^32
+constexpr float kInvFix = 2.3283064365386963e-10f; // 2^-32
+
+enum ViewKind : int { kTable = 0, kStandard = 1, kBeijing = 2 };
+enum AxisMode : int { kAxisXY = 2, kAxisXYZ = 3 };
+
+struct View {
+  double S[3], D0[3], U[3], V[3];
+};
+
+// Launch-time description of one projector call (passed by value).
+struct Geo {
+  int nx, ny, nz;       // volume size, x fastest
+  int nu, nv;           // detector size, u fastest
+  int n_views;
+  int axis_mode;        // kAxisXY or kAxisXYZ
+  int kind;             // ViewKind: where the per-view vectors come from
+  double lo[3], hi[3];  // box whose dominant-axis planes phase the samples
+  double margin;        // voxels added at both ends of the sample range
+  double scale;         // output multiplier
+  double sid, sdd;      // kStandard only
+  const double* table;  // kTable: [n_views][12] = S, D0, U, V
+  const double* angles; // kStandard: degrees
+  const float* pvec;    // kBeijing: reference projection vectors [n_views][12]
+};
+
+// Standard circular orbit (forwardKernel.cu:30-40, world unit = one pixel,
+// voxel = SID/SDD world units, p = world / (SID/SDD) + N/2).
+DTV_HD inline void view_standard(double angle, double sid, double sdd, int nx, int ny,
+                                 int nz, int nu, int nv, View& v) {
+#pragma clang fp contract(off)
+  const double s = sid / sdd;
+  const double th = angle / 180.0 * kPiRef - kPiRef / 2.0;
+  const double c = cos(th), sn = sin(th);
+  v.S[0] = -c * sid / s + nx / 2.0;
+  v.S[1] = -sn * sid / s + ny / 2.0;
+  v.S[2] = nz / 2.0;
+  v.D0[0] = v.S[0] + (c * sdd + sn * (nu / 2.0)) / s;
+  v.D0[1] = v.S[1] + (sn * sdd - c * (nu / 2.0)) / s;
+  v.D0[2] = v.S[2] + (-nv / 2.0) / s;
+  v.U[0] = -sn / s;
+  v.U[1] = c / s;
+  v.U[2] = 0.0;
+  v.V[0] = 0.0;
+  v.V[1] = 0.0;
+  v.V[2] = 1.0 / s;
+}
+
+// Beijing projection vector (forwardKernel.cu:27-33): source (0..2) and
+// detector centre (3..5) with z negated, v (6..8), u (9..11), pixel centres
+// at +0.5, voxel frame p = kernel frame + N/2.
+DTV_HD inline void view_beijing(const float* p, int nx, int ny, int nz, int nu, int nv,
+                                View& v) {
+#pragma clang fp contract(off)
+  const double hx = nx / 2.0, hy = ny / 2.0, hz = nz / 2.0;
+  v.S[0] = (double)p[0] + hx;
+  v.S[1] = (double)p[1] + hy;
+  v.S[2] = -(double)p[2] + hz;
+  const double dc0 = (double)p[3] + hx;
+  const double dc1 = (double)p[4] + hy;
+  const double dc2 = -(double)p[5] + hz;
+  const double au = 0.5 - nu / 2.0, av = 0.5 - nv / 2.0;
+  for (int c = 0; c < 3; ++c) {
+    v.U[c] = (double)p[9 + c];
+    v.V[c] = (double)p[6 + c];
+  }
+  v.D0[0] = dc0 + au * v.U[0] + av * v.V[0];
+  v.D0[1] = dc1 + au * v.U[1] + av * v.V[1];
+  v.D0[2] = dc2 + au * v.U[2] + av * v.V[2];
+}
+
+DTV_HD inline void load_view(const Geo& g, int a, View& v) {
+  if (g.kind == kStandard) {
+    view_standard(g.angles[a], g.sid, g.sdd, g.nx, g.ny, g.nz, g.nu, g.nv, v);
+  } else if (g.kind == kBeijing) {
+    view_beijing(g.pvec + 12 * (size_t)a, g.nx, g.ny, g.nz, g.nu, g.nv, v);
+  } else {
+    const double* t = g.table + 12 * (size_t)a;
+    for (int c = 0; c < 3; ++c) {
+      v.S[c] = t[c];
+      v.D0[c] = t[3 + c];
+      v.U[c] = t[6 + c];
+      v.V[c] = t[9 + c];
+    }
+  }
+}
+
+// One ray, ready to march: position of sample k is P + k*D in 32.32 fixed
+// point; samples k in [k0, k1) may touch the volume, every other sample of
+// the reference's range lies wholly outside the trilinear support.
+struct Ray {
+  int64_t P[3];
+  int64_t D[3];
+  int k0, k1;
+};
+
+DTV_HD inline void ray_dir(const View& v, int iu, int iv, double d[3]) {
+#pragma clang fp contract(off)
+  double r[3];
+  for (int c = 0; c < 3; ++c) {
+    const double pix = v.D0[c] + iu * v.U[c] + iv * v.V[c];
+    r[c] = pix - v.S[c];
+  }
+  const double len = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  d[0] = r[0] / len;
+  d[1] = r[1] / len;
+  d[2] = r[2] / len;
+}
+
+DTV_HD inline void ray_setup(const Geo& g, const View& v, int iu, int iv, Ray& ray) {
+#pragma clang fp contract(off)
+  double d[3];
+  ray_dir(v, iu, iv, d);
+  const double adx = fabs(d[0]), ady = fabs(d[1]), adz = fabs(d[2]);
+  int ax;
+  if (g.axis_mode == kAxisXY) {
+    ax = (adx >= ady) ? 0 : 1;
+  } else {
+    const double m = fmax(adx, fmax(adz, ady));
+    ax = (adx == m) ? 0 : ((ady == m) ? 1 : 2);
+  }
+  // select without dynamic indexing so the view stays in registers
+  const double dax = ax == 0 ? d[0] : (ax == 1 ? d[1] : d[2]);
+  const double sax = ax == 0 ? v.S[0] : (ax == 1 ? v.S[1] : v.S[2]);
+  const double lax = ax == 0 ? g.lo[0] : (ax == 1 ? g.lo[1] : g.lo[2]);
+  const double hax = ax == 0 ? g.hi[0] : (ax == 1 ? g.hi[1] : g.hi[2]);
+  const double a0 = (lax - sax) / dax;
+  const double a1 = (hax - sax) / dax;
+  const double bs = fmin(a0, a1) - g.margin;
+  const double be = fmax(a0, a1) + g.margin;
+  // samples k >= 0 with bs + k < be  ->  k in [0, kend)
+  // (ceil(be - bs) is within one of the exact count; non-finite or absurd
+  // ranges give no samples)
+  int kend = 0;
+  if (be - bs > 0.0 && be - bs < 1.0e7) {
+    kend = (int)ceil(be - bs);
+    if (kend > 0 && !(bs + (double)(kend - 1) < be)) --kend;
+    if (bs + (double)kend < be) ++kend;
+  }
+  // window where the sample's trilinear support can touch the volume:
+  // p in [-1, N) on every axis, widened by one sample at each end
+  const double n3[3] = {(double)g.nx, (double)g.ny, (double)g.nz};
+  double klo = 0.0, khi = (double)kend;
+  double P0[3];
+  for (int c = 0; c < 3; ++c) P0[c] = v.S[c] + bs * d[c];
+  for (int c = 0; c < 3; ++c) {
+    if (d[c] != 0.0) {
+      const double t0 = (-1.0 - P0[c]) / d[c];
+      const double t1 = (n3[c] - P0[c]) / d[c];
+      klo = fmax(klo, fmin(t0, t1) - 1.0);
+      khi = fmin(khi, fmax(t0, t1) + 2.0);
+    } else if (!(P0[c] >= -1.0 && P0[c] < n3[c])) {
+      khi = -1.0;
+    }
+  }
+  if (!(khi > klo)) {
+    ray.k0 = ray.k1 = 0;
+  } else {
+    ray.k0 = (int)floor(klo);
+    ray.k1 = (int)ceil(khi);
+    if (ray.k1 > kend) ray.k1 = kend;
+    if (ray.k0 < 0) ray.k0 = 0;
+    if (ray.k1 < ray.k0) ray.k1 = ray.k0;
+  }
+  for (int c = 0; c < 3; ++c) {
+    ray.P[c] = llrint(P0[c] * kFix);
+    ray.D[c] = llrint(d[c] * kFix);
+  }
+}
+
+}  // namespace dtv

@@ -1,0 +1,641 @@
+// gfx950 kernels of the cone-beam projector pair and their C ABI.
+//
+//   k_fwd_gather    A: one lane per detector ray, explicit 8-tap FP32
+//                   trilinear gathers from HBM (no texture unit), 32.32
+//                   fixed-point sample positions, clipped to the samples
+//                   whose support touches the volume.
+//   k_adj_scatter   matched A^T: the same enumeration, each sample's 8
+//                   weights scattered with hardware FP32 atomics.
+//   k_bp_standard   reference backprojector of Standard/plug/kernel/
+//                   backwardKernel.cu:22-53 (voxel-driven, bilinear, FDK
+//                   weight), all views accumulated in registers.
+//   k_bp_beijing    reference backprojector of Beijing/plug/kernel/
+//                   backwardKernel.cu:18-45.
+#include <hip/hip_runtime.h>
+
+#include "dtv_geom.h"
+#include "dtvproj.h"
+
+using namespace dtv;
+
+namespace {
+
+constexpr int kRayTileU = 64;   // rays per wave along u
+constexpr int kRayTileV = 4;    // waves per block along v
+
+// Trilinear sample at the fixed-point position (px, py, pz); taps outside
+// the volume read as zero (border addressing of the reference texture).
+__device__ __forceinline__ float tri_checked(const float* __restrict__ vol, int nx, int ny,
+                                             int nz, int64_t px, int64_t py, int64_t pz) {
+  const int ix = (int)(px >> 32), iy = (int)(py >> 32), iz = (int)(pz >> 32);
+  const float fx = (float)(uint32_t)px * kInvFix;
+  const float fy = (float)(uint32_t)py * kInvFix;
+  const float fz = (float)(uint32_t)pz * kInvFix;
+  const bool x0 = (unsigned)ix < (unsigned)nx, x1 = (unsigned)(ix + 1) < (unsigned)nx;
+  const bool y0 = (unsigned)iy < (unsigned)ny, y1 = (unsigned)(iy + 1) < (unsigned)ny;
+  const bool z0 = (unsigned)iz < (unsigned)nz, z1 = (unsigned)(iz + 1) < (unsigned)nz;
+  const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+  const size_t base = (size_t)iz * sz + (size_t)iy * sy + (size_t)ix;
+  float t[8];
+  t[0] = (z0 && y0 && x0) ? vol[base] : 0.f;
+  t[1] = (z0 && y0 && x1) ? vol[base + 1] : 0.f;
+  t[2] = (z0 && y1 && x0) ? vol[base + sy] : 0.f;
+  t[3] = (z0 && y1 && x1) ? vol[base + sy + 1] : 0.f;
+  t[4] = (z1 && y0 && x0) ? vol[base + sz] : 0.f;
+  t[5] = (z1 && y0 && x1) ? vol[base + sz + 1] : 0.f;
+  t[6] = (z1 && y1 && x0) ? vol[base + sz + sy] : 0.f;
+  t[7] = (z1 && y1 && x1) ? vol[base + sz + sy + 1] : 0.f;
+  const float a0 = t[0] + fx * (t[1] - t[0]);
+  const float a1 = t[2] + fx * (t[3] - t[2]);
+  const float a2 = t[4] + fx * (t[5] - t[4]);
+  const float a3 = t[6] + fx * (t[7] - t[6]);
+  const float b0 = a0 + fy * (a1 - a0);
+  const float b1 = a2 + fy * (a3 - a2);
+  return b0 + fz * (b1 - b0);
+}
+
+__global__ __launch_bounds__(kRayTileU * kRayTileV) void k_fwd_gather(
+    const float* __restrict__ vol, float* __restrict__ sino, Geo g, int batch) {
+  const int iu = blockIdx.x * kRayTileU + threadIdx.x;
+  const int iv = blockIdx.y * kRayTileV + threadIdx.y;
+  const int a = blockIdx.z;
+  if (iu >= g.nu || iv >= g.nv) return;
+  View v;
+  load_view(g, a, v);
+  Ray r;
+  ray_setup(g, v, iu, iv, r);
+  const size_t vstride = (size_t)g.nx * g.ny * g.nz;
+  const float scale = (float)g.scale;
+  for (int b = 0; b < batch; ++b) {
+    const float* vb = vol + (size_t)b * vstride;
+    float acc = 0.f;
+    int64_t px = r.P[0] + (int64_t)r.k0 * r.D[0];
+    int64_t py = r.P[1] + (int64_t)r.k0 * r.D[1];
+    int64_t pz = r.P[2] + (int64_t)r.k0 * r.D[2];
+    for (int k = r.k0; k < r.k1; ++k) {
+      acc += tri_checked(vb, g.nx, g.ny, g.nz, px, py, pz);
+      px += r.D[0];
+      py += r.D[1];
+      pz += r.D[2];
+    }
+    sino[(((size_t)b * g.n_views + a) * g.nv + iv) * g.nu + iu] = acc * scale;
+  }
+}
+
+__device__ __forceinline__ void scatter_tri(float* __restrict__ vol, int nx, int ny, int nz,
+                                            int64_t px, int64_t py, int64_t pz, float val) {
+  const int ix = (int)(px >> 32), iy = (int)(py >> 32), iz = (int)(pz >> 32);
+  const float fx = (float)(uint32_t)px * kInvFix;
+  const float fy = (float)(uint32_t)py * kInvFix;
+  const float fz = (float)(uint32_t)pz * kInvFix;
+  const bool x0 = (unsigned)ix < (unsigned)nx, x1 = (unsigned)(ix + 1) < (unsigned)nx;
+  const bool y0 = (unsigned)iy < (unsigned)ny, y1 = (unsigned)(iy + 1) < (unsigned)ny;
+  const bool z0 = (unsigned)iz < (unsigned)nz, z1 = (unsigned)(iz + 1) < (unsigned)nz;
+  const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+  const size_t base = (size_t)iz * sz + (size_t)iy * sy + (size_t)ix;
+  const float wz0 = (1.f - fz) * val, wz1 = fz * val;
+  const float w00 = (1.f - fy) * wz0, w01 = fy * wz0, w10 = (1.f - fy) * wz1, w11 = fy * wz1;
+  const float gx = 1.f - fx;
+  if (z0 && y0 && x0) unsafeAtomicAdd(vol + base, gx * w00);
+  if (z0 && y0 && x1) unsafeAtomicAdd(vol + base + 1, fx * w00);
+  if (z0 && y1 && x0) unsafeAtomicAdd(vol + base + sy, gx * w01);
+  if (z0 && y1 && x1) unsafeAtomicAdd(vol + base + sy + 1, fx * w01);
+  if (z1 && y0 && x0) unsafeAtomicAdd(vol + base + sz, gx * w10);
+  if (z1 && y0 && x1) unsafeAtomicAdd(vol + base + sz + 1, fx * w10);
+  if (z1 && y1 && x0) unsafeAtomicAdd(vol + base + sz + sy, gx * w11);
+  if (z1 && y1 && x1) unsafeAtomicAdd(vol + base + sz + sy + 1, fx * w11);
+}
+
+__global__ __launch_bounds__(kRayTileU * kRayTileV) void k_adj_scatter(
+    const float* __restrict__ sino, float* __restrict__ vol, Geo g, int batch) {
+  const int iu = blockIdx.x * kRayTileU + threadIdx.x;
+  const int iv = blockIdx.y * kRayTileV + threadIdx.y;
+  const int a = blockIdx.z;
+  if (iu >= g.nu || iv >= g.nv) return;
+  View v;
+  load_view(g, a, v);
+  Ray r;
+  ray_setup(g, v, iu, iv, r);
+  const size_t vstride = (size_t)g.nx * g.ny * g.nz;
+  const float scale = (float)g.scale;
+  for (int b = 0; b < batch; ++b) {
+    const float val = sino[(((size_t)b * g.n_views + a) * g.nv + iv) * g.nu + iu] * scale;
+    if (val == 0.f) continue;
+    float* vb = vol + (size_t)b * vstride;
+    int64_t px = r.P[0] + (int64_t)r.k0 * r.D[0];
+    int64_t py = r.P[1] + (int64_t)r.k0 * r.D[1];
+    int64_t pz = r.P[2] + (int64_t)r.k0 * r.D[2];
+    for (int k = r.k0; k < r.k1; ++k) {
+      scatter_tri(vb, g.nx, g.ny, g.nz, px, py, pz, val);
+      px += r.D[0];
+      py += r.D[1];
+      pz += r.D[2];
+    }
+  }
+}
+
+__global__ void k_fill_zero(float* __restrict__ p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0.f;
+}
+
+// Bilinear sample of one detector plane at a texel-centred coordinate given
+// as integer part + fraction; zero outside (border addressing).
+__device__ __forceinline__ float bil_checked(const float* __restrict__ img, int nu, int nv,
+                                             int ix, float fx, int iy, float fy) {
+  const bool x0 = (unsigned)ix < (unsigned)nu, x1 = (unsigned)(ix + 1) < (unsigned)nu;
+  const bool y0 = (unsigned)iy < (unsigned)nv, y1 = (unsigned)(iy + 1) < (unsigned)nv;
+  const size_t base = (size_t)iy * nu + ix;
+  const float t00 = (y0 && x0) ? img[base] : 0.f;
+  const float t01 = (y0 && x1) ? img[base + 1] : 0.f;
+  const float t10 = (y1 && x0) ? img[base + nu] : 0.f;
+  const float t11 = (y1 && x1) ? img[base + nu + 1] : 0.f;
+  const float a0 = t00 + fx * (t01 - t00);
+  const float a1 = t10 + fx * (t11 - t10);
+  return a0 + fy * (a1 - a0);
+}
+
+constexpr int kBpTileX = 64;
+constexpr int kBpZ = 16;          // voxels along z per lane
+constexpr int kBpViewChunk = 64;  // views staged in LDS at a time
+
+// Standard backwardKernel.cu:22-53.  For voxel centre c = ((ix-X/2)s, (iy-Y/2)s)
+// and view a, the source-through-voxel line meets the detector line at
+// u = SDD<c,ey>/L + U/2 with L = SID + <c,ex> (the closed form of
+// helper_geometry.h:26-42's intersection), v_k = (k - Z/2) SDD s/L + V/2,
+// weight (SDD/L)^2 pi/A.
+__global__ __launch_bounds__(kBpTileX) void k_bp_standard(
+    const float* __restrict__ sino, float* __restrict__ vol, Geo g, int b) {
+  __shared__ double cs_[kBpViewChunk][2];
+  const int ix = blockIdx.x * kBpTileX + threadIdx.x;
+  const int iy = blockIdx.y;
+  const int z0 = blockIdx.z * kBpZ;
+  const double s = g.sid / g.sdd;
+  const double cx = (ix - g.nx / 2.0) * s;
+  const double cy = (iy - g.ny / 2.0) * s;
+  const float wpi = (float)(kPiRef / g.n_views);
+  float acc[kBpZ];
+#pragma unroll
+  for (int k = 0; k < kBpZ; ++k) acc[k] = 0.f;
+  const float* sb = sino + (size_t)b * g.n_views * g.nu * g.nv;
+  for (int a0 = 0; a0 < g.n_views; a0 += kBpViewChunk) {
+    const int na = min(kBpViewChunk, g.n_views - a0);
+    __syncthreads();
+    if (threadIdx.x < na) {
+      const double th = g.angles[a0 + threadIdx.x] / 180.0 * kPiRef - kPiRef / 2.0;
+      cs_[threadIdx.x][0] = cos(th);
+      cs_[threadIdx.x][1] = sin(th);
+    }
+    __syncthreads();
+    if (ix < g.nx) {
+      for (int j = 0; j < na; ++j) {
+        const double c = cs_[j][0], sn = cs_[j][1];
+        const double L = g.sid + (cx * c + cy * sn);
+        const double m = g.sdd / L;
+        const double u = m * (cx * -sn + cy * c) + g.nu / 2.0;
+        const double dz = m * s;
+        const double fu = floor(u);
+        const int iu = (int)fu;
+        const float fx = (float)(u - fu);
+        const float w = (float)(m * m) * wpi;
+        const float* img = sb + (size_t)(a0 + j) * g.nu * g.nv;
+        double vv = (z0 - g.nz / 2.0) * dz + g.nv / 2.0;
+#pragma unroll
+        for (int k = 0; k < kBpZ; ++k) {
+          const double fv = floor(vv);
+          acc[k] += w * bil_checked(img, g.nu, g.nv, iu, fx, (int)fv, (float)(vv - fv));
+          vv += dz;
+        }
+      }
+    }
+  }
+  if (ix >= g.nx) return;
+  float* vb = vol + (size_t)b * g.nx * g.ny * g.nz;
+#pragma unroll
+  for (int k = 0; k < kBpZ; ++k) {
+    const int iz = z0 + k;
+    if (iz < g.nz) vb[((size_t)iz * g.ny + iy) * g.nx + ix] = acc[k];
+  }
+}
+
+// Beijing backwardKernel.cu:18-45 with the triple products expanded per view:
+// den = (S-c).(u x v), sx = ((c-S).(v x (S-D)))/den + U/2,
+// sy = ((c-S).((S-D) x u))/den + V/2, fr = ((S-D).(u x v))/den; views whose
+// (sx, sy) fall outside [-1, X+1] x [-1, Y+1] are skipped (the reference
+// tests against the volume size); bilinear at (sx - 0.5, sy - 0.5).
+struct BjView {
+  double S[3], n[3], m1[3], m2[3], fr;
+};
+
+__global__ __launch_bounds__(kBpTileX) void k_bp_beijing(
+    const float* __restrict__ sino, float* __restrict__ vol, Geo g, int b) {
+  __shared__ BjView vw[kBpViewChunk];
+  const int ix = blockIdx.x * kBpTileX + threadIdx.x;
+  const int iy = blockIdx.y;
+  const int z0 = blockIdx.z * kBpZ;
+  float acc[kBpZ];
+#pragma unroll
+  for (int k = 0; k < kBpZ; ++k) acc[k] = 0.f;
+  const float* sb = sino + (size_t)b * g.n_views * g.nu * g.nv;
+  const double cx = ix - g.nx / 2.0, cy = iy - g.ny / 2.0;
+  const double xlim = g.nx + 1.0, ylim = g.ny + 1.0;
+  for (int a0 = 0; a0 < g.n_views; a0 += kBpViewChunk) {
+    const int na = min(kBpViewChunk, g.n_views - a0);
+    __syncthreads();
+    if (threadIdx.x < na) {
+      const float* p = g.pvec + 12 * (size_t)(a0 + threadIdx.x);
+      const double S[3] = {(double)p[0], (double)p[1], -(double)p[2]};
+      const double D[3] = {(double)p[3], (double)p[4], -(double)p[5]};
+      const double vv[3] = {(double)p[6], (double)p[7], (double)p[8]};
+      const double uu[3] = {(double)p[9], (double)p[10], (double)p[11]};
+      const double sd[3] = {S[0] - D[0], S[1] - D[1], S[2] - D[2]};
+      BjView& w = vw[threadIdx.x];
+      w.n[0] = uu[1] * vv[2] - uu[2] * vv[1];
+      w.n[1] = uu[2] * vv[0] - uu[0] * vv[2];
+      w.n[2] = uu[0] * vv[1] - uu[1] * vv[0];
+      w.m1[0] = vv[1] * sd[2] - vv[2] * sd[1];
+      w.m1[1] = vv[2] * sd[0] - vv[0] * sd[2];
+      w.m1[2] = vv[0] * sd[1] - vv[1] * sd[0];
+      w.m2[0] = sd[1] * uu[2] - sd[2] * uu[1];
+      w.m2[1] = sd[2] * uu[0] - sd[0] * uu[2];
+      w.m2[2] = sd[0] * uu[1] - sd[1] * uu[0];
+      w.fr = sd[0] * w.n[0] + sd[1] * w.n[1] + sd[2] * w.n[2];
+      for (int c = 0; c < 3; ++c) w.S[c] = S[c];
+    }
+    __syncthreads();
+    if (ix < g.nx) {
+      for (int j = 0; j < na; ++j) {
+        const BjView& w = vw[j];
+        const double csx = cx - w.S[0], csy = cy - w.S[1];
+        double csz = (z0 - g.nz / 2.0) - w.S[2];
+        const float* img = sb + (size_t)(a0 + j) * g.nu * g.nv;
+#pragma unroll
+        for (int k = 0; k < kBpZ; ++k) {
+          const double den = -(csx * w.n[0] + csy * w.n[1] + csz * w.n[2]);
+          const double f = 1.0 / den;
+          const double sx = f * (csx * w.m1[0] + csy * w.m1[1] + csz * w.m1[2]) + g.nu / 2.0;
+          const double sy = f * (csx * w.m2[0] + csy * w.m2[1] + csz * w.m2[2]) + g.nv / 2.0;
+          if (!(sx < -1.0 || sx > xlim || sy < -1.0 || sy > ylim)) {
+            const double x = sx - 0.5, y = sy - 0.5;
+            const double fxd = floor(x), fyd = floor(y);
+            acc[k] += (float)(f * w.fr) *
+                      bil_checked(img, g.nu, g.nv, (int)fxd, (float)(x - fxd), (int)fyd,
+                                  (float)(y - fyd));
+          }
+          csz += 1.0;
+        }
+      }
+    }
+  }
+  if (ix >= g.nx) return;
+  const float wgt = (float)(2.0 * kPiRef / (double)(g.n_views / 21));
+  float* vb = vol + (size_t)b * g.nx * g.ny * g.nz;
+#pragma unroll
+  for (int k = 0; k < kBpZ; ++k) {
+    const int iz = z0 + k;
+    if (iz < g.nz) vb[((size_t)iz * g.ny + iy) * g.nx + ix] = acc[k] * wgt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+int check_sizes(int32_t batch, const int32_t* vs, const int32_t* ds, int32_t n_views) {
+  if (!vs || !ds) return DTV_ERR_NULL_POINTER;
+  if (batch < 1 || n_views < 1) return DTV_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < 3; ++i)
+    if (vs[i] < 1) return DTV_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < 2; ++i)
+    if (ds[i] < 1) return DTV_ERR_INVALID_ARGUMENT;
+  const int64_t nvox = (int64_t)vs[0] * vs[1] * vs[2];
+  const int64_t nray = (int64_t)ds[0] * ds[1];
+  if (nvox >= (1ll << 31) || nray >= (1ll << 31)) return DTV_ERR_TOO_LARGE;
+  if (n_views > 65535) return DTV_ERR_TOO_LARGE;
+  if (vs[0] > (1 << 20) || vs[1] > (1 << 20) || vs[2] > (1 << 20)) return DTV_ERR_TOO_LARGE;
+  return DTV_OK;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  bool changed = false;
+  int status = DTV_OK;
+  explicit DeviceGuard(int32_t device) {
+    if (device < 0) return;
+    if (hipGetDevice(&prev) != hipSuccess) {
+      status = DTV_ERR_HIP;
+      return;
+    }
+    if (prev != device) {
+      if (hipSetDevice(device) != hipSuccess) {
+        status = DTV_ERR_HIP;
+        return;
+      }
+      changed = true;
+    }
+  }
+  ~DeviceGuard() {
+    if (changed) (void)hipSetDevice(prev);
+  }
+};
+
+Geo make_geo(const int32_t* vs, const int32_t* ds, int32_t n_views) {
+  Geo g{};
+  g.nx = vs[0];
+  g.ny = vs[1];
+  g.nz = vs[2];
+  g.nu = ds[0];
+  g.nv = ds[1];
+  g.n_views = n_views;
+  return g;
+}
+
+void scheme_standard(Geo& g, double sid, double sdd) {
+  const double s = sid / sdd;
+  g.axis_mode = kAxisXY;
+  const double n3[3] = {(double)g.nx, (double)g.ny, (double)g.nz};
+  for (int c = 0; c < 3; ++c) {
+    const double half = n3[c] / 2.0 / s;
+    g.lo[c] = n3[c] / 2.0 - half;
+    g.hi[c] = n3[c] / 2.0 + half;
+  }
+  g.margin = 3.0 / s;
+  g.scale = s;
+  g.sid = sid;
+  g.sdd = sdd;
+}
+
+void scheme_beijing(Geo& g) {
+  g.axis_mode = kAxisXYZ;
+  g.lo[0] = g.lo[1] = g.lo[2] = 0.0;
+  g.hi[0] = g.nx;
+  g.hi[1] = g.ny;
+  g.hi[2] = g.nz;
+  g.margin = 3.0;
+  g.scale = 1.0;
+}
+
+int launch_status() {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? DTV_OK : DTV_ERR_HIP;
+}
+
+int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st) {
+  const dim3 block(kRayTileU, kRayTileV);
+  const dim3 grid((g.nu + kRayTileU - 1) / kRayTileU, (g.nv + kRayTileV - 1) / kRayTileV,
+                  g.n_views);
+  hipLaunchKernelGGL(k_fwd_gather, grid, block, 0, st, vol, sino, g, (int)batch);
+  return launch_status();
+}
+
+int run_zero(float* p, size_t n, hipStream_t st) {
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, st, p, n);
+  return launch_status();
+}
+
+int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st) {
+  int rc = run_zero(vol, (size_t)batch * g.nx * g.ny * g.nz, st);
+  if (rc) return rc;
+  const dim3 block(kRayTileU, kRayTileV);
+  const dim3 grid((g.nu + kRayTileU - 1) / kRayTileU, (g.nv + kRayTileV - 1) / kRayTileV,
+                  g.n_views);
+  hipLaunchKernelGGL(k_adj_scatter, grid, block, 0, st, sino, vol, g, (int)batch);
+  return launch_status();
+}
+
+}  // namespace
+
+extern "C" {
+
+int dtv_standard_forward(const float* volume, float* sino, int32_t batch, const double* angles,
+                         int32_t n_angles, const int32_t volume_size[3],
+                         const int32_t detector_size[2], double sid, double sdd,
+                         int32_t device, void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_angles);
+  if (rc) return rc;
+  if (!volume || !sino || !angles) return DTV_ERR_NULL_POINTER;
+  if (!(sid > 0.0) || !(sdd > sid)) return DTV_ERR_INVALID_ARGUMENT;
+  DeviceGuard dg(device);
+  if (dg.status) return dg.status;
+  Geo g = make_geo(volume_size, detector_size, n_angles);
+  scheme_standard(g, sid, sdd);
+  g.kind = kStandard;
+  g.angles = angles;
+  return run_forward(volume, sino, batch, g, (hipStream_t)stream);
+}
+
+int dtv_standard_adjoint(const float* sino, float* volume, int32_t batch, const double* angles,
+                         int32_t n_angles, const int32_t volume_size[3],
+                         const int32_t detector_size[2], double sid, double sdd,
+                         int32_t device, void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_angles);
+  if (rc) return rc;
+  if (!volume || !sino || !angles) return DTV_ERR_NULL_POINTER;
+  if (!(sid > 0.0) || !(sdd > sid)) return DTV_ERR_INVALID_ARGUMENT;
+  DeviceGuard dg(device);
+  if (dg.status) return dg.status;
+  Geo g = make_geo(volume_size, detector_size, n_angles);
+  scheme_standard(g, sid, sdd);
+  g.kind = kStandard;
+  g.angles = angles;
+  return run_adjoint(sino, volume, batch, g, (hipStream_t)stream);
+}
+
+int dtv_standard_backward(const float* sino, float* volume, int32_t batch, const double* angles,
+                          int32_t n_angles, const int32_t volume_size[3],
+                          const int32_t detector_size[2], double sid, double sdd,
+                          int32_t device, void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_angles);
+  if (rc) return rc;
+  if (!volume || !sino || !angles) return DTV_ERR_NULL_POINTER;
+  if (!(sid > 0.0) || !(sdd > sid)) return DTV_ERR_INVALID_ARGUMENT;
+  DeviceGuard dg(device);
+  if (dg.status) return dg.status;
+  Geo g = make_geo(volume_size, detector_size, n_angles);
+  scheme_standard(g, sid, sdd);
+  g.kind = kStandard;
+  g.angles = angles;
+  const dim3 grid((g.nx + kBpTileX - 1) / kBpTileX, g.ny, (g.nz + kBpZ - 1) / kBpZ);
+  for (int b = 0; b < batch; ++b) {
+    hipLaunchKernelGGL(k_bp_standard, grid, dim3(kBpTileX), 0, (hipStream_t)stream, sino,
+                       volume, g, b);
+    rc = launch_status();
+    if (rc) return rc;
+  }
+  return DTV_OK;
+}
+
+int dtv_beijing_forward(const float* volume, float* sino, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const float* project_vector, int32_t n_views, int32_t device,
+                        void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!volume || !sino || !project_vector) return DTV_ERR_NULL_POINTER;
+  DeviceGuard dg(device);
+  if (dg.status) return dg.status;
+  Geo g = make_geo(volume_size, detector_size, n_views);
+  scheme_beijing(g);
+  g.kind = kBeijing;
+  g.pvec = project_vector;
+  return run_forward(volume, sino, batch, g, (hipStream_t)stream);
+}
+
+int dtv_beijing_adjoint(const float* sino, float* volume, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const float* project_vector, int32_t n_views, int32_t device,
+                        void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!volume || !sino || !project_vector) return DTV_ERR_NULL_POINTER;
+  DeviceGuard dg(device);
+  if (dg.status) return dg.status;
+  Geo g = make_geo(volume_size, detector_size, n_views);
+  scheme_beijing(g);
+  g.kind = kBeijing;
+  g.pvec = project_vector;
+  return run_adjoint(sino, volume, batch, g, (hipStream_t)stream);
+}
+
+int dtv_beijing_backward(const float* sino, float* volume, int32_t batch,
+                         const int32_t volume_size[3], const int32_t detector_size[2],
+                         const float* project_vector, int32_t n_views, int32_t device,
+                         void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!volume || !sino || !project_vector) return DTV_ERR_NULL_POINTER;
+  if (n_views % 21 != 0) return DTV_ERR_VIEW_COUNT;
+  DeviceGuard dg(device);
+  if (dg.status) return dg.status;
+  Geo g = make_geo(volume_size, detector_size, n_views);
+  scheme_beijing(g);
+  g.kind = kBeijing;
+  g.pvec = project_vector;
+  const dim3 grid((g.nx + kBpTileX - 1) / kBpTileX, g.ny, (g.nz + kBpZ - 1) / kBpZ);
+  for (int b = 0; b < batch; ++b) {
+    hipLaunchKernelGGL(k_bp_beijing, grid, dim3(kBpTileX), 0, (hipStream_t)stream, sino,
+                       volume, g, b);
+    rc = launch_status();
+    if (rc) return rc;
+  }
+  return DTV_OK;
+}
+
+static int fill_table_geo(Geo& g, const int32_t* vs, const int32_t* ds, const double* views,
+                          int32_t n_views, const dtv_scheme* sc) {
+  if (!views || !sc) return DTV_ERR_NULL_POINTER;
+  if (sc->axis_mode != kAxisXY && sc->axis_mode != kAxisXYZ) return DTV_ERR_INVALID_ARGUMENT;
+  g = make_geo(vs, ds, n_views);
+  g.axis_mode = sc->axis_mode;
+  for (int c = 0; c < 3; ++c) {
+    g.lo[c] = sc->box_lo[c];
+    g.hi[c] = sc->box_hi[c];
+  }
+  g.margin = sc->margin;
+  g.scale = sc->scale;
+  g.kind = kTable;
+  g.table = views;
+  return DTV_OK;
+}
+
+int dtv_cone_forward(const float* volume, float* sino, int32_t batch,
+                     const int32_t volume_size[3], const int32_t detector_size[2],
+                     const double* views, int32_t n_views, const dtv_scheme* scheme,
+                     void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!volume || !sino) return DTV_ERR_NULL_POINTER;
+  Geo g;
+  rc = fill_table_geo(g, volume_size, detector_size, views, n_views, scheme);
+  if (rc) return rc;
+  return run_forward(volume, sino, batch, g, (hipStream_t)stream);
+}
+
+int dtv_cone_adjoint(const float* sino, float* volume, int32_t batch,
+                     const int32_t volume_size[3], const int32_t detector_size[2],
+                     const double* views, int32_t n_views, const dtv_scheme* scheme,
+                     void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!volume || !sino) return DTV_ERR_NULL_POINTER;
+  Geo g;
+  rc = fill_table_geo(g, volume_size, detector_size, views, n_views, scheme);
+  if (rc) return rc;
+  return run_adjoint(sino, volume, batch, g, (hipStream_t)stream);
+}
+
+static void export_scheme(const Geo& g, dtv_scheme* out) {
+  out->axis_mode = g.axis_mode;
+  out->reserved = 0;
+  for (int c = 0; c < 3; ++c) {
+    out->box_lo[c] = g.lo[c];
+    out->box_hi[c] = g.hi[c];
+  }
+  out->margin = g.margin;
+  out->scale = g.scale;
+}
+
+int dtv_views_standard(const double* angles, int32_t n_angles, const int32_t volume_size[3],
+                       const int32_t detector_size[2], double sid, double sdd,
+                       double* views_out, dtv_scheme* scheme_out) {
+  int rc = check_sizes(1, volume_size, detector_size, n_angles);
+  if (rc) return rc;
+  if (!angles || !views_out || !scheme_out) return DTV_ERR_NULL_POINTER;
+  if (!(sid > 0.0) || !(sdd > sid)) return DTV_ERR_INVALID_ARGUMENT;
+  Geo g = make_geo(volume_size, detector_size, n_angles);
+  scheme_standard(g, sid, sdd);
+  for (int a = 0; a < n_angles; ++a) {
+    View v;
+    view_standard(angles[a], sid, sdd, g.nx, g.ny, g.nz, g.nu, g.nv, v);
+    double* o = views_out + 12 * (size_t)a;
+    for (int c = 0; c < 3; ++c) {
+      o[c] = v.S[c];
+      o[3 + c] = v.D0[c];
+      o[6 + c] = v.U[c];
+      o[9 + c] = v.V[c];
+    }
+  }
+  export_scheme(g, scheme_out);
+  return DTV_OK;
+}
+
+int dtv_views_beijing(const float* project_vector, int32_t n_views, const int32_t volume_size[3],
+                      const int32_t detector_size[2], double* views_out,
+                      dtv_scheme* scheme_out) {
+  int rc = check_sizes(1, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!project_vector || !views_out || !scheme_out) return DTV_ERR_NULL_POINTER;
+  Geo g = make_geo(volume_size, detector_size, n_views);
+  scheme_beijing(g);
+  for (int a = 0; a < n_views; ++a) {
+    View v;
+    view_beijing(project_vector + 12 * (size_t)a, g.nx, g.ny, g.nz, g.nu, g.nv, v);
+    double* o = views_out + 12 * (size_t)a;
+    for (int c = 0; c < 3; ++c) {
+      o[c] = v.S[c];
+      o[3 + c] = v.D0[c];
+      o[6 + c] = v.U[c];
+      o[9 + c] = v.V[c];
+    }
+  }
+  export_scheme(g, scheme_out);
+  return DTV_OK;
+}
+
+const char* dtv_strerror(int code) {
+  switch (code) {
+    case DTV_OK: return "success";
+    case DTV_ERR_INVALID_ARGUMENT: return "invalid argument (sizes, batch, view count or geometry)";
+    case DTV_ERR_NULL_POINTER: return "null pointer argument";
+    case DTV_ERR_TOO_LARGE: return "problem too large for 32-bit per-volume indexing";
+    case DTV_ERR_HIP: return "HIP runtime error (launch or device selection failed)";
+    case DTV_ERR_VIEW_COUNT: return "view count must be a multiple of 21 for the Beijing backprojector";
+    default: return "unknown dtvproj error";
+  }
+}
+
+const char* dtv_version(void) { return "dtvproj 0.1.0 gfx950"; }
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+"""ctypes binding of ``libdtvproj.so`` (declared in ``include/dtvproj.h``).
+
+The library is loaded lazily on first use.  There is no fallback: if the
+library is missing or a GPU entry point fails, a RuntimeError is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+from .build import LIB_PATH
+
+_lock = threading.Lock()
+_lib = None
+
+c_int32 = ctypes.c_int32
+c_double = ctypes.c_double
+c_void_p = ctypes.c_void_p
+c_int32_p = ctypes.POINTER(ctypes.c_int32)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class DtvScheme(ctypes.Structure):
+    _fields_ = [
+        ("axis_mode", c_int32),
+        ("reserved", c_int32),
+        ("box_lo", c_double * 3),
+        ("box_hi", c_double * 3),
+        ("margin", c_double),
+        ("scale", c_double),
+    ]
+
+
+# name -> argtypes; every entry point returns int status
+_SIGS = {
+    "dtv_standard_forward": [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_int32_p, c_int32_p,
+                             c_double, c_double, c_int32, c_void_p],
+    "dtv_standard_backward": [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_int32_p, c_int32_p,
+                              c_double, c_double, c_int32, c_void_p],
+    "dtv_standard_adjoint": [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_int32_p, c_int32_p,
+                             c_double, c_double, c_int32, c_void_p],
+    "dtv_beijing_forward": [c_void_p, c_void_p, c_int32, c_int32_p, c_int32_p, c_void_p, c_int32,
+                            c_int32, c_void_p],
+    "dtv_beijing_backward": [c_void_p, c_void_p, c_int32, c_int32_p, c_int32_p, c_void_p, c_int32,
+                             c_int32, c_void_p],
+    "dtv_beijing_adjoint": [c_void_p, c_void_p, c_int32, c_int32_p, c_int32_p, c_void_p, c_int32,
+                            c_int32, c_void_p],
+    "dtv_cone_forward": [c_void_p, c_void_p, c_int32, c_int32_p, c_int32_p, c_void_p, c_int32,
+                         ctypes.POINTER(DtvScheme), c_void_p],
+    "dtv_cone_adjoint": [c_void_p, c_void_p, c_int32, c_int32_p, c_int32_p, c_void_p, c_int32,
+                         ctypes.POINTER(DtvScheme), c_void_p],
+    "dtv_views_standard": [c_double_p, c_int32, c_int32_p, c_int32_p, c_double, c_double,
+                           c_double_p, ctypes.POINTER(DtvScheme)],
+    "dtv_views_beijing": [c_float_p, c_int32, c_int32_p, c_int32_p, c_double_p,
+                          ctypes.POINTER(DtvScheme)],
+}
+
+EXPORTED = tuple(_SIGS) + ("dtv_strerror", "dtv_version")
+
+
+def lib():
+    """The loaded library (raises RuntimeError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"dtvproj HIP library not found at {LIB_PATH}; build it with "
+                "`python -m dtvproj.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, argtypes in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = argtypes
+            fn.restype = c_int32
+        L.dtv_strerror.argtypes = [c_int32]
+        L.dtv_strerror.restype = ctypes.c_char_p
+        L.dtv_version.argtypes = []
+        L.dtv_version.restype = ctypes.c_char_p
+        _lib = L
+        return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().dtv_strerror(rc).decode()
+        raise RuntimeError(f"{what} failed: {msg} (code {rc})")
+
+
+def int3(vals):
+    return (c_int32 * 3)(*[int(v) for v in vals])
+
+
+def int2(vals):
+    return (c_int32 * 2)(*[int(v) for v in vals])
+
+
+def scheme_struct(scheme) -> DtvScheme:
+    s = DtvScheme()
+    s.axis_mode = int(scheme.axis_mode)
+    s.reserved = 0
+    for i in range(3):
+        s.box_lo[i] = float(scheme.box_lo[i])
+        s.box_hi[i] = float(scheme.box_hi[i])
+    s.margin = float(scheme.margin)
+    s.scale = float(scheme.scale)
+    return s
+
+
+def version() -> str:
+    return lib().dtv_version().decode()

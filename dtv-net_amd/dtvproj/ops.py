@@ -1,0 +1,155 @@
+"""Tensor-level projector calls through the C ABI.
+
+Inputs are device tensors (ROCm ``torch.device('cuda', i)``), FP32 and
+C-contiguous; outputs are allocated here with ``torch.empty`` on the input's
+device and every element is written by the kernels.  Work is enqueued on the
+current torch stream of that device.  There is no CPU path: a CPU tensor or
+a missing library raises RuntimeError.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _capi
+from .geometry import ConeGeometry, KIND_BEIJING, KIND_STANDARD
+
+
+def _stream(device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def check_input(t: torch.Tensor, name: str) -> None:
+    """CHECK_CUDA / CHECK_CONTIGUOUS of the reference (forwardKernel.cu:15-17)."""
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+
+
+def _check_f32(t: torch.Tensor, name: str) -> None:
+    check_input(t, name)
+    if t.dtype != torch.float32:
+        raise RuntimeError(f"{name} must be a float32 tensor, got {t.dtype}")
+
+
+def _batch_of(t: torch.Tensor, per_item: int, name: str) -> int:
+    if t.dim() < 1 or t.shape[0] < 1:
+        raise RuntimeError(f"{name} must have a leading batch dimension")
+    b = int(t.shape[0])
+    if t.numel() != b * per_item:
+        raise RuntimeError(
+            f"{name} has {t.numel()} elements, expected batch {b} x {per_item}")
+    return b
+
+
+def _vol_numel(geom: ConeGeometry) -> int:
+    X, Y, Z = geom.volume_size
+    return X * Y * Z
+
+
+def _sino_numel(geom: ConeGeometry) -> int:
+    U, V = geom.detector_size
+    return geom.n_views * U * V
+
+
+def forward(volume: torch.Tensor, geom: ConeGeometry) -> torch.Tensor:
+    """A: ray sums of ``volume`` [B,1,Z,Y,X] -> sinogram [B,1,A,V,U]."""
+    _check_f32(volume, "volume")
+    B = _batch_of(volume, _vol_numel(geom), "volume")
+    out = torch.empty(geom.sino_shape(B), dtype=torch.float32, device=volume.device)
+    views = geom.device_views(volume.device)
+    sc = _capi.scheme_struct(geom.scheme)
+    with torch.cuda.device(volume.device):
+        rc = _capi.lib().dtv_cone_forward(
+            volume.data_ptr(), out.data_ptr(), B, _capi.int3(geom.volume_size),
+            _capi.int2(geom.detector_size), views.data_ptr(), geom.n_views,
+            ctypes.byref(sc), _stream(volume.device))
+    _capi.check(rc, "dtv_cone_forward")
+    return out
+
+
+def adjoint(sino: torch.Tensor, geom: ConeGeometry) -> torch.Tensor:
+    """Matched A^T: exact transpose of :func:`forward`."""
+    _check_f32(sino, "sino")
+    B = _batch_of(sino, _sino_numel(geom), "sino")
+    out = torch.empty(geom.volume_shape(B), dtype=torch.float32, device=sino.device)
+    views = geom.device_views(sino.device)
+    sc = _capi.scheme_struct(geom.scheme)
+    with torch.cuda.device(sino.device):
+        rc = _capi.lib().dtv_cone_adjoint(
+            sino.data_ptr(), out.data_ptr(), B, _capi.int3(geom.volume_size),
+            _capi.int2(geom.detector_size), views.data_ptr(), geom.n_views,
+            ctypes.byref(sc), _stream(sino.device))
+    _capi.check(rc, "dtv_cone_adjoint")
+    return out
+
+
+def _ref_angles(geom: ConeGeometry, device):
+    key = ("angles", str(device))
+    t = geom._dev_cache.get(key)
+    if t is None:
+        t = torch.from_numpy(np.ascontiguousarray(geom.ref["angles"], dtype=np.float64)).to(device)
+        geom._dev_cache[key] = t
+    return t
+
+
+def _ref_vectors(geom: ConeGeometry, device):
+    key = ("vectors", str(device))
+    t = geom._dev_cache.get(key)
+    if t is None:
+        t = torch.from_numpy(np.ascontiguousarray(geom.ref["vectors"], dtype=np.float32)).to(device)
+        geom._dev_cache[key] = t
+    return t
+
+
+def bp_ref(sino: torch.Tensor, geom: ConeGeometry) -> torch.Tensor:
+    """Reference-mode backprojector (FDK-style weights, not the adjoint):
+    Standard backwardKernel.cu:22-53 or Beijing backwardKernel.cu:18-45."""
+    _check_f32(sino, "sino")
+    B = _batch_of(sino, _sino_numel(geom), "sino")
+    out = torch.empty(geom.volume_shape(B), dtype=torch.float32, device=sino.device)
+    dev = sino.device
+    with torch.cuda.device(dev):
+        if geom.kind == KIND_STANDARD:
+            rc = _capi.lib().dtv_standard_backward(
+                sino.data_ptr(), out.data_ptr(), B, _ref_angles(geom, dev).data_ptr(),
+                geom.n_views, _capi.int3(geom.volume_size), _capi.int2(geom.detector_size),
+                geom.ref["sid"], geom.ref["sdd"], -1, _stream(dev))
+            _capi.check(rc, "dtv_standard_backward")
+        elif geom.kind == KIND_BEIJING:
+            rc = _capi.lib().dtv_beijing_backward(
+                sino.data_ptr(), out.data_ptr(), B, _capi.int3(geom.volume_size),
+                _capi.int2(geom.detector_size), _ref_vectors(geom, dev).data_ptr(),
+                geom.n_views, -1, _stream(dev))
+            _capi.check(rc, "dtv_beijing_backward")
+        else:
+            raise RuntimeError("the reference backprojector exists only for the Standard "
+                               "and Beijing geometries; use adjoint() for table geometries")
+    return out
+
+
+def host_views_standard(angles, volume_size, detector_size, sid, sdd):
+    """Canonical view table computed by the library's host code (no GPU)."""
+    a = np.ascontiguousarray(angles, dtype=np.float64)
+    out = np.zeros((a.shape[0], 12), dtype=np.float64)
+    sc = _capi.DtvScheme()
+    rc = _capi.lib().dtv_views_standard(
+        a.ctypes.data_as(_capi.c_double_p), a.shape[0], _capi.int3(volume_size),
+        _capi.int2(detector_size), float(sid), float(sdd),
+        out.ctypes.data_as(_capi.c_double_p), ctypes.byref(sc))
+    _capi.check(rc, "dtv_views_standard")
+    return out, sc
+
+
+def host_views_beijing(project_vector, volume_size, detector_size):
+    P = np.ascontiguousarray(project_vector, dtype=np.float32)
+    out = np.zeros((P.shape[0], 12), dtype=np.float64)
+    sc = _capi.DtvScheme()
+    rc = _capi.lib().dtv_views_beijing(
+        P.ctypes.data_as(_capi.c_float_p), P.shape[0], _capi.int3(volume_size),
+        _capi.int2(detector_size), out.ctypes.data_as(_capi.c_double_p), ctypes.byref(sc))
+    _capi.check(rc, "dtv_views_beijing")
+    return out, sc

@@ -1,0 +1,135 @@
+/*
+ * dtvproj -- C ABI of the MI355X (gfx950) cone-beam projector pair.
+ *
+ * All array arguments are device pointers to C-contiguous FP32 data:
+ *   volume   [batch][Z][Y][X]        (x fastest; the reference's [B,1,Z,Y,X])
+ *   sinogram [batch][views][V][U]    (u fastest; the reference's [B,1,A,V,U])
+ * Outputs are caller-allocated and fully overwritten; the library keeps no
+ * pointer after a call returns and allocates no device memory.  Work is
+ * enqueued on ``stream`` (a hipStream_t; NULL = the null stream) and runs
+ * on the device that is current on the calling thread.  Every entry point
+ * returns 0 on success or a DTV_ERR_* code; dtv_strerror() describes it.
+ * Calls are reentrant: there is no global state.
+ */
+#ifndef DTVPROJ_H
+#define DTVPROJ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DTV_OK 0
+#define DTV_ERR_INVALID_ARGUMENT 1
+#define DTV_ERR_NULL_POINTER 2
+#define DTV_ERR_TOO_LARGE 3
+#define DTV_ERR_HIP 4
+#define DTV_ERR_VIEW_COUNT 5
+
+/* Sample phase of a canonical view table (see dtv_cone_forward). */
+typedef struct dtv_scheme {
+  int32_t axis_mode;   /* 2: dominant axis among x,y (|dx|>=|dy| -> x); 3: among x,y,z (ties x->y->z) */
+  int32_t reserved;
+  double box_lo[3];    /* box whose dominant-axis planes phase the samples (voxel frame) */
+  double box_hi[3];
+  double margin;       /* voxels added at both ends of the sample range */
+  double scale;        /* output multiplier */
+} dtv_scheme;
+
+/* ---- Reference-shaped entry points ------------------------------------ */
+
+/* Standard circular geometry, ray-driven forward projector A.
+ * Replaces JITStandardGeometry.forward,
+ * model/ConeBeamLayers/Standard/plug/include/forward.h:3 (kernel
+ * Standard/plug/kernel/forwardKernel.cu:22-143).  angles: device float64
+ * degrees [n_angles]; volume_size = {X, Y, Z}; detector_size = {U, V};
+ * device: HIP ordinal the pointers live on (-1 = current device). */
+int dtv_standard_forward(const float* volume, float* sino, int32_t batch,
+                         const double* angles, int32_t n_angles,
+                         const int32_t volume_size[3], const int32_t detector_size[2],
+                         double sid, double sdd, int32_t device, void* stream);
+
+/* Standard reference backprojector (voxel-driven, bilinear, FDK weights
+ * (SDD/L)^2*pi/A; NOT the adjoint of A).  Replaces
+ * JITStandardGeometry.backward, Standard/plug/include/backward.h:3 (kernel
+ * Standard/plug/kernel/backwardKernel.cu:22-110). */
+int dtv_standard_backward(const float* sino, float* volume, int32_t batch,
+                          const double* angles, int32_t n_angles,
+                          const int32_t volume_size[3], const int32_t detector_size[2],
+                          double sid, double sdd, int32_t device, void* stream);
+
+/* Standard matched adjoint: the exact transpose of dtv_standard_forward.
+ * No reference counterpart (the reference wires backward.h:3 as A's
+ * gradient, StandardGeometry.py:18-21); used for autograd and the
+ * adjoint tests. */
+int dtv_standard_adjoint(const float* sino, float* volume, int32_t batch,
+                         const double* angles, int32_t n_angles,
+                         const int32_t volume_size[3], const int32_t detector_size[2],
+                         double sid, double sdd, int32_t device, void* stream);
+
+/* Beijing vector geometry, forward projector A (without the Python-side
+ * sampleInterval factor).  Replaces JITBeijingGeometry.forward,
+ * model/ConeBeamLayers/Beijing/plug/include/forward.h:3 (kernel
+ * Beijing/plug/kernel/forwardKernel.cu:17-135).  project_vector: device
+ * float32 [n_views][12]. */
+int dtv_beijing_forward(const float* volume, float* sino, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const float* project_vector, int32_t n_views,
+                        int32_t device, void* stream);
+
+/* Beijing reference backprojector (triple-product projection, weight
+ * fr*2pi/(n_views/21), skip rule on volume dimensions).  Replaces
+ * JITBeijingGeometry.backward, Beijing/plug/include/backward.h:3 (kernel
+ * Beijing/plug/kernel/backwardKernel.cu:18-104).  n_views must be a
+ * multiple of 21 (the reference processes 21 views per launch). */
+int dtv_beijing_backward(const float* sino, float* volume, int32_t batch,
+                         const int32_t volume_size[3], const int32_t detector_size[2],
+                         const float* project_vector, int32_t n_views,
+                         int32_t device, void* stream);
+
+/* Beijing matched adjoint: exact transpose of dtv_beijing_forward.  No
+ * reference counterpart (BeijingGeometry.py:20-25 wires backward.h:3). */
+int dtv_beijing_adjoint(const float* sino, float* volume, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const float* project_vector, int32_t n_views,
+                        int32_t device, void* stream);
+
+/* ---- Canonical view-table entry points -------------------------------- */
+
+/* Forward projector for any geometry given as a device float64 view table
+ * [n_views][12] = S, D0, U, V in the voxel frame (voxel (ix,iy,iz) centred
+ * at the integer point).  Generalises both reference kernels
+ * (Standard forwardKernel.cu:22-86, Beijing forwardKernel.cu:17-76). */
+int dtv_cone_forward(const float* volume, float* sino, int32_t batch,
+                     const int32_t volume_size[3], const int32_t detector_size[2],
+                     const double* views, int32_t n_views, const dtv_scheme* scheme,
+                     void* stream);
+
+/* Exact transpose of dtv_cone_forward. */
+int dtv_cone_adjoint(const float* sino, float* volume, int32_t batch,
+                     const int32_t volume_size[3], const int32_t detector_size[2],
+                     const double* views, int32_t n_views, const dtv_scheme* scheme,
+                     void* stream);
+
+/* ---- Host-only helpers (no GPU work) ---------------------------------- */
+
+/* Canonical view table and scheme of the Standard geometry, computed on the
+ * host with the same code the kernels run (views_out: host [n][12]). */
+int dtv_views_standard(const double* angles, int32_t n_angles,
+                       const int32_t volume_size[3], const int32_t detector_size[2],
+                       double sid, double sdd, double* views_out, dtv_scheme* scheme_out);
+
+/* Same for Beijing projection vectors (host float32 [n][12]). */
+int dtv_views_beijing(const float* project_vector, int32_t n_views,
+                      const int32_t volume_size[3], const int32_t detector_size[2],
+                      double* views_out, dtv_scheme* scheme_out);
+
+const char* dtv_strerror(int code);
+const char* dtv_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DTVPROJ_H */

@@ -1,0 +1,170 @@
+"""GPU parity: HIP kernels (through the C ABI) vs the FP64 CPU oracle.
+
+Bar (north star): relative L2 error <= 1e-5 for A and A^T, adjoint
+dot-product mismatch <= 1e-5.  Inputs come from seeded CPU generators, so the
+oracle and the GPU see identical data.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+
+from oracle import cone
+from dtvproj import configs, geometry as G, ops, phantom
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+# ---- Standard geometry (reference-shaped entry points) -------------------
+
+STD_SMALL = dict(vol=(24, 20, 12), det=(40, 14), n_views=12, sid=100.0, sdd=200.0)
+
+
+def _std_inputs(cfg, seed):
+    X, Y, Z = cfg["vol"]
+    vol = phantom.gaussian((1, 1, Z, Y, X), seed)
+    ang = G.uniform_angles(cfg["n_views"])
+    return vol, ang
+
+
+def test_standard_forward_jit_small(cuda):
+    import JITStandardGeometry as J
+    cfg = STD_SMALL
+    vol, ang = _std_inputs(cfg, 11)
+    _print_case("standard_forward_jit_small", **cfg)
+    out = J.forward(vol.to(cuda), torch.from_numpy(ang).to(cuda), torch.tensor(cfg["vol"]).to(cuda),
+                    torch.tensor(cfg["det"]).to(cuda), cfg["sid"], cfg["sdd"], 0)
+    ref = cone.std_forward(vol, ang, cfg["vol"], cfg["det"], cfg["sid"], cfg["sdd"])
+    assert out.shape == ref.shape
+    assert rel_l2(out, ref) <= TOL
+
+
+def test_standard_reference_config_subset(cuda):
+    """options.yaml Standard geometry (256x256x64, 288x28, 48 views,
+    SID 1000, SDD 2000), 6 of the 48 views, through the table path."""
+    vs, ds = (256, 256, 64), (288, 28)
+    ang = G.uniform_angles(48)[::8]
+    g = G.standard(ang, vs, ds, 1000.0, 2000.0)
+    _print_case("standard_reference_config_subset", views=len(ang))
+    vol = phantom.gaussian((1, 1, 64, 256, 256), 12)
+    out = ops.forward(vol.to(cuda), g)
+    ref = cone.std_forward(vol, ang, vs, ds, 1000.0, 2000.0)
+    assert rel_l2(out, ref) <= TOL
+    y = phantom.gaussian(ref.shape, 13)
+    adj = ops.adjoint(y.to(cuda), g)
+    adj_ref = cone.std_adjoint(y, ang, vs, ds, 1000.0, 2000.0)
+    assert rel_l2(adj, adj_ref) <= TOL
+
+
+def test_standard_adjoint_and_bp_jit(cuda):
+    import JITStandardGeometry as J
+    cfg = STD_SMALL
+    X, Y, Z = cfg["vol"]
+    U, V = cfg["det"]
+    ang = G.uniform_angles(cfg["n_views"])
+    y = phantom.gaussian((2, 1, cfg["n_views"], V, U), 14)
+    args = (torch.from_numpy(ang).to(cuda), torch.tensor(cfg["vol"]).to(cuda),
+            torch.tensor(cfg["det"]).to(cuda), cfg["sid"], cfg["sdd"], 0)
+    _print_case("standard_adjoint_and_bp_jit", **cfg)
+    adj = J.adjoint(y.to(cuda), *args)
+    assert rel_l2(adj, cone.std_adjoint(y, ang, cfg["vol"], cfg["det"], cfg["sid"], cfg["sdd"])) <= TOL
+    bp = J.backward(y.to(cuda), *args)
+    assert rel_l2(bp, cone.std_bp_ref(y, ang, cfg["vol"], cfg["det"], cfg["sid"], cfg["sdd"])) <= TOL
+
+
+# ---- Beijing geometry ------------------------------------------------------
+
+def _bj_case():
+    vs, ds = (40, 36, 16), (20, 24)
+    P = G.beijing_default_vectors(n_angles=4, planes=21, sub_detector=ds, sid=70.0, sdd=114.0)
+    return vs, ds, P
+
+
+def test_beijing_forward_adjoint_bp_jit(cuda):
+    import JITBeijingGeometry as J
+    vs, ds, P = _bj_case()
+    X, Y, Z = vs
+    n = P.shape[0]
+    _print_case("beijing_forward_adjoint_bp_jit", vol=vs, det=ds, n_views=n)
+    vol = phantom.gaussian((1, 1, Z, Y, X), 21)
+    args = (torch.tensor(vs, dtype=torch.int32).to(cuda), torch.tensor(ds, dtype=torch.int32).to(cuda),
+            torch.from_numpy(P).to(cuda), 0)
+    out = J.forward(vol.to(cuda), *args)
+    assert rel_l2(out, cone.bj_forward(vol, P, vs, ds)) <= TOL
+    y = phantom.gaussian((1, 1, n, ds[1], ds[0]), 22)
+    adj = J.adjoint(y.to(cuda), *args)
+    assert rel_l2(adj, cone.bj_adjoint(y, P, vs, ds)) <= TOL
+    bp = J.backward(y.to(cuda), *args)
+    assert rel_l2(bp, cone.bj_bp_ref(y, P, vs, ds)) <= TOL
+
+
+# ---- Physical circular geometry (benchmark configurations) -----------------
+
+def test_fista_64_phantom_and_random(cuda):
+    g = configs.fista_64()
+    _print_case("fista_64", views=g.n_views)
+    ph = phantom.phantom((64, 64, 64), 1.0, 10, seed=1)
+    out = ops.forward(ph.to(cuda), g)
+    ref = cone.circ_forward(ph, G.uniform_angles(72), (64, 64, 64), 1.0, (128, 128), 1.0, 500.0, 1000.0)
+    assert rel_l2(out, ref) <= TOL
+    y = phantom.gaussian(ref.shape, 3)
+    adj = ops.adjoint(y.to(cuda), g)
+    adj_ref = cone.circ_adjoint(y, G.uniform_angles(72), (64, 64, 64), 1.0, (128, 128), 1.0, 500.0, 1000.0)
+    assert rel_l2(adj, adj_ref) <= TOL
+
+
+def test_sparse_256_views_subset(cuda):
+    """Full-size 256^3 volume and 384^2 detector, 3 of the 60 views (views are
+    independent, so the subset is a faithful slice of the workload)."""
+    g = configs.sparse_256()
+    idx = [0, 17, 41]
+    gs = g.subset(idx)
+    _print_case("sparse_256_views_subset", views=idx)
+    vol = phantom.gaussian((1, 1, 256, 256, 256), 20)
+    out = ops.forward(vol.to(cuda), gs)
+    ang = G.uniform_angles(60)[idx]
+    ref = cone.circ_forward(vol, ang, (256, 256, 256), 0.5, (384, 384), 0.8, 600.0, 1100.0)
+    assert rel_l2(out, ref) <= TOL
+
+
+def test_sparse_256_adjoint_dot(cuda):
+    """<A x, y> = <x, A^T y> at full size (x, y ~ N(0,1), seeds 20/21)."""
+    g = configs.sparse_256()
+    _print_case("sparse_256_adjoint_dot")
+    x = phantom.gaussian((1, 1, 256, 256, 256), 20).to(cuda)
+    y = phantom.gaussian(g.sino_shape(1), 21).to(cuda)
+    ax = ops.forward(x, g)
+    aty = ops.adjoint(y, g)
+    lhs = float((ax.double() * y.double()).sum())
+    rhs = float((x.double() * aty.double()).sum())
+    assert abs(lhs - rhs) / max(abs(lhs), abs(rhs)) <= TOL
+
+
+def test_batch_independence(cuda):
+    g = configs.fd_32()
+    _print_case("batch_independence")
+    x = phantom.gaussian((3, 1, 32, 32, 32), 5).to(cuda)
+    full = ops.forward(x, g)
+    for b in range(3):
+        one = ops.forward(x[b:b + 1].contiguous(), g)
+        assert torch.equal(one[0], full[b])
+    y = phantom.gaussian(g.sino_shape(3), 6).to(cuda)
+    fa = ops.adjoint(y, g)
+    for b in range(3):
+        one = ops.adjoint(y[b:b + 1].contiguous(), g)
+        assert rel_l2(one[0], fa[b]) <= 1e-6
+
+
+def test_forward_deterministic(cuda):
+    g = configs.fd_32()
+    x = phantom.gaussian((1, 1, 32, 32, 32), 7).to(cuda)
+    a = ops.forward(x, g)
+    b = ops.forward(x, g)
+    assert torch.equal(a, b)
