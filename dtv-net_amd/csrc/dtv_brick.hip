@@ -1,0 +1,314 @@
+// Volume-stationary ("brick") kernels for A and the matched A^T on gfx950.
+//
+// One workgroup owns one brick of the volume for the whole call and visits
+// every view.  Per view it projects the 8 corners of the brick's sample
+// support box onto the detector (continuous pixel coordinates), takes the
+// bounding rectangle, and lets each lane march one ray of that rectangle
+// through the brick using the same fixed-point positions as the ray-driven
+// kernels (shared ray_core, so A and A^T stay exact transposes).
+//
+//   k_fwd_brick  A: the brick plus a one-voxel halo (zero outside the
+//                volume) sits in LDS; every sample belongs to exactly one
+//                brick (the one holding floor(p)), so the 8 taps are plain
+//                LDS reads.  The per-ray partial sum is added to the
+//                sinogram with one hardware FP32 atomic per (ray, brick).
+//                The volume is read from HBM once per call.
+//   k_adj_brick  matched A^T: the brick's owned voxels are FP32
+//                accumulators in LDS; each sample whose stencil touches
+//                the brick adds its owned taps with LDS atomics; the brick
+//                is written to HBM once at the end (no global atomics, no
+//                zero-fill of the output).
+#include <hip/hip_runtime.h>
+
+#include "dtv_geom.h"
+#include "dtv_launch.h"
+
+using namespace dtv;
+
+namespace {
+
+// Rectangle of detector pixels whose rays can pass through the box
+// [lo, hi) (voxel frame); computed by 8 lanes, one per corner, and reduced
+// by lane 0.  Written to rect[4] = {u0, u1, v0, v1} (inclusive; u1 < u0 =
+// empty).
+__device__ void brick_rect(const Geo& g, const View& v, const ViewProj& vp, const double lo[3],
+                           const double hi[3], double* ca, double* cb, int* ok, int* rect) {
+  const int t = threadIdx.x;
+  if (t < 8) {
+    const double c[3] = {(t & 1) ? hi[0] : lo[0], (t & 2) ? hi[1] : lo[1],
+                         (t & 4) ? hi[2] : lo[2]};
+    double a = 0.0, b = 0.0;
+    ok[t] = project_point(v, vp, c, a, b) ? 1 : 0;
+    ca[t] = a;
+    cb[t] = b;
+  }
+  __syncthreads();
+  if (t == 0) {
+    bool all = true;
+    double amin = 1e300, amax = -1e300, bmin = 1e300, bmax = -1e300;
+    for (int i = 0; i < 8; ++i) {
+      all = all && ok[i];
+      amin = fmin(amin, ca[i]);
+      amax = fmax(amax, ca[i]);
+      bmin = fmin(bmin, cb[i]);
+      bmax = fmax(bmax, cb[i]);
+    }
+    int u0 = 0, u1 = g.nu - 1, v0 = 0, v1 = g.nv - 1;
+    if (all) {
+      // pixel index i has its ray through coordinate i exactly; keep one
+      // pixel of slack on each side against rounding
+      const double fu0 = floor(amin) - 1.0, fu1 = ceil(amax) + 1.0;
+      const double fv0 = floor(bmin) - 1.0, fv1 = ceil(bmax) + 1.0;
+      u0 = fu0 < 0.0 ? 0 : (fu0 > g.nu ? g.nu : (int)fu0);
+      u1 = fu1 < -1.0 ? -1 : (fu1 > g.nu - 1 ? g.nu - 1 : (int)fu1);
+      v0 = fv0 < 0.0 ? 0 : (fv0 > g.nv ? g.nv : (int)fv0);
+      v1 = fv1 < -1.0 ? -1 : (fv1 > g.nv - 1 ? g.nv - 1 : (int)fv1);
+    }
+    rect[0] = u0;
+    rect[1] = u1;
+    rect[2] = v0;
+    rect[3] = v1;
+  }
+  __syncthreads();
+}
+
+// Sample window of a ray inside the box p in [lo, hi) on every axis,
+// computed in FP32 from the FP64 start point and widened by one sample at
+// each end (every sample is then membership-checked exactly).
+__device__ __forceinline__ void box_window(const double P0[3], const double d[3],
+                                           const double lo[3], const double hi[3], int kend,
+                                           int& ka, int& kb) {
+  float klo = 0.f, khi = (float)kend;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float dc = (float)d[c];
+    const float l = (float)(lo[c] - P0[c]);
+    const float h = (float)(hi[c] - P0[c]);
+    if (dc != 0.f) {
+      const float r = 1.0f / dc;
+      const float t0 = l * r, t1 = h * r;
+      klo = fmaxf(klo, fminf(t0, t1));
+      khi = fminf(khi, fmaxf(t0, t1));
+    } else if (!(l <= 0.f && h > 0.f)) {
+      khi = -2.f;
+    }
+  }
+  if (!(khi >= klo - 2.f)) {
+    ka = kb = 0;
+    return;
+  }
+  ka = (int)floorf(klo) - 1;
+  kb = (int)ceilf(khi) + 2;
+  if (ka < 0) ka = 0;
+  if (kb > kend) kb = kend;
+}
+
+__global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol,
+                                                   float* __restrict__ sino, Geo g, int bx,
+                                                   int by, int bz, int nbx, int nby) {
+  extern __shared__ float tile[];  // (bx+1)*(by+1)*(bz+1)
+  __shared__ View sv;
+  __shared__ ViewProj sp;
+  __shared__ double ca[8], cb[8];
+  __shared__ int cok[8], rect[4];
+
+  const int brick = blockIdx.x;
+  const int b = blockIdx.y;
+  const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
+  // bricks tile floor(p) over [-1, N-1]; the tile holds voxels [o, o+b]
+  const int ox = -1 + jx * bx, oy = -1 + jy * by, oz = -1 + jz * bz;
+  const int ex = min(bx, g.nx - ox), ey = min(by, g.ny - oy), ez = min(bz, g.nz - oz);
+  const int tx = bx + 1, ty = by + 1, tz = bz + 1;
+  const float* vb = vol + (size_t)b * g.nx * g.ny * g.nz;
+  for (int i = threadIdx.x; i < tx * ty * tz; i += blockDim.x) {
+    const int lx = i % tx, ly = (i / tx) % ty, lz = i / (tx * ty);
+    const int x = ox + lx, y = oy + ly, z = oz + lz;
+    float val = 0.f;
+    if ((unsigned)x < (unsigned)g.nx && (unsigned)y < (unsigned)g.ny &&
+        (unsigned)z < (unsigned)g.nz)
+      val = vb[((size_t)z * g.ny + y) * g.nx + x];
+    tile[i] = val;
+  }
+  const double lo[3] = {(double)ox, (double)oy, (double)oz};
+  const double hi[3] = {(double)(ox + ex), (double)(oy + ey), (double)(oz + ez)};
+  const float scale = (float)g.scale;
+  const int sty = tx, stz = tx * ty;
+
+  for (int a = 0; a < g.n_views; ++a) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      View v;
+      load_view(g, a, v);
+      sv = v;
+      ViewProj p;
+      view_proj(v, p);
+      sp = p;
+    }
+    __syncthreads();
+    brick_rect(g, sv, sp, lo, hi, ca, cb, cok, rect);
+    const int u0 = rect[0], nur = rect[1] - rect[0] + 1;
+    const int v0 = rect[2], nvr = rect[3] - rect[2] + 1;
+    if (nur <= 0 || nvr <= 0) continue;
+    const View v = sv;
+    float* srow = sino + ((size_t)b * g.n_views + a) * g.nv * g.nu;
+    for (int r = threadIdx.x; r < nur * nvr; r += blockDim.x) {
+      const int iu = u0 + r % nur, iv = v0 + r / nur;
+      double d[3], P0[3];
+      int kend;
+      ray_core(g, v, iu, iv, d, P0, kend);
+      int ka, kb;
+      box_window(P0, d, lo, hi, kend, ka, kb);
+      if (kb <= ka) continue;
+      const int64_t Dx = to_fix(d[0]), Dy = to_fix(d[1]), Dz = to_fix(d[2]);
+      int64_t px = to_fix(P0[0]) + (int64_t)ka * Dx;
+      int64_t py = to_fix(P0[1]) + (int64_t)ka * Dy;
+      int64_t pz = to_fix(P0[2]) + (int64_t)ka * Dz;
+      float acc = 0.f;
+      bool hit = false;
+      for (int k = ka; k < kb; ++k) {
+        const int ix = (int)(px >> 32) - ox, iy = (int)(py >> 32) - oy,
+                  iz = (int)(pz >> 32) - oz;
+        if ((unsigned)ix < (unsigned)ex && (unsigned)iy < (unsigned)ey &&
+            (unsigned)iz < (unsigned)ez) {
+          const float fx = (float)(uint32_t)px * kInvFix;
+          const float fy = (float)(uint32_t)py * kInvFix;
+          const float fz = (float)(uint32_t)pz * kInvFix;
+          const float* t0 = tile + iz * stz + iy * sty + ix;
+          const float c000 = t0[0], c100 = t0[1], c010 = t0[sty], c110 = t0[sty + 1];
+          const float c001 = t0[stz], c101 = t0[stz + 1], c011 = t0[stz + sty],
+                      c111 = t0[stz + sty + 1];
+          const float a0 = c000 + fx * (c100 - c000);
+          const float a1 = c010 + fx * (c110 - c010);
+          const float a2 = c001 + fx * (c101 - c001);
+          const float a3 = c011 + fx * (c111 - c011);
+          const float b0 = a0 + fy * (a1 - a0);
+          const float b1 = a2 + fy * (a3 - a2);
+          acc += b0 + fz * (b1 - b0);
+          hit = true;
+        }
+        px += Dx;
+        py += Dy;
+        pz += Dz;
+      }
+      if (hit) unsafeAtomicAdd(srow + (size_t)iv * g.nu + iu, acc * scale);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sino,
+                                                   float* __restrict__ vol, Geo g, int bx,
+                                                   int by, int bz, int nbx, int nby) {
+  extern __shared__ float acc[];  // bx*by*bz owned voxels
+  __shared__ View sv;
+  __shared__ ViewProj sp;
+  __shared__ double ca[8], cb[8];
+  __shared__ int cok[8], rect[4];
+
+  const int brick = blockIdx.x;
+  const int b = blockIdx.y;
+  const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
+  const int x0 = jx * bx, y0 = jy * by, z0 = jz * bz;
+  const int ex = min(bx, g.nx - x0), ey = min(by, g.ny - y0), ez = min(bz, g.nz - z0);
+  for (int i = threadIdx.x; i < bx * by * bz; i += blockDim.x) acc[i] = 0.f;
+  // samples with floor(p) in [x0-1, x0+ex-1] touch owned voxels
+  const double lo[3] = {(double)(x0 - 1), (double)(y0 - 1), (double)(z0 - 1)};
+  const double hi[3] = {(double)(x0 + ex), (double)(y0 + ey), (double)(z0 + ez)};
+  const float scale = (float)g.scale;
+  const int sty = bx, stz = bx * by;
+
+  for (int a = 0; a < g.n_views; ++a) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      View v;
+      load_view(g, a, v);
+      sv = v;
+      ViewProj p;
+      view_proj(v, p);
+      sp = p;
+    }
+    __syncthreads();
+    brick_rect(g, sv, sp, lo, hi, ca, cb, cok, rect);
+    const int u0 = rect[0], nur = rect[1] - rect[0] + 1;
+    const int v0 = rect[2], nvr = rect[3] - rect[2] + 1;
+    if (nur <= 0 || nvr <= 0) continue;
+    const View v = sv;
+    const float* srow = sino + ((size_t)b * g.n_views + a) * g.nv * g.nu;
+    for (int r = threadIdx.x; r < nur * nvr; r += blockDim.x) {
+      const int iu = u0 + r % nur, iv = v0 + r / nur;
+      const float val = srow[(size_t)iv * g.nu + iu] * scale;
+      if (val == 0.f) continue;
+      double d[3], P0[3];
+      int kend;
+      ray_core(g, v, iu, iv, d, P0, kend);
+      int ka, kb;
+      box_window(P0, d, lo, hi, kend, ka, kb);
+      if (kb <= ka) continue;
+      const int64_t Dx = to_fix(d[0]), Dy = to_fix(d[1]), Dz = to_fix(d[2]);
+      int64_t px = to_fix(P0[0]) + (int64_t)ka * Dx;
+      int64_t py = to_fix(P0[1]) + (int64_t)ka * Dy;
+      int64_t pz = to_fix(P0[2]) + (int64_t)ka * Dz;
+      for (int k = ka; k < kb; ++k) {
+        const int ix = (int)(px >> 32) - x0, iy = (int)(py >> 32) - y0,
+                  iz = (int)(pz >> 32) - z0;
+        // floor(p) relative to the brick must lie in [-1, e-1]
+        if ((unsigned)(ix + 1) <= (unsigned)ex && (unsigned)(iy + 1) <= (unsigned)ey &&
+            (unsigned)(iz + 1) <= (unsigned)ez) {
+          const float fx = (float)(uint32_t)px * kInvFix;
+          const float fy = (float)(uint32_t)py * kInvFix;
+          const float fz = (float)(uint32_t)pz * kInvFix;
+          const bool mx0 = ix >= 0, mx1 = ix + 1 < ex;
+          const bool my0 = iy >= 0, my1 = iy + 1 < ey;
+          const bool mz0 = iz >= 0, mz1 = iz + 1 < ez;
+          const float wz0 = (1.f - fz) * val, wz1 = fz * val;
+          const float w00 = (1.f - fy) * wz0, w10 = fy * wz0;
+          const float w01 = (1.f - fy) * wz1, w11 = fy * wz1;
+          const float gx = 1.f - fx;
+          const int base = iz * stz + iy * sty + ix;
+          if (mz0 && my0 && mx0) atomicAdd(acc + base, gx * w00);
+          if (mz0 && my0 && mx1) atomicAdd(acc + base + 1, fx * w00);
+          if (mz0 && my1 && mx0) atomicAdd(acc + base + sty, gx * w10);
+          if (mz0 && my1 && mx1) atomicAdd(acc + base + sty + 1, fx * w10);
+          if (mz1 && my0 && mx0) atomicAdd(acc + base + stz, gx * w01);
+          if (mz1 && my0 && mx1) atomicAdd(acc + base + stz + 1, fx * w01);
+          if (mz1 && my1 && mx0) atomicAdd(acc + base + stz + sty, gx * w11);
+          if (mz1 && my1 && mx1) atomicAdd(acc + base + stz + sty + 1, fx * w11);
+        }
+        px += Dx;
+        py += Dy;
+        pz += Dz;
+      }
+    }
+  }
+  __syncthreads();
+  float* vb = vol + (size_t)b * g.nx * g.ny * g.nz;
+  for (int i = threadIdx.x; i < bx * by * bz; i += blockDim.x) {
+    const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
+    if (lx < ex && ly < ey && lz < ez)
+      vb[((size_t)(z0 + lz) * g.ny + (y0 + ly)) * g.nx + (x0 + lx)] = acc[i];
+  }
+}
+
+}  // namespace
+
+namespace dtv {
+
+int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int bx, int by,
+                     int bz, int block, hipStream_t st) {
+  const int nbx = (g.nx + 1 + bx - 1) / bx, nby = (g.ny + 1 + by - 1) / by,
+            nbz = (g.nz + 1 + bz - 1) / bz;
+  const size_t lds = (size_t)(bx + 1) * (by + 1) * (bz + 1) * sizeof(float);
+  hipLaunchKernelGGL(k_fwd_brick, dim3(nbx * nby * nbz, batch), dim3(block), lds, st, vol, sino,
+                     g, bx, by, bz, nbx, nby);
+  return hipGetLastError() == hipSuccess ? 0 : 4;
+}
+
+int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int bx, int by,
+                     int bz, int block, hipStream_t st) {
+  const int nbx = (g.nx + bx - 1) / bx, nby = (g.ny + by - 1) / by, nbz = (g.nz + bz - 1) / bz;
+  const size_t lds = (size_t)bx * by * bz * sizeof(float);
+  hipLaunchKernelGGL(k_adj_brick, dim3(nbx * nby * nbz, batch), dim3(block), lds, st, sino, vol,
+                     g, bx, by, bz, nbx, nby);
+  return hipGetLastError() == hipSuccess ? 0 : 4;
+}
+
+}  // namespace dtv

@@ -126,7 +126,12 @@ struct Ray {
   int k0, k1;
 };
 
-DTV_HD inline void ray_dir(const View& v, int iu, int iv, double d[3]) {
+// Per-ray set-up shared by every kernel, so that A and the matched A^T see
+// bitwise-identical sample positions.  Outputs the unit direction d, the
+// position P0 of sample k = 0 (both voxel frame, FP64) and kend: the
+// reference's samples are k in [0, kend) (beta_start + k < beta_end).
+DTV_HD inline void ray_core(const Geo& g, const View& v, int iu, int iv, double d[3],
+                            double P0[3], int& kend) {
 #pragma clang fp contract(off)
   double r[3];
   for (int c = 0; c < 3; ++c) {
@@ -134,15 +139,10 @@ DTV_HD inline void ray_dir(const View& v, int iu, int iv, double d[3]) {
     r[c] = pix - v.S[c];
   }
   const double len = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-  d[0] = r[0] / len;
-  d[1] = r[1] / len;
-  d[2] = r[2] / len;
-}
-
-DTV_HD inline void ray_setup(const Geo& g, const View& v, int iu, int iv, Ray& ray) {
-#pragma clang fp contract(off)
-  double d[3];
-  ray_dir(v, iu, iv, d);
+  const double il = 1.0 / len;
+  d[0] = r[0] * il;
+  d[1] = r[1] * il;
+  d[2] = r[2] * il;
   const double adx = fabs(d[0]), ady = fabs(d[1]), adz = fabs(d[2]);
   int ax;
   if (g.axis_mode == kAxisXY) {
@@ -156,25 +156,35 @@ DTV_HD inline void ray_setup(const Geo& g, const View& v, int iu, int iv, Ray& r
   const double sax = ax == 0 ? v.S[0] : (ax == 1 ? v.S[1] : v.S[2]);
   const double lax = ax == 0 ? g.lo[0] : (ax == 1 ? g.lo[1] : g.lo[2]);
   const double hax = ax == 0 ? g.hi[0] : (ax == 1 ? g.hi[1] : g.hi[2]);
-  const double a0 = (lax - sax) / dax;
-  const double a1 = (hax - sax) / dax;
+  const double idax = 1.0 / dax;
+  const double a0 = (lax - sax) * idax;
+  const double a1 = (hax - sax) * idax;
   const double bs = fmin(a0, a1) - g.margin;
   const double be = fmax(a0, a1) + g.margin;
-  // samples k >= 0 with bs + k < be  ->  k in [0, kend)
-  // (ceil(be - bs) is within one of the exact count; non-finite or absurd
-  // ranges give no samples)
-  int kend = 0;
+  // samples k >= 0 with bs + k < be; ceil(be - bs) is within one of the
+  // exact count, non-finite or absurd ranges give no samples
+  kend = 0;
   if (be - bs > 0.0 && be - bs < 1.0e7) {
     kend = (int)ceil(be - bs);
     if (kend > 0 && !(bs + (double)(kend - 1) < be)) --kend;
     if (bs + (double)kend < be) ++kend;
   }
-  // window where the sample's trilinear support can touch the volume:
-  // p in [-1, N) on every axis, widened by one sample at each end
+  for (int c = 0; c < 3; ++c) P0[c] = v.S[c] + bs * d[c];
+}
+
+DTV_HD inline int64_t to_fix(double x) { return (int64_t)llrint(x * kFix); }
+
+// Whole-ray set-up: fixed-point start and step plus the window [k0, k1)
+// of samples whose trilinear support can touch the volume (p in [-1, N) on
+// every axis, widened by one sample at each end; every sample is still
+// bounds-checked tap by tap where this window is used).
+DTV_HD inline void ray_setup(const Geo& g, const View& v, int iu, int iv, Ray& ray) {
+#pragma clang fp contract(off)
+  double d[3], P0[3];
+  int kend;
+  ray_core(g, v, iu, iv, d, P0, kend);
   const double n3[3] = {(double)g.nx, (double)g.ny, (double)g.nz};
   double klo = 0.0, khi = (double)kend;
-  double P0[3];
-  for (int c = 0; c < 3; ++c) P0[c] = v.S[c] + bs * d[c];
   for (int c = 0; c < 3; ++c) {
     if (d[c] != 0.0) {
       const double t0 = (-1.0 - P0[c]) / d[c];
@@ -195,9 +205,56 @@ DTV_HD inline void ray_setup(const Geo& g, const View& v, int iu, int iv, Ray& r
     if (ray.k1 < ray.k0) ray.k1 = ray.k0;
   }
   for (int c = 0; c < 3; ++c) {
-    ray.P[c] = llrint(P0[c] * kFix);
-    ray.D[c] = llrint(d[c] * kFix);
+    ray.P[c] = to_fix(P0[c]);
+    ray.D[c] = to_fix(d[c]);
   }
+}
+
+// Detector-plane projection helpers of one view: a point c maps to the
+// continuous pixel coordinates (a, b) with S + t (c - S) = D0 + a U + b V:
+// t = h / ((c - S).n), a = t ((c - S).ea) - qa, b = t ((c - S).eb) - qb.
+struct ViewProj {
+  double n[3], ea[3], eb[3];
+  double h, qa, qb;
+};
+
+DTV_HD inline void cross3(const double a[3], const double b[3], double o[3]) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+DTV_HD inline double dot3(const double a[3], const double b[3]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+DTV_HD inline void view_proj(const View& v, ViewProj& p) {
+  double vn[3], nu[3], ds[3];
+  cross3(v.U, v.V, p.n);
+  cross3(v.V, p.n, vn);
+  cross3(p.n, v.U, nu);
+  const double sa = 1.0 / dot3(v.U, vn), sb = 1.0 / dot3(v.V, nu);
+  for (int c = 0; c < 3; ++c) {
+    p.ea[c] = vn[c] * sa;
+    p.eb[c] = nu[c] * sb;
+    ds[c] = v.D0[c] - v.S[c];
+  }
+  p.h = dot3(ds, p.n);
+  p.qa = dot3(ds, p.ea);
+  p.qb = dot3(ds, p.eb);
+}
+
+// Projects point c; returns false when c is not strictly in front of the
+// source (then the caller must take the whole detector).
+DTV_HD inline bool project_point(const View& v, const ViewProj& p, const double c[3], double& a,
+                                 double& b) {
+  const double w[3] = {c[0] - v.S[0], c[1] - v.S[1], c[2] - v.S[2]};
+  const double wn = dot3(w, p.n);
+  if (!(wn * p.h > 0.0)) return false;
+  const double t = p.h / wn;
+  a = t * dot3(w, p.ea) - p.qa;
+  b = t * dot3(w, p.eb) - p.qb;
+  return true;
 }
 
 }  // namespace dtv

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dtv_geom.h"
+#include "dtv_launch.h"
 #include "dtvproj.h"
 
 using namespace dtv;
@@ -381,7 +382,51 @@ int launch_status() {
   return e == hipSuccess ? DTV_OK : DTV_ERR_HIP;
 }
 
-int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st) {
+int run_zero(float* p, size_t n, hipStream_t st) {
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, st, p, n);
+  return launch_status();
+}
+
+// Resolved kernel choice (defaults: brick 16^3, 256 lanes).
+struct Launch {
+  int algo = DTV_ALGO_BRICK;
+  int bx = 16, by = 16, bz = 16;
+  int block = 256;
+};
+
+int resolve_launch(const dtv_launch* l, Launch& out) {
+  out = Launch();
+  if (!l) return DTV_OK;
+  if (l->algorithm != DTV_ALGO_DEFAULT) {
+    if (l->algorithm != DTV_ALGO_RAY && l->algorithm != DTV_ALGO_BRICK)
+      return DTV_ERR_INVALID_ARGUMENT;
+    out.algo = l->algorithm;
+  }
+  if (l->brick[0] || l->brick[1] || l->brick[2]) {
+    for (int i = 0; i < 3; ++i)
+      if (l->brick[i] < 2 || l->brick[i] > 64) return DTV_ERR_INVALID_ARGUMENT;
+    // the forward tile holds (b+1)^3 floats, the adjoint b^3; keep both in 64 KiB
+    if ((size_t)(l->brick[0] + 1) * (l->brick[1] + 1) * (l->brick[2] + 1) * 4 > 65536)
+      return DTV_ERR_INVALID_ARGUMENT;
+    out.bx = l->brick[0];
+    out.by = l->brick[1];
+    out.bz = l->brick[2];
+  }
+  if (l->block) {
+    if (l->block < 64 || l->block > 256 || l->block % 64) return DTV_ERR_INVALID_ARGUMENT;
+    out.block = l->block;
+  }
+  return DTV_OK;
+}
+
+int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st,
+                const Launch& L = Launch()) {
+  if (L.algo == DTV_ALGO_BRICK) {
+    int rc = run_zero(sino, (size_t)batch * g.n_views * g.nu * g.nv, st);
+    if (rc) return rc;
+    return launch_fwd_brick(vol, sino, batch, g, L.bx, L.by, L.bz, L.block, st);
+  }
   const dim3 block(kRayTileU, kRayTileV);
   const dim3 grid((g.nu + kRayTileU - 1) / kRayTileU, (g.nv + kRayTileV - 1) / kRayTileV,
                   g.n_views);
@@ -389,13 +434,10 @@ int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipS
   return launch_status();
 }
 
-int run_zero(float* p, size_t n, hipStream_t st) {
-  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, st, p, n);
-  return launch_status();
-}
-
-int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st) {
+int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st,
+                const Launch& L = Launch()) {
+  if (L.algo == DTV_ALGO_BRICK)
+    return launch_adj_brick(sino, vol, batch, g, L.bx, L.by, L.bz, L.block, st);
   int rc = run_zero(vol, (size_t)batch * g.nx * g.ny * g.nz, st);
   if (rc) return rc;
   const dim3 block(kRayTileU, kRayTileV);
@@ -564,6 +606,38 @@ int dtv_cone_adjoint(const float* sino, float* volume, int32_t batch,
   rc = fill_table_geo(g, volume_size, detector_size, views, n_views, scheme);
   if (rc) return rc;
   return run_adjoint(sino, volume, batch, g, (hipStream_t)stream);
+}
+
+int dtv_cone_forward_ex(const float* volume, float* sino, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const double* views, int32_t n_views, const dtv_scheme* scheme,
+                        const dtv_launch* launch, void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!volume || !sino) return DTV_ERR_NULL_POINTER;
+  Geo g;
+  rc = fill_table_geo(g, volume_size, detector_size, views, n_views, scheme);
+  if (rc) return rc;
+  Launch L;
+  rc = resolve_launch(launch, L);
+  if (rc) return rc;
+  return run_forward(volume, sino, batch, g, (hipStream_t)stream, L);
+}
+
+int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const double* views, int32_t n_views, const dtv_scheme* scheme,
+                        const dtv_launch* launch, void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!volume || !sino) return DTV_ERR_NULL_POINTER;
+  Geo g;
+  rc = fill_table_geo(g, volume_size, detector_size, views, n_views, scheme);
+  if (rc) return rc;
+  Launch L;
+  rc = resolve_launch(launch, L);
+  if (rc) return rc;
+  return run_adjoint(sino, volume, batch, g, (hipStream_t)stream, L);
 }
 
 static void export_scheme(const Geo& g, dtv_scheme* out) {

@@ -33,6 +33,27 @@ class DtvScheme(ctypes.Structure):
     ]
 
 
+class DtvLaunch(ctypes.Structure):
+    _fields_ = [
+        ("algorithm", c_int32),
+        ("brick", c_int32 * 3),
+        ("block", c_int32),
+        ("reserved", c_int32 * 3),
+    ]
+
+
+ALGO_DEFAULT, ALGO_RAY, ALGO_BRICK = 0, 1, 2
+
+
+def launch_struct(algorithm=0, brick=(0, 0, 0), block=0) -> DtvLaunch:
+    l = DtvLaunch()
+    l.algorithm = int(algorithm)
+    for i in range(3):
+        l.brick[i] = int(brick[i])
+    l.block = int(block)
+    return l
+
+
 # name -> argtypes; every entry point returns int status
 _SIGS = {
     "dtv_standard_forward": [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_int32_p, c_int32_p,
@@ -51,6 +72,10 @@ _SIGS = {
                          ctypes.POINTER(DtvScheme), c_void_p],
     "dtv_cone_adjoint": [c_void_p, c_void_p, c_int32, c_int32_p, c_int32_p, c_void_p, c_int32,
                          ctypes.POINTER(DtvScheme), c_void_p],
+    "dtv_cone_forward_ex": [c_void_p, c_void_p, c_int32, c_int32_p, c_int32_p, c_void_p, c_int32,
+                            ctypes.POINTER(DtvScheme), c_void_p, c_void_p],
+    "dtv_cone_adjoint_ex": [c_void_p, c_void_p, c_int32, c_int32_p, c_int32_p, c_void_p, c_int32,
+                            ctypes.POINTER(DtvScheme), c_void_p, c_void_p],
     "dtv_views_standard": [c_double_p, c_int32, c_int32_p, c_int32_p, c_double, c_double,
                            c_double_p, ctypes.POINTER(DtvScheme)],
     "dtv_views_beijing": [c_float_p, c_int32, c_int32_p, c_int32_p, c_double_p,

@@ -55,23 +55,37 @@ def _sino_numel(geom: ConeGeometry) -> int:
     return geom.n_views * U * V
 
 
-def forward(volume: torch.Tensor, geom: ConeGeometry) -> torch.Tensor:
-    """A: ray sums of ``volume`` [B,1,Z,Y,X] -> sinogram [B,1,A,V,U]."""
+def _launch_arg(launch):
+    """None -> NULL (library defaults); dict/tuple -> dtv_launch struct."""
+    if launch is None:
+        return None
+    if isinstance(launch, _capi.DtvLaunch):
+        return ctypes.byref(launch)
+    if isinstance(launch, dict):
+        return ctypes.byref(_capi.launch_struct(**launch))
+    raise TypeError("launch must be None, a dict or a DtvLaunch")
+
+
+def forward(volume: torch.Tensor, geom: ConeGeometry, launch=None) -> torch.Tensor:
+    """A: ray sums of ``volume`` [B,1,Z,Y,X] -> sinogram [B,1,A,V,U].
+
+    ``launch`` optionally fixes the kernel (``dict(algorithm=, brick=,
+    block=)``); every choice gives the same result to FP32 rounding."""
     _check_f32(volume, "volume")
     B = _batch_of(volume, _vol_numel(geom), "volume")
     out = torch.empty(geom.sino_shape(B), dtype=torch.float32, device=volume.device)
     views = geom.device_views(volume.device)
     sc = _capi.scheme_struct(geom.scheme)
     with torch.cuda.device(volume.device):
-        rc = _capi.lib().dtv_cone_forward(
+        rc = _capi.lib().dtv_cone_forward_ex(
             volume.data_ptr(), out.data_ptr(), B, _capi.int3(geom.volume_size),
             _capi.int2(geom.detector_size), views.data_ptr(), geom.n_views,
-            ctypes.byref(sc), _stream(volume.device))
+            ctypes.byref(sc), _launch_arg(launch), _stream(volume.device))
     _capi.check(rc, "dtv_cone_forward")
     return out
 
 
-def adjoint(sino: torch.Tensor, geom: ConeGeometry) -> torch.Tensor:
+def adjoint(sino: torch.Tensor, geom: ConeGeometry, launch=None) -> torch.Tensor:
     """Matched A^T: exact transpose of :func:`forward`."""
     _check_f32(sino, "sino")
     B = _batch_of(sino, _sino_numel(geom), "sino")
@@ -79,10 +93,10 @@ def adjoint(sino: torch.Tensor, geom: ConeGeometry) -> torch.Tensor:
     views = geom.device_views(sino.device)
     sc = _capi.scheme_struct(geom.scheme)
     with torch.cuda.device(sino.device):
-        rc = _capi.lib().dtv_cone_adjoint(
+        rc = _capi.lib().dtv_cone_adjoint_ex(
             sino.data_ptr(), out.data_ptr(), B, _capi.int3(geom.volume_size),
             _capi.int2(geom.detector_size), views.data_ptr(), geom.n_views,
-            ctypes.byref(sc), _stream(sino.device))
+            ctypes.byref(sc), _launch_arg(launch), _stream(sino.device))
     _capi.check(rc, "dtv_cone_adjoint")
     return out
 

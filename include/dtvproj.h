@@ -37,6 +37,18 @@ typedef struct dtv_scheme {
   double scale;        /* output multiplier */
 } dtv_scheme;
 
+/* Kernel choice for the *_ex entry points (0 fields = library defaults). */
+typedef struct dtv_launch {
+  int32_t algorithm;   /* 0: default, 1: ray-driven (gather / atomic scatter), 2: brick (LDS) */
+  int32_t brick[3];    /* brick edge in voxels (x, y, z) for algorithm 2 */
+  int32_t block;       /* threads per workgroup for algorithm 2 (64..256) */
+  int32_t reserved[3];
+} dtv_launch;
+
+#define DTV_ALGO_DEFAULT 0
+#define DTV_ALGO_RAY 1
+#define DTV_ALGO_BRICK 2
+
 /* ---- Reference-shaped entry points ------------------------------------ */
 
 /* Standard circular geometry, ray-driven forward projector A.
@@ -111,6 +123,18 @@ int dtv_cone_adjoint(const float* sino, float* volume, int32_t batch,
                      const int32_t volume_size[3], const int32_t detector_size[2],
                      const double* views, int32_t n_views, const dtv_scheme* scheme,
                      void* stream);
+
+/* dtv_cone_forward / dtv_cone_adjoint with an explicit kernel choice (used
+ * by the autotuner; launch = NULL means defaults).  Results of every
+ * algorithm agree to FP32 rounding. */
+int dtv_cone_forward_ex(const float* volume, float* sino, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const double* views, int32_t n_views, const dtv_scheme* scheme,
+                        const dtv_launch* launch, void* stream);
+int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const double* views, int32_t n_views, const dtv_scheme* scheme,
+                        const dtv_launch* launch, void* stream);
 
 /* ---- Host-only helpers (no GPU work) ---------------------------------- */
 
