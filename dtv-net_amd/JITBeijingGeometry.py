@@ -32,6 +32,7 @@ def _vectors(projectVector):
 
 def forward(volume, volumeSize, detectorSize, projectVector, device):
     """Ray-driven forward projection (Beijing/plug/kernel/forwardKernel.cu:78-135)."""
+    check_input(volume, "volume")   # argument order of the reference's CHECK_INPUTs
     vs, ds = _sizes(volumeSize, detectorSize)
     pv = _vectors(projectVector)
     B = _check_volume(volume, "volume", vs[0] * vs[1] * vs[2])
@@ -47,6 +48,7 @@ def forward(volume, volumeSize, detectorSize, projectVector, device):
 
 def backward(sino, volumeSize, detectorSize, projectVector, device):
     """Reference backprojector (Beijing/plug/kernel/backwardKernel.cu:47-104)."""
+    check_input(sino, "sino")   # argument order of the reference's CHECK_INPUTs
     vs, ds = _sizes(volumeSize, detectorSize)
     pv = _vectors(projectVector)
     n = int(pv.size(0))
@@ -62,6 +64,7 @@ def backward(sino, volumeSize, detectorSize, projectVector, device):
 
 def adjoint(sino, volumeSize, detectorSize, projectVector, device):
     """Exact transpose of :func:`forward` (no reference counterpart)."""
+    check_input(sino, "sino")   # argument order of the reference's CHECK_INPUTs
     vs, ds = _sizes(volumeSize, detectorSize)
     pv = _vectors(projectVector)
     n = int(pv.size(0))

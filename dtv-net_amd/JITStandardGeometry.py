@@ -51,6 +51,7 @@ def _check_volume(t, name, per_item):
 
 def forward(volume, angles, volumeSize, detectorSize, sid, sdd, device):
     """Ray-driven forward projection (Standard/plug/kernel/forwardKernel.cu:88-143)."""
+    check_input(volume, "volume")   # argument order of the reference's CHECK_INPUTs
     angles = _angles(angles)
     vs, ds = _sizes(volumeSize, detectorSize)
     B = _check_volume(volume, "volume", vs[0] * vs[1] * vs[2])
@@ -67,6 +68,7 @@ def forward(volume, angles, volumeSize, detectorSize, sid, sdd, device):
 
 def backward(sino, angles, volumeSize, detectorSize, sid, sdd, device):
     """Reference backprojector (Standard/plug/kernel/backwardKernel.cu:55-110)."""
+    check_input(sino, "sino")   # argument order of the reference's CHECK_INPUTs
     angles = _angles(angles)
     vs, ds = _sizes(volumeSize, detectorSize)
     A = int(angles.numel())
@@ -83,6 +85,7 @@ def backward(sino, angles, volumeSize, detectorSize, sid, sdd, device):
 
 def adjoint(sino, angles, volumeSize, detectorSize, sid, sdd, device):
     """Exact transpose of :func:`forward` (no reference counterpart)."""
+    check_input(sino, "sino")   # argument order of the reference's CHECK_INPUTs
     angles = _angles(angles)
     vs, ds = _sizes(volumeSize, detectorSize)
     A = int(angles.numel())

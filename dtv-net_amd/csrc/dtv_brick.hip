@@ -198,7 +198,9 @@ __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol
 __global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sino,
                                                    float* __restrict__ vol, Geo g, int bx,
                                                    int by, int bz, int nbx, int nby) {
-  extern __shared__ float acc[];  // bx*by*bz owned voxels
+  // owned voxels, rows padded to odd strides (bx+1, (bx+1)*by+1) so that
+  // lanes whose samples differ in y or z fall into different LDS banks
+  extern __shared__ float acc[];
   __shared__ View sv;
   __shared__ ViewProj sp;
   __shared__ double ca[8], cb[8];
@@ -209,12 +211,12 @@ __global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sin
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   const int x0 = jx * bx, y0 = jy * by, z0 = jz * bz;
   const int ex = min(bx, g.nx - x0), ey = min(by, g.ny - y0), ez = min(bz, g.nz - z0);
-  for (int i = threadIdx.x; i < bx * by * bz; i += blockDim.x) acc[i] = 0.f;
+  const int sty = bx + 1, stz = (bx + 1) * by + 1;
+  for (int i = threadIdx.x; i < stz * bz; i += blockDim.x) acc[i] = 0.f;
   // samples with floor(p) in [x0-1, x0+ex-1] touch owned voxels
   const double lo[3] = {(double)(x0 - 1), (double)(y0 - 1), (double)(z0 - 1)};
   const double hi[3] = {(double)(x0 + ex), (double)(y0 + ey), (double)(z0 + ez)};
   const float scale = (float)g.scale;
-  const int sty = bx, stz = bx * by;
 
   for (int a = 0; a < g.n_views; ++a) {
     __syncthreads();
@@ -233,8 +235,13 @@ __global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sin
     if (nur <= 0 || nvr <= 0) continue;
     const View v = sv;
     const float* srow = sino + ((size_t)b * g.n_views + a) * g.nv * g.nu;
-    for (int r = threadIdx.x; r < nur * nvr; r += blockDim.x) {
-      const int iu = u0 + r % nur, iv = v0 + r / nur;
+    // lanes take every third ray along u and v, so rays sharing a wave are
+    // ~3 pixels apart and rarely add into the same voxel in one instruction
+    const int mu = (nur + 2) / 3, mv = (nvr + 2) / 3;
+    for (int r = threadIdx.x; r < 9 * mu * mv; r += blockDim.x) {
+      const int cu = r % (3 * mu), cv = r / (3 * mu);
+      const int iu = u0 + 3 * (cu % mu) + cu / mu, iv = v0 + 3 * (cv % mv) + cv / mv;
+      if (iu > rect[1] || iv > rect[3]) continue;
       const float val = srow[(size_t)iv * g.nu + iu] * scale;
       if (val == 0.f) continue;
       double d[3], P0[3];
@@ -284,7 +291,8 @@ __global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sin
   for (int i = threadIdx.x; i < bx * by * bz; i += blockDim.x) {
     const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
     if (lx < ex && ly < ey && lz < ez)
-      vb[((size_t)(z0 + lz) * g.ny + (y0 + ly)) * g.nx + (x0 + lx)] = acc[i];
+      vb[((size_t)(z0 + lz) * g.ny + (y0 + ly)) * g.nx + (x0 + lx)] =
+          acc[lz * stz + ly * sty + lx];
   }
 }
 
@@ -305,7 +313,7 @@ int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int
 int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int bx, int by,
                      int bz, int block, hipStream_t st) {
   const int nbx = (g.nx + bx - 1) / bx, nby = (g.ny + by - 1) / by, nbz = (g.nz + bz - 1) / bz;
-  const size_t lds = (size_t)bx * by * bz * sizeof(float);
+  const size_t lds = (size_t)((bx + 1) * by + 1) * bz * sizeof(float);
   hipLaunchKernelGGL(k_adj_brick, dim3(nbx * nby * nbz, batch), dim3(block), lds, st, sino, vol,
                      g, bx, by, bz, nbx, nby);
   return hipGetLastError() == hipSuccess ? 0 : 4;

@@ -243,16 +243,18 @@ def uniform_angles(n_views: int, full_deg: float = 360.0):
 
 
 def beijing_default_vectors(n_angles=144, planes=21, sub_detector=(80, 128),
-                            sid=708.0, sdd=1143.0, plane_gap=0.0):
+                            sid=708.0, sdd=1143.0, pitch=(1.0, 1.0), plane_gap=0.0):
     """Synthetic stand-in for the reference's ``projection_matrix`` .mat file.
 
     The reference loads per-view vectors from an absolute path that does not
     ship with it (options.yaml:42).  This builds the same layout -- for each
     of ``n_angles`` source angles, ``planes`` flat sub-detectors tiled side by
     side along u -- in the kernel's voxel units (before its z negation), so
-    the Beijing path can run without the file.
+    the Beijing path can run without the file.  ``pitch`` = (u, v) pixel
+    pitch in voxels.
     """
     U, V = sub_detector
+    pu, pv = pitch
     rows = []
     ang = uniform_angles(n_angles)
     c, sn = _rotation(ang)
@@ -261,10 +263,10 @@ def beijing_default_vectors(n_angles=144, planes=21, sub_detector=(80, 128),
         ey = np.array([-sn[a], c[a], 0.0])
         src = -ex * sid
         for p in range(planes):
-            shift = (p - (planes - 1) / 2.0) * (U + plane_gap)
+            shift = (p - (planes - 1) / 2.0) * (U + plane_gap) * pu
             det = ex * (sdd - sid) + ey * shift
-            u = ey
-            v = np.array([0.0, 0.0, 1.0])
+            u = ey * pu
+            v = np.array([0.0, 0.0, pv])
             rows.append([src[0], src[1], -src[2], det[0], det[1], -det[2],
                          v[0], v[1], v[2], u[0], u[1], u[2]])
     return np.asarray(rows, dtype=np.float32)

@@ -1,0 +1,149 @@
+"""CPU: product geometry vs the oracle, the C ABI library, host-side error
+paths and the algorithmic byte accounting (no GPU compute)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cone
+from dtvproj import _capi, configs, geometry as G, ops, roofline
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _table_forward(g, vol):
+    sc = g.scheme
+    return cone.table_forward(vol, g.views, g.volume_size, g.detector_size, sc.axis_mode,
+                              sc.box_lo, sc.box_hi, sc.margin, sc.scale)
+
+
+def test_standard_table_matches_native_units():
+    vs, ds = (9, 8, 7), (11, 6)
+    ang = np.linspace(0, 360, 8, False) + 1.5
+    vol = torch.from_numpy(np.random.default_rng(1).standard_normal((1, 1, 7, 8, 9)))
+    g = G.standard(ang, vs, ds, 10.0, 20.0)
+    a = cone.std_forward(vol, ang, vs, ds, 10.0, 20.0)
+    assert float((a - _table_forward(g, vol)).abs().max()) < 1e-12
+
+
+def test_beijing_table_matches_native_units():
+    vs, ds = (9, 8, 7), (11, 6)
+    vol = torch.from_numpy(np.random.default_rng(2).standard_normal((1, 1, 7, 8, 9)))
+    P = G.beijing_default_vectors(3, 4, ds, 30.0, 50.0, (0.25, 0.3))
+    g = G.beijing(P, vs, ds)
+    a = cone.bj_forward(vol, P, vs, ds)
+    assert float((a - _table_forward(g, vol)).abs().max()) < 1e-12
+
+
+def test_circular_table_matches_native_units():
+    vs, ds = (9, 8, 7), (11, 6)
+    ang = np.linspace(0, 360, 8, False) + 1.5
+    vol = torch.from_numpy(np.random.default_rng(3).standard_normal((1, 1, 7, 8, 9)))
+    g = G.circular(ang, vs, 0.7, ds, (0.9, 1.1), 20.0, 35.0)
+    a = cone.circ_forward(vol, ang, vs, 0.7, ds, (0.9, 1.1), 20.0, 35.0)
+    assert float((a - _table_forward(g, vol)).abs().max()) < 1e-12
+
+
+def test_subset_views():
+    g = configs.fd_32()
+    s = g.subset([3, 5])
+    assert s.n_views == 2 and np.array_equal(s.views, g.views[[3, 5]])
+
+
+def _header_symbols():
+    txt = open(os.path.join(ROOT, "include", "dtvproj.h")).read()
+    return sorted(set(re.findall(r"\b(dtv_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_library_loads_and_exports_header_symbols():
+    lib = _capi.lib()
+    syms = _header_symbols()
+    assert len(syms) >= 14
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_capi.EXPORTED)
+    assert "gfx950" in _capi.version()
+
+
+def test_host_view_builders_match_python_geometry():
+    vs, ds = (37, 29, 21), (45, 27)
+    ang = np.linspace(0, 360, 13, False) + 0.3
+    views, sc = ops.host_views_standard(ang, vs, ds, 700.0, 1300.0)
+    g = G.standard(ang, vs, ds, 700.0, 1300.0)
+    np.testing.assert_allclose(views, g.views, rtol=0, atol=1e-9)
+    assert sc.axis_mode == g.scheme.axis_mode and sc.margin == pytest.approx(g.scheme.margin)
+    assert sc.scale == pytest.approx(g.scheme.scale)
+    assert list(sc.box_lo) == pytest.approx(list(g.scheme.box_lo))
+    P = G.beijing_default_vectors(2, 21, ds, 300.0, 500.0, (0.5, 0.6))
+    views, sc = ops.host_views_beijing(P, vs, ds)
+    gb = G.beijing(P, vs, ds)
+    np.testing.assert_allclose(views, gb.views, rtol=0, atol=1e-9)
+    assert list(sc.box_hi) == list(gb.scheme.box_hi)
+
+
+def test_capi_rejects_bad_arguments_without_gpu():
+    lib = _capi.lib()
+    out = np.zeros((1, 12))
+    sc = _capi.DtvScheme()
+    a = np.zeros(1)
+    cp = a.ctypes.data_as(_capi.c_double_p)
+    op = out.ctypes.data_as(_capi.c_double_p)
+    # zero-sized volume, sdd <= sid, null outputs
+    assert lib.dtv_views_standard(cp, 1, _capi.int3((0, 4, 4)), _capi.int2((4, 4)), 1.0, 2.0, op,
+                                  ctypes.byref(sc)) == 1
+    assert lib.dtv_views_standard(cp, 1, _capi.int3((4, 4, 4)), _capi.int2((4, 4)), 2.0, 1.0, op,
+                                  ctypes.byref(sc)) == 1
+    assert lib.dtv_views_standard(cp, 1, _capi.int3((4, 4, 4)), _capi.int2((4, 4)), 1.0, 2.0, None,
+                                  ctypes.byref(sc)) == 2
+    # the GPU entry points validate before touching the device
+    assert lib.dtv_cone_forward(None, None, 1, _capi.int3((4, 4, 4)), _capi.int2((4, 4)), None, 1,
+                                ctypes.byref(sc), None) == 2
+    assert lib.dtv_beijing_backward(ctypes.c_void_p(16), ctypes.c_void_p(16), 1,
+                                    _capi.int3((4, 4, 4)), _capi.int2((4, 4)),
+                                    ctypes.c_void_p(16), 20, -1, None) == 5
+    assert "multiple of 21" in lib.dtv_strerror(5).decode()
+
+
+def test_ops_require_device_tensors():
+    g = configs.fd_32()
+    with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
+        ops.forward(torch.zeros(g.volume_shape(1)), g)
+    with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
+        ops.adjoint(torch.zeros(g.sino_shape(1)), g)
+
+
+def test_jit_bindings_reference_error_messages():
+    import JITBeijingGeometry as JB
+    import JITStandardGeometry as JS
+    with pytest.raises(RuntimeError, match="volume must be a CUDA tensor"):
+        JS.forward(torch.zeros(1, 1, 4, 4, 4), torch.zeros(3, dtype=torch.float64),
+                   torch.tensor([4, 4, 4]), torch.tensor([4, 4]), 1.0, 2.0, 0)
+    with pytest.raises(RuntimeError, match="sino must be a CUDA tensor"):
+        JS.backward(torch.zeros(1, 1, 3, 4, 4), torch.zeros(3, dtype=torch.float64),
+                    torch.tensor([4, 4, 4]), torch.tensor([4, 4]), 1.0, 2.0, 0)
+    with pytest.raises(RuntimeError, match="volume must be a CUDA tensor"):
+        JB.forward(torch.zeros(1, 1, 4, 4, 4), torch.tensor([4, 4, 4]), torch.tensor([4, 4]),
+                   torch.zeros(21, 12), 0)
+
+
+def test_sample_count_matches_oracle_enumeration():
+    g = configs.fd_32()
+    sc = g.scheme
+    r = cone.table_rays(g.views, g.detector_size, sc.axis_mode, sc.box_lo, sc.box_hi, sc.margin,
+                        sc.scale)
+    assert roofline.sample_count(g) == cone.sample_count(r, g.volume_size)
+    ang = G.uniform_angles(8)
+    gs = G.standard(ang, (40, 36, 20), (50, 16), 100.0, 200.0)
+    rs = cone.std_rays(ang, (40, 36, 20), (50, 16), 100.0, 200.0)
+    assert roofline.sample_count(gs) == cone.sample_count(rs, (40, 36, 20))
+
+
+def test_pair_bytes_accounting():
+    g = configs.fd_32()
+    b = roofline.pair_bytes(g, samples=1000)
+    assert b["forward"] == 32 * 1000 + 4 * g.n_rays
+    assert b["adjoint"] == 32 * 1000 + 4 * 32 ** 3
+    assert b["pair"] == b["forward"] + b["adjoint"]
