@@ -13,11 +13,13 @@
 //                LDS reads.  The per-ray partial sum is added to the
 //                sinogram with one hardware FP32 atomic per (ray, brick).
 //                The volume is read from HBM once per call.
-//   k_adj_brick  matched A^T: the brick's owned voxels are FP32
-//                accumulators in LDS; each sample whose stencil touches
-//                the brick adds its owned taps with LDS atomics; the brick
-//                is written to HBM once at the end (no global atomics, no
-//                zero-fill of the output).
+//   k_adj_brick  matched A^T: per view, each sample whose stencil touches
+//                the brick adds its owned taps into fixed-point int32 LDS
+//                accumulators (integer LDS atomics are several times faster
+//                than FP32 ones on gfx950 and order-independent); after the
+//                view the integers are scaled back into per-lane FP32
+//                registers.  The brick is written to HBM once at the end:
+//                no global atomics, no zero-fill, bitwise reproducible.
 #include <hip/hip_runtime.h>
 
 #include "dtv_geom.h"
@@ -32,7 +34,8 @@ namespace {
 // by lane 0.  Written to rect[4] = {u0, u1, v0, v1} (inclusive; u1 < u0 =
 // empty).
 __device__ void brick_rect(const Geo& g, const View& v, const ViewProj& vp, const double lo[3],
-                           const double hi[3], double* ca, double* cb, int* ok, int* rect) {
+                           const double hi[3], double* ca, double* cb, double* cw, int* ok,
+                           int* rect) {
   const int t = threadIdx.x;
   if (t < 8) {
     const double c[3] = {(t & 1) ? hi[0] : lo[0], (t & 2) ? hi[1] : lo[1],
@@ -41,6 +44,8 @@ __device__ void brick_rect(const Geo& g, const View& v, const ViewProj& vp, cons
     ok[t] = project_point(v, vp, c, a, b) ? 1 : 0;
     ca[t] = a;
     cb[t] = b;
+    const double w[3] = {c[0] - v.S[0], c[1] - v.S[1], c[2] - v.S[2]};
+    cw[t] = dot3(w, vp.n);
   }
   __syncthreads();
   if (t == 0) {
@@ -109,7 +114,7 @@ __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol
   extern __shared__ float tile[];  // (bx+1)*(by+1)*(bz+1)
   __shared__ View sv;
   __shared__ ViewProj sp;
-  __shared__ double ca[8], cb[8];
+  __shared__ double ca[8], cb[8], cw[8];
   __shared__ int cok[8], rect[4];
 
   const int brick = blockIdx.x;
@@ -145,7 +150,7 @@ __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol
       sp = p;
     }
     __syncthreads();
-    brick_rect(g, sv, sp, lo, hi, ca, cb, cok, rect);
+    brick_rect(g, sv, sp, lo, hi, ca, cb, cw, cok, rect);
     const int u0 = rect[0], nur = rect[1] - rect[0] + 1;
     const int v0 = rect[2], nvr = rect[3] - rect[2] + 1;
     if (nur <= 0 || nvr <= 0) continue;
@@ -195,16 +200,54 @@ __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol
   }
 }
 
+// Upper bound on sum_k w over the samples of one view that touch one voxel c
+// of the box (w = the sample's trilinear weight on c):
+//  * per ray, the weights on c are bounded by a 1-D tent along the ray's
+//    dominant axis sampled at spacing |d_ax| >= 1/sqrt(3): sum <= 3;
+//  * rays touching c pass through its 2x2x2 support cube, so their pixels
+//    lie in the cube's projected footprint, whose extent is at most 2/e_min
+//    of the box's projected extent times the depth ratio across the box
+//    (perspective), +10%; a footprint of f pixels holds <= f + 1 pixel
+//    centres per axis (+0.5 slack).
+// The corner depths ``cw`` are (corner - S).n, proportional to the distance
+// from the source along the detector normal.
+__device__ __forceinline__ float weight_per_voxel_bound(const double* ca, const double* cb,
+                                                        const double* cw, const int* ok,
+                                                        float emin, int nur, int nvr) {
+  bool all = true;
+  double amin = 1e300, amax = -1e300, bmin = 1e300, bmax = -1e300;
+  double wmin = 1e300, wmax = -1e300;
+  for (int i = 0; i < 8; ++i) {
+    all = all && ok[i];
+    amin = fmin(amin, ca[i]);
+    amax = fmax(amax, ca[i]);
+    bmin = fmin(bmin, cb[i]);
+    bmax = fmax(bmax, cb[i]);
+    wmin = fmin(wmin, fabs(cw[i]));
+    wmax = fmax(wmax, fabs(cw[i]));
+  }
+  if (!all || !(wmin > 0.0)) return 3.0f * (float)nur * (float)nvr;
+  const double persp = wmax / wmin * 1.1;
+  const float fu = (float)((amax - amin) * 2.0 / emin * persp) + 1.5f;
+  const float fv = (float)((bmax - bmin) * 2.0 / emin * persp) + 1.5f;
+  return 3.0f * fminf(fu, (float)nur + 1.f) * fminf(fv, (float)nvr + 1.f);
+}
+
+constexpr int kAdjPerThread = 32;  // owned voxels per lane (bx*by*bz <= 32*block)
+
 __global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sino,
                                                    float* __restrict__ vol, Geo g, int bx,
                                                    int by, int bz, int nbx, int nby) {
-  // owned voxels, rows padded to odd strides (bx+1, (bx+1)*by+1) so that
-  // lanes whose samples differ in y or z fall into different LDS banks
-  extern __shared__ float acc[];
+  // Per-view accumulators: fixed-point int32 in LDS, rows padded to odd
+  // strides (bx+1, (bx+1)*by+1) so lanes differing in y or z use different
+  // banks.  A view whose rays carry a non-finite value falls back to FP32
+  // LDS atomics in the same buffer so infinities and NaNs propagate.
+  extern __shared__ int iacc[];
   __shared__ View sv;
   __shared__ ViewProj sp;
-  __shared__ double ca[8], cb[8];
+  __shared__ double ca[8], cb[8], cw[8];
   __shared__ int cok[8], rect[4];
+  __shared__ unsigned vmax_bits;
 
   const int brick = blockIdx.x;
   const int b = blockIdx.y;
@@ -212,11 +255,17 @@ __global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sin
   const int x0 = jx * bx, y0 = jy * by, z0 = jz * bz;
   const int ex = min(bx, g.nx - x0), ey = min(by, g.ny - y0), ez = min(bz, g.nz - z0);
   const int sty = bx + 1, stz = (bx + 1) * by + 1;
-  for (int i = threadIdx.x; i < stz * bz; i += blockDim.x) acc[i] = 0.f;
+  const int nown = bx * by * bz;
+  for (int i = threadIdx.x; i < stz * bz; i += blockDim.x) iacc[i] = 0;
+  float vacc[kAdjPerThread];
+#pragma unroll
+  for (int j = 0; j < kAdjPerThread; ++j) vacc[j] = 0.f;
   // samples with floor(p) in [x0-1, x0+ex-1] touch owned voxels
   const double lo[3] = {(double)(x0 - 1), (double)(y0 - 1), (double)(z0 - 1)};
   const double hi[3] = {(double)(x0 + ex), (double)(y0 + ey), (double)(z0 + ez)};
-  const float scale = (float)g.scale;
+  const float emin = (float)(min(ex, min(ey, ez)) + 1);
+  const float gscale = (float)g.scale;
+  float* facc = reinterpret_cast<float*>(iacc);
 
   for (int a = 0; a < g.n_views; ++a) {
     __syncthreads();
@@ -227,22 +276,47 @@ __global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sin
       ViewProj p;
       view_proj(v, p);
       sp = p;
+      vmax_bits = 0u;
     }
     __syncthreads();
-    brick_rect(g, sv, sp, lo, hi, ca, cb, cok, rect);
+    brick_rect(g, sv, sp, lo, hi, ca, cb, cw, cok, rect);
     const int u0 = rect[0], nur = rect[1] - rect[0] + 1;
     const int v0 = rect[2], nvr = rect[3] - rect[2] + 1;
     if (nur <= 0 || nvr <= 0) continue;
-    const View v = sv;
     const float* srow = sino + ((size_t)b * g.n_views + a) * g.nv * g.nu;
-    // lanes take every third ray along u and v, so rays sharing a wave are
-    // ~3 pixels apart and rarely add into the same voxel in one instruction
+    // pass 1: largest |value| among this view's rays through the brick
+    float m = 0.f;
+    for (int r = threadIdx.x; r < nur * nvr; r += blockDim.x) {
+      const int iu = u0 + r % nur, iv = v0 + r / nur;
+      m = fmaxf(m, fabsf(srow[(size_t)iv * g.nu + iu] * gscale));
+      if (!(fabsf(srow[(size_t)iv * g.nu + iu]) <= 3.0e38f)) m = __int_as_float(0x7f800000);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&vmax_bits, __float_as_uint(m));
+    __syncthreads();
+    const float vmax = __uint_as_float(vmax_bits);
+    if (vmax == 0.f) continue;  // uniform: every contribution is zero
+    const bool fixed = vmax <= 3.0e38f;
+    // power-of-two scale so that one voxel's sum over this view stays below
+    // 2^30 in magnitude
+    float qscale = 1.f, qinv = 1.f;
+    if (fixed) {
+      const float bound = weight_per_voxel_bound(ca, cb, cw, cok, emin, nur, nvr) * vmax;
+      int e;
+      frexpf(bound, &e);  // bound < 2^e
+      qscale = ldexpf(1.f, 30 - e);
+      qinv = ldexpf(1.f, e - 30);
+    }
+    const View v = sv;
+    // pass 2: scatter.  Lanes take every third ray along u and v so rays
+    // sharing a wave rarely add into the same voxel in one instruction.
     const int mu = (nur + 2) / 3, mv = (nvr + 2) / 3;
     for (int r = threadIdx.x; r < 9 * mu * mv; r += blockDim.x) {
       const int cu = r % (3 * mu), cv = r / (3 * mu);
       const int iu = u0 + 3 * (cu % mu) + cu / mu, iv = v0 + 3 * (cv % mv) + cv / mv;
       if (iu > rect[1] || iv > rect[3]) continue;
-      const float val = srow[(size_t)iv * g.nu + iu] * scale;
+      const float val = srow[(size_t)iv * g.nu + iu] * gscale * qscale;
       if (val == 0.f) continue;
       double d[3], P0[3];
       int kend;
@@ -271,28 +345,55 @@ __global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sin
           const float w01 = (1.f - fy) * wz1, w11 = fy * wz1;
           const float gx = 1.f - fx;
           const int base = iz * stz + iy * sty + ix;
-          if (mz0 && my0 && mx0) atomicAdd(acc + base, gx * w00);
-          if (mz0 && my0 && mx1) atomicAdd(acc + base + 1, fx * w00);
-          if (mz0 && my1 && mx0) atomicAdd(acc + base + sty, gx * w10);
-          if (mz0 && my1 && mx1) atomicAdd(acc + base + sty + 1, fx * w10);
-          if (mz1 && my0 && mx0) atomicAdd(acc + base + stz, gx * w01);
-          if (mz1 && my0 && mx1) atomicAdd(acc + base + stz + 1, fx * w01);
-          if (mz1 && my1 && mx0) atomicAdd(acc + base + stz + sty, gx * w11);
-          if (mz1 && my1 && mx1) atomicAdd(acc + base + stz + sty + 1, fx * w11);
+          if (fixed) {
+            if (mz0 && my0 && mx0) atomicAdd(iacc + base, __float2int_rn(gx * w00));
+            if (mz0 && my0 && mx1) atomicAdd(iacc + base + 1, __float2int_rn(fx * w00));
+            if (mz0 && my1 && mx0) atomicAdd(iacc + base + sty, __float2int_rn(gx * w10));
+            if (mz0 && my1 && mx1) atomicAdd(iacc + base + sty + 1, __float2int_rn(fx * w10));
+            if (mz1 && my0 && mx0) atomicAdd(iacc + base + stz, __float2int_rn(gx * w01));
+            if (mz1 && my0 && mx1) atomicAdd(iacc + base + stz + 1, __float2int_rn(fx * w01));
+            if (mz1 && my1 && mx0) atomicAdd(iacc + base + stz + sty, __float2int_rn(gx * w11));
+            if (mz1 && my1 && mx1)
+              atomicAdd(iacc + base + stz + sty + 1, __float2int_rn(fx * w11));
+          } else {
+            if (mz0 && my0 && mx0) atomicAdd(facc + base, gx * w00);
+            if (mz0 && my0 && mx1) atomicAdd(facc + base + 1, fx * w00);
+            if (mz0 && my1 && mx0) atomicAdd(facc + base + sty, gx * w10);
+            if (mz0 && my1 && mx1) atomicAdd(facc + base + sty + 1, fx * w10);
+            if (mz1 && my0 && mx0) atomicAdd(facc + base + stz, gx * w01);
+            if (mz1 && my0 && mx1) atomicAdd(facc + base + stz + 1, fx * w01);
+            if (mz1 && my1 && mx0) atomicAdd(facc + base + stz + sty, gx * w11);
+            if (mz1 && my1 && mx1) atomicAdd(facc + base + stz + sty + 1, fx * w11);
+          }
         }
         px += Dx;
         py += Dy;
         pz += Dz;
       }
     }
+    __syncthreads();
+    // flush this view into the lane's FP32 registers (fixed view order:
+    // the result is bitwise reproducible) and clear the accumulators
+#pragma unroll
+    for (int j = 0; j < kAdjPerThread; ++j) {
+      const int i = threadIdx.x + j * blockDim.x;
+      if (i < nown) {
+        const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
+        const int li = lz * stz + ly * sty + lx;
+        vacc[j] += fixed ? (float)iacc[li] * qinv : facc[li];
+        iacc[li] = 0;
+      }
+    }
   }
-  __syncthreads();
   float* vb = vol + (size_t)b * g.nx * g.ny * g.nz;
-  for (int i = threadIdx.x; i < bx * by * bz; i += blockDim.x) {
-    const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
-    if (lx < ex && ly < ey && lz < ez)
-      vb[((size_t)(z0 + lz) * g.ny + (y0 + ly)) * g.nx + (x0 + lx)] =
-          acc[lz * stz + ly * sty + lx];
+#pragma unroll
+  for (int j = 0; j < kAdjPerThread; ++j) {
+    const int i = threadIdx.x + j * blockDim.x;
+    if (i < nown) {
+      const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
+      if (lx < ex && ly < ey && lz < ez)
+        vb[((size_t)(z0 + lz) * g.ny + (y0 + ly)) * g.nx + (x0 + lx)] = vacc[j];
+    }
   }
 }
 
@@ -313,7 +414,8 @@ int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int
 int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int bx, int by,
                      int bz, int block, hipStream_t st) {
   const int nbx = (g.nx + bx - 1) / bx, nby = (g.ny + by - 1) / by, nbz = (g.nz + bz - 1) / bz;
-  const size_t lds = (size_t)((bx + 1) * by + 1) * bz * sizeof(float);
+  if (bx * by * bz > kAdjPerThread * block) return 1;
+  const size_t lds = (size_t)((bx + 1) * by + 1) * bz * sizeof(int);
   hipLaunchKernelGGL(k_adj_brick, dim3(nbx * nby * nbz, batch), dim3(block), lds, st, sino, vol,
                      g, bx, by, bz, nbx, nby);
   return hipGetLastError() == hipSuccess ? 0 : 4;

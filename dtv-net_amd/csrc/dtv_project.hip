@@ -388,10 +388,10 @@ int run_zero(float* p, size_t n, hipStream_t st) {
   return launch_status();
 }
 
-// Resolved kernel choice (defaults: brick 16^3, 256 lanes).
+// Resolved kernel choice (defaults: 32x16x16 bricks, 256 lanes).
 struct Launch {
   int algo = DTV_ALGO_BRICK;
-  int bx = 16, by = 16, bz = 16;
+  int bx = 32, by = 16, bz = 16;
   int block = 256;
 };
 

@@ -169,15 +169,16 @@ def test_batch_independence(cuda):
 
 
 def test_repeatability(cuda):
-    """The ray-driven gather is bitwise repeatable; the brick kernels add
-    partial sums with FP32 atomics, so repeated calls may differ in the last
-    bits only (bounded here at 1e-6 relative)."""
+    """The ray-driven gather and the brick adjoint (integer accumulation) are
+    bitwise repeatable; the brick forward adds per-brick partial sums with
+    FP32 atomics, so repeated calls may differ in the last bits only."""
     g = configs.fd_32()
     x = phantom.gaussian((1, 1, 32, 32, 32), 7).to(cuda)
     assert torch.equal(ops.forward(x, g, RAY), ops.forward(x, g, RAY))
     assert rel_l2(ops.forward(x, g), ops.forward(x, g)) <= 1e-6
+    # the brick adjoint adds integers: bitwise repeatable
     y = phantom.gaussian(g.sino_shape(1), 8).to(cuda)
-    assert rel_l2(ops.adjoint(y, g), ops.adjoint(y, g)) <= 1e-6
+    assert torch.equal(ops.adjoint(y, g), ops.adjoint(y, g))
 
 
 # ---- kernel variants agree ---------------------------------------------------
@@ -203,7 +204,10 @@ def test_brick_vs_ray_and_oracle(cuda, launch):
     ar = ops.adjoint(y.to(cuda), g, RAY)
     ab = ops.adjoint(y.to(cuda), g, launch)
     assert rel_l2(fb, fr) <= 1e-6
-    assert rel_l2(ab, ar) <= 1e-6
+    # the brick adjoint accumulates each view in 32-bit fixed point (scale
+    # bounded so no voxel can overflow); its rounding stays below 5e-6 even
+    # at this near-source geometry
+    assert rel_l2(ab, ar) <= 5e-6
     ref_f = cone.circ_forward(x, ang, vs, 0.9, ds, 1.1, 60.0, 130.0)
     ref_a = cone.circ_adjoint(y, ang, vs, 0.9, ds, 1.1, 60.0, 130.0)
     assert rel_l2(fb, ref_f) <= TOL
@@ -219,4 +223,4 @@ def test_standard_quirk_brick_matches_ray(cuda):
     x = phantom.gaussian((1, 1, 24, 40, 48), 33).to(cuda)
     assert rel_l2(ops.forward(x, g, BRICKS[0]), ops.forward(x, g, RAY)) <= 1e-6
     y = phantom.gaussian(g.sino_shape(1), 34).to(cuda)
-    assert rel_l2(ops.adjoint(y, g, BRICKS[0]), ops.adjoint(y, g, RAY)) <= 1e-6
+    assert rel_l2(ops.adjoint(y, g, BRICKS[0]), ops.adjoint(y, g, RAY)) <= 5e-6
