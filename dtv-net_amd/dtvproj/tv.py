@@ -1,0 +1,173 @@
+"""Directional finite differences, total variation and the fused update step.
+
+Torch operators (differentiable, any device) restate the reference's sparse
+difference matrices (``model/PDHG/Utils/Gradients.py:8-47``) without building
+them: for C-order flattened volumes (x fastest) D_a f = f[i + s_a] - f[i]
+with
+
+* ``mode="reference"``: s = (1, Y, X*Y) and the reference's rows -- D_x has
+  no +1 where (i+1) % X == 0, D_y strides by Y (= volumeSize[1]) over the
+  whole flat array (crossing slice ends), D_z stops at the last slice; rows
+  without a +1 give -f[i];
+* ``mode="neumann"``: conventional forward differences, zero at the far
+  boundary of each axis.
+
+``fista_tv_step`` runs the fused gfx950 kernel that, per voxel, evaluates the
+gradient of the smoothed total variation TV_eps(y) = sum_c sqrt(|D y|_c^2 +
+eps^2), takes the step x = y - L (g + lam grad TV_eps(y)), clamps at zero if
+asked and applies the momentum y' = x + beta (x - x_prev) -- one HBM pass.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _capi
+from .ops import _stream, check_input
+
+MODES = {"neumann": 0, "reference": 1}
+
+
+def _flat(f):
+    shp = f.shape
+    return f.reshape(*shp[:-3], shp[-3] * shp[-2] * shp[-1]), shp
+
+
+def D(f: torch.Tensor, axis: str, mode: str = "neumann") -> torch.Tensor:
+    """Forward difference along ``axis`` of f [..., Z, Y, X]."""
+    Z, Y, X = f.shape[-3:]
+    if mode == "neumann":
+        dim = {"x": -1, "y": -2, "z": -3}[axis]
+        d = torch.diff(f, dim=dim)
+        pad = [0, 0] * 3
+        pad[2 * (-dim - 1) + 1] = 1
+        return torch.nn.functional.pad(d, pad)
+    if mode != "reference":
+        raise ValueError(mode)
+    ff, shp = _flat(f)
+    N = ff.shape[-1]
+    out = -ff
+    if axis == "x":
+        nxt = torch.zeros_like(ff)
+        nxt[..., :-1] = ff[..., 1:]
+        mask = (torch.arange(N, device=f.device) + 1) % X != 0
+        out = out + nxt * mask
+    elif axis == "y":
+        out = torch.cat([ff[..., Y:] - ff[..., :N - Y], -ff[..., N - Y:]], -1)
+    elif axis == "z":
+        s = X * Y
+        out = torch.cat([ff[..., s:] - ff[..., :N - s], -ff[..., N - s:]], -1)
+    else:
+        raise ValueError(axis)
+    return out.reshape(shp)
+
+
+def DT(p: torch.Tensor, axis: str, mode: str = "neumann") -> torch.Tensor:
+    """Transpose of :func:`D` applied to p [..., Z, Y, X]."""
+    Z, Y, X = p.shape[-3:]
+    if mode == "neumann":
+        dim = {"x": -1, "y": -2, "z": -3}[axis]
+        n = p.shape[dim]
+        head = p.narrow(dim, 0, n - 1)
+        out = torch.nn.functional.pad(head, _pad_for(dim, front=1)) - \
+            torch.nn.functional.pad(head, _pad_for(dim, back=1))
+        return out
+    pp, shp = _flat(p)
+    N = pp.shape[-1]
+    out = -pp
+    if axis == "x":
+        mask = (torch.arange(N - 1, device=p.device) + 1) % X != 0
+        prev = torch.zeros_like(pp)
+        prev[..., 1:] = pp[..., :-1] * mask
+        out = out + prev
+    elif axis == "y":
+        out = torch.cat([-pp[..., :Y], pp[..., :N - Y] - pp[..., Y:]], -1)
+    elif axis == "z":
+        s = X * Y
+        out = torch.cat([-pp[..., :s], pp[..., :N - s] - pp[..., s:]], -1)
+    else:
+        raise ValueError(axis)
+    return out.reshape(shp)
+
+
+def _pad_for(dim, front=0, back=0):
+    pad = [0, 0] * 3
+    k = -dim - 1
+    pad[2 * k] = front
+    pad[2 * k + 1] = back
+    return pad
+
+
+def grad3(f, mode="neumann"):
+    return D(f, "x", mode), D(f, "y", mode), D(f, "z", mode)
+
+
+def tv_eps(f, eps, mode="neumann"):
+    dx, dy, dz = grad3(f, mode)
+    return torch.sqrt(dx * dx + dy * dy + dz * dz + eps * eps).sum()
+
+
+def tv_eps_grad(f, eps, mode="neumann"):
+    """Gradient of :func:`tv_eps` (explicit, no autograd)."""
+    dx, dy, dz = grad3(f, mode)
+    n = torch.sqrt(dx * dx + dy * dy + dz * dz + eps * eps)
+    return DT(dx / n, "x", mode) + DT(dy / n, "y", mode) + DT(dz / n, "z", mode)
+
+
+def shrink(v, thr):
+    """Soft threshold sign(v) max(|v| - thr, 0)."""
+    return torch.sign(v) * torch.clamp(v.abs() - thr, min=0)
+
+
+# --------------------------------------------------------------------------
+# Fused gfx950 step
+# --------------------------------------------------------------------------
+
+class TvItem(ctypes.Structure):
+    """One instance of a batched fused step (mirrors dtv_tv_item)."""
+    _fields_ = [
+        ("y", ctypes.c_void_p), ("g", ctypes.c_void_p), ("xprev", ctypes.c_void_p),
+        ("xnew", ctypes.c_void_p), ("ynew", ctypes.c_void_p),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("nonneg", ctypes.c_int32),
+        ("step", ctypes.c_float), ("lam", ctypes.c_float), ("eps", ctypes.c_float),
+        ("beta", ctypes.c_float),
+    ]
+
+
+def fista_tv_step(y, g, xprev, step, lam, eps, beta, nonneg=True, mode="neumann",
+                  xnew=None, ynew=None):
+    """x = [y - step (g + lam grad TV_eps(y))]_+ ; y' = x + beta (x - xprev).
+
+    All volumes [B, 1, Z, Y, X] FP32 on the GPU; returns (x, y')."""
+    for t, n in ((y, "y"), (g, "g"), (xprev, "xprev")):
+        check_input(t, n)
+        if t.dtype != torch.float32 or t.shape != y.shape:
+            raise RuntimeError(f"{n} must be float32 with the shape of y")
+    xnew = torch.empty_like(y) if xnew is None else xnew
+    ynew = torch.empty_like(y) if ynew is None else ynew
+    for o, n in ((xnew, "xnew"), (ynew, "ynew")):
+        check_input(o, n)
+        if o.shape != y.shape or o.dtype != torch.float32:
+            raise RuntimeError(f"{n} must be float32 with the shape of y")
+        if o.data_ptr() == y.data_ptr() or o.data_ptr() == g.data_ptr():
+            raise RuntimeError(f"{n} must not share storage with y or g (stencil reads)")
+    Z, Y, X = y.shape[-3:]
+    B = y.numel() // (X * Y * Z)
+    with torch.cuda.device(y.device):
+        rc = _capi.lib().dtv_tv_fista_step(
+            y.data_ptr(), g.data_ptr(), xprev.data_ptr(), xnew.data_ptr(), ynew.data_ptr(), B,
+            _capi.int3((X, Y, Z)), ctypes.c_float(step), ctypes.c_float(lam),
+            ctypes.c_float(eps), ctypes.c_float(beta), int(bool(nonneg)), MODES[mode],
+            _stream(y.device))
+    _capi.check(rc, "dtv_tv_fista_step")
+    return xnew, ynew
+
+
+def fista_tv_step_torch(y, g, xprev, step, lam, eps, beta, nonneg=True, mode="neumann"):
+    """Unfused torch formulation of :func:`fista_tv_step` (the plain path)."""
+    x = y - step * (g + lam * tv_eps_grad(y, eps, mode))
+    if nonneg:
+        x = torch.clamp(x, min=0)
+    return x, x + beta * (x - xprev)

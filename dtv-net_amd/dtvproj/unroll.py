@@ -1,0 +1,137 @@
+"""Reconstruction unrolls around the projector pair.
+
+* :func:`power_norm` -- ||A||^2 by power iteration on A^T A (step sizes for
+  the matched pair; the reference hard-codes steps tuned for its FDK
+  backprojector, e.g. L = 1e-4 at FISTA.py:14).
+* :func:`fista_tv` -- FISTA on 1/2 ||A x - b||^2 + lam TV_eps(x), x >= 0,
+  with the fused gfx950 update (``fused=True``) or the plain torch update
+  (the per-volume reference-style path).  Same iteration as
+  model/FISTA/FISTA.py:16-27 (gradient step, proximal/shrink step,
+  t-momentum) with the smoothed TV gradient in place of the l1 shrink.
+* :class:`PDHGDTVNet` -- learned primal-dual (Chambolle-Pock) unroll for
+  directional TV, following model/PDHG/DTVCP.py:29-60 (data dual
+  w = (w + sigma res)/(1 + sigma), directional duals, primal step,
+  over-relaxation f_bar = 2 f - f_old) with learned per-iteration sigma,
+  tau and per-axis TV weights; the exact dual prox of lam ||D_a f||_1 (a
+  clip to [-lam, lam]) replaces the reference's Newton l1-ball projection.
+  Each iteration runs under torch.utils.checkpoint to cap activation memory.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+import torch.nn.functional as F
+from torch.utils.checkpoint import checkpoint
+
+from . import autograd as _ag
+from . import ops, tv
+from .geometry import ConeGeometry
+
+
+@torch.no_grad()
+def power_norm(geom: ConeGeometry, iters: int = 20, seed: int = 0, device="cuda") -> float:
+    """Largest eigenvalue of A^T A (= ||A||^2) by power iteration."""
+    gen = torch.Generator().manual_seed(seed)
+    x = torch.rand(geom.volume_shape(1), generator=gen).to(device)
+    x /= x.norm()
+    lam = 0.0
+    for _ in range(iters):
+        y = ops.adjoint(ops.forward(x, geom), geom)
+        lam = float(y.norm())
+        x = y / lam
+    return lam
+
+
+def fista_momentum(n: int):
+    """beta_k = (t_k - 1) / t_{k+1} with t_1 = 1, t_{k+1} = (1 + sqrt(1 + 4 t_k^2)) / 2
+    (FISTA.py:23-25)."""
+    t = 1.0
+    out = []
+    for _ in range(n):
+        tn = (1.0 + math.sqrt(1.0 + 4.0 * t * t)) / 2.0
+        out.append((t - 1.0) / tn)
+        t = tn
+    return out
+
+
+@torch.no_grad()
+def fista_tv(b, geom: ConeGeometry, iters: int, lam: float, eps: float, step: float = None,
+             x0=None, fused: bool = True, mode: str = "neumann", lip: float = None):
+    """FISTA-TV reconstruction of sinogram ``b`` [B,1,A,V,U]; returns x."""
+    B = b.shape[0]
+    if step is None:
+        if lip is None:
+            lip = power_norm(geom, device=b.device)
+        # gradient Lipschitz: ||A||^2 + lam ||D||^2 / eps, ||D||^2 <= 12 in 3-D
+        step = 1.0 / (lip + lam * 12.0 / eps)
+    x = torch.zeros(geom.volume_shape(B), device=b.device) if x0 is None else x0.clone()
+    y = x.clone()
+    ybuf = torch.empty_like(x)
+    for beta in fista_momentum(iters):
+        r = ops.forward(y, geom)
+        r -= b
+        g = ops.adjoint(r, geom)
+        if fused:
+            x, ynew = tv.fista_tv_step(y, g, x, step, lam, eps, beta, True, mode,
+                                       xnew=x, ynew=ybuf)
+            y, ybuf = ynew, y
+        else:
+            x, y = tv.fista_tv_step_torch(y, g, x, step, lam, eps, beta, True, mode)
+    return x
+
+
+class PDHGDTVNet(torch.nn.Module):
+    """Learned ``iters``-iteration PDHG for directional TV with per-iteration
+    sigma_k, tau_k and per-axis weights lam_{a,k} (softplus-positive)."""
+
+    def __init__(self, geom: ConeGeometry, iters: int = 10, sigma: float = 0.5,
+                 tau: float = 0.5, lam=(1e-2, 1e-2, 1e-2), norm: float = 1.0,
+                 mode: str = "neumann", checkpointing: bool = True, project=None,
+                 backproject=None):
+        super().__init__()
+        self.geom = geom
+        # differentiable A and A^T; defaults are the gfx950 matched pair
+        self.A = project or (lambda x: _ag.project(x, geom))
+        self.AT = backproject or (lambda y: _ag.backproject(y, geom))
+        self.iters = iters
+        self.mode = mode
+        self.checkpointing = checkpointing
+        # sigma * tau * ||K||^2 < 1 with K = [A; D]; ||D||^2 <= 12
+        self.knorm = math.sqrt(norm + 12.0)
+        inv = lambda v: math.log(math.expm1(v))  # softplus^-1
+        self.sigma = torch.nn.Parameter(torch.full((iters,), inv(sigma)))
+        self.tau = torch.nn.Parameter(torch.full((iters,), inv(tau)))
+        self.lam = torch.nn.Parameter(torch.tensor([[inv(l) for l in lam]] * iters))
+
+    def _iteration(self, k, f, fbar, w, px, py, pz):
+        s = F.softplus(self.sigma[k]) / self.knorm
+        t = F.softplus(self.tau[k]) / self.knorm
+        lam = F.softplus(self.lam[k])
+        res = self.A(fbar) - self._b
+        w = (w + s * res) / (1 + s)
+        px = torch.clamp(px + s * tv.D(fbar, "x", self.mode), -lam[0], lam[0])
+        py = torch.clamp(py + s * tv.D(fbar, "y", self.mode), -lam[1], lam[1])
+        pz = torch.clamp(pz + s * tv.D(fbar, "z", self.mode), -lam[2], lam[2])
+        grad = self.AT(w) + tv.DT(px, "x", self.mode) + \
+            tv.DT(py, "y", self.mode) + tv.DT(pz, "z", self.mode)
+        fn = F.relu(f - t * grad)
+        return fn, 2 * fn - f, w, px, py, pz
+
+    def forward(self, b, f0=None):
+        self._b = b
+        B = b.shape[0]
+        f = torch.zeros(self.geom.volume_shape(B), device=b.device, dtype=b.dtype) \
+            if f0 is None else f0
+        w = torch.zeros_like(b)
+        px = torch.zeros_like(f)
+        py = torch.zeros_like(f)
+        pz = torch.zeros_like(f)
+        fbar = f
+        for k in range(self.iters):
+            if self.checkpointing and torch.is_grad_enabled():
+                f, fbar, w, px, py, pz = checkpoint(self._iteration, k, f, fbar, w, px, py, pz,
+                                                    use_reentrant=False)
+            else:
+                f, fbar, w, px, py, pz = self._iteration(k, f, fbar, w, px, py, pz)
+        return f

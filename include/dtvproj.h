@@ -136,6 +136,36 @@ int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,
                         const double* views, int32_t n_views, const dtv_scheme* scheme,
                         const dtv_launch* launch, void* stream);
 
+/* ---- Fused reconstruction update ---------------------------------------- */
+
+/* One instance of the fused FISTA / smoothed-TV step (device pointers). */
+typedef struct dtv_tv_item {
+  const float* y;      /* current (momentum) point */
+  const float* g;      /* data-term gradient A^T (A y - b) */
+  const float* xprev;  /* previous iterate */
+  float* xnew;         /* out: [y - step (g + lam grad TV_eps(y))]_+ */
+  float* ynew;         /* out: xnew + beta (xnew - xprev) */
+  int32_t nx, ny, nz;
+  int32_t nonneg;
+  float step, lam, eps, beta;
+} dtv_tv_item;
+
+/* Fused step for ``batch`` equal-size volumes.  mode 0: Neumann forward
+ * differences; mode 1: the reference's difference matrices
+ * (model/PDHG/Utils/Gradients.py:8-47, incl. its D_y stride of Y).  The
+ * reference has no fused counterpart: its unrolls apply torch.sparse
+ * matrices per batch item (e.g. model/PDHG/ASDPOCS.py:38-43,
+ * model/FISTA/DTVFISTA.py:29-36). */
+int dtv_tv_fista_step(const float* y, const float* g, const float* xprev, float* xnew,
+                      float* ynew, int32_t batch, const int32_t volume_size[3], float step,
+                      float lam, float eps, float beta, int32_t nonneg, int32_t mode,
+                      void* stream);
+
+/* Same step over uneven instances in one launch; ``items`` is a device array
+ * of n_items descriptors, max_voxels the largest instance's voxel count. */
+int dtv_tv_fista_step_multi(const dtv_tv_item* items, int32_t n_items, int64_t max_voxels,
+                            int32_t mode, void* stream);
+
 /* ---- Host-only helpers (no GPU work) ---------------------------------- */
 
 /* Canonical view table and scheme of the Standard geometry, computed on the

@@ -1,0 +1,174 @@
+"""GPU: autograd pair vs FP64 finite differences, FISTA-TV and the learned
+PDHG/DTV unroll vs the same algorithms driven by the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+
+from oracle import cone
+from dtvproj import autograd, configs, geometry as G, ops, phantom, tv, unroll
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+def _oracle_ops(g):
+    """FP64 CPU A and A^T of a table geometry as differentiable functions."""
+    sc = g.scheme
+    args = (g.views, g.volume_size, g.detector_size, sc.axis_mode, sc.box_lo, sc.box_hi,
+            sc.margin, sc.scale)
+
+    class A(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x):
+            return cone.table_forward(x, *args)
+
+        @staticmethod
+        def backward(ctx, gr):
+            return cone.table_adjoint(gr, *args)
+
+    class AT(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, y):
+            return cone.table_adjoint(y, *args)
+
+        @staticmethod
+        def backward(ctx, gr):
+            return cone.table_forward(gr, *args)
+
+    return A.apply, AT.apply
+
+
+def test_projector_gradients_vs_fp64_finite_differences(cuda):
+    """d/dx 1/2 ||A x - b||^2 through the autograd Function (backward = the
+    A^T kernel) vs the FP64 oracle gradient and FP64 central differences."""
+    g = configs.fd_32()
+    _case("fd_32 gradients")
+    A64, AT64 = _oracle_ops(g)
+    x = (phantom.phantom((32, 32, 32), 1.0, 6, seed=5) + 0.05 * phantom.gaussian(
+        (1, 1, 32, 32, 32), 6)).double()
+    b = A64(x.clone()) + 0.1 * phantom.gaussian(g.sino_shape(1), 7, dtype=torch.float64)
+    xg = x.float().to(cuda).requires_grad_(True)
+    loss = 0.5 * ((autograd.project(xg, g) - b.float().to(cuda)) ** 2).sum()
+    loss.backward()
+    grad = xg.grad.double().cpu()
+    ref = AT64(A64(x) - b)
+    assert rel_l2(grad, ref) <= 1e-5
+    L = lambda z: 0.5 * float(((A64(z) - b) ** 2).sum())
+    for seed in (11, 12, 13):
+        v = phantom.gaussian(x.shape, seed, dtype=torch.float64)
+        h = 1e-3
+        fd = (L(x + h * v) - L(x - h * v)) / (2 * h)
+        an = float((grad * v).sum())
+        assert abs(fd - an) <= 1e-5 * abs(fd)
+    # and the backward of the backprojector is the A kernel
+    yg = b.float().to(cuda).requires_grad_(True)
+    c = phantom.gaussian(x.shape, 8).to(cuda)
+    (0.5 * ((autograd.backproject(yg, g) - c) ** 2).sum()).backward()
+    ref2 = A64(AT64(b) - c.double().cpu())
+    assert rel_l2(yg.grad, ref2) <= 1e-5
+
+
+def test_fista_tv_fused_plain_and_oracle():
+    """10 FISTA-TV iterations: fused kernel path vs plain torch path (GPU) vs
+    the same iteration in FP64 on the oracle; RMSE / peak <= 1e-4."""
+    cuda = torch.device("cuda", 0)
+    g = configs.fd_32()
+    _case("fista_tv fd_32")
+    A64, AT64 = _oracle_ops(g)
+    ph = phantom.phantom((32, 32, 32), 1.0, 6, seed=9)
+    b64 = A64(ph.double())
+    lip = unroll.power_norm(g, device=cuda)
+    lam, eps, iters = 0.02, 0.01, 10
+    step = 1.0 / (lip + lam * 12.0 / eps)
+    xf = unroll.fista_tv(b64.float().to(cuda), g, iters, lam, eps, step=step, fused=True)
+    xp = unroll.fista_tv(b64.float().to(cuda), g, iters, lam, eps, step=step, fused=False)
+    # FP64 oracle run of the same iteration
+    x = torch.zeros_like(ph, dtype=torch.float64)
+    y = x.clone()
+    for beta in unroll.fista_momentum(iters):
+        gr = AT64(A64(y) - b64)
+        x, y = tv.fista_tv_step_torch(y, gr, x, step, lam, eps, beta)
+    peak = float(x.abs().max())
+    for got in (xf, xp):
+        rmse = float(((got.double().cpu() - x) ** 2).mean().sqrt())
+        assert rmse / peak <= 1e-4
+    assert float(x.abs().sum()) > 0
+
+
+def test_fista_64_reconstruction_converges():
+    """The 64^3 / 72-view workload: 10 fused FISTA-TV iterations reduce the
+    data residual and move towards the phantom."""
+    cuda = torch.device("cuda", 0)
+    g = configs.fista_64()
+    _case("fista_64 recon")
+    ph = phantom.phantom((64, 64, 64), 1.0, 10, seed=1).to(cuda)
+    b = ops.forward(ph, g)
+    x = unroll.fista_tv(b, g, 10, lam=1e-3, eps=1e-3)
+    r0 = float(b.norm())
+    r1 = float((ops.forward(x, g) - b).norm())
+    assert r1 < 0.2 * r0
+    assert float((x - ph).norm()) < float(ph.norm())
+
+
+def test_pdhg_dtv_net_matches_fp64_oracle_network():
+    """Forward output and parameter gradients of the checkpointed learned
+    unroll on the GPU vs the identical network on FP64 oracle operators."""
+    cuda = torch.device("cuda", 0)
+    g = configs.fd_32()
+    _case("pdhg_dtv fd_32")
+    A64, AT64 = _oracle_ops(g)
+    ph = phantom.phantom((32, 32, 32), 1.0, 6, seed=3)
+    b = A64(ph.double())
+    net = unroll.PDHGDTVNet(g, iters=3, norm=unroll.power_norm(g, device=cuda)).to(cuda)
+    out = net(b.float().to(cuda))
+    loss = ((out - ph.to(cuda)) ** 2).mean()
+    loss.backward()
+    net64 = unroll.PDHGDTVNet(g, iters=3, norm=net.knorm ** 2 - 12.0, project=A64,
+                              backproject=AT64, checkpointing=False).double()
+    with torch.no_grad():
+        for p64, p in zip(net64.parameters(), net.parameters()):
+            p64.copy_(p.double().cpu())
+    out64 = net64(b)
+    loss64 = ((out64 - ph.double()) ** 2).mean()
+    loss64.backward()
+    assert rel_l2(out, out64) <= 1e-4
+    for p, p64 in zip(net.parameters(), net64.parameters()):
+        assert rel_l2(p.grad, p64.grad) <= 1e-3
+    # FP64 central differences on the oracle network validate its autograd
+    with torch.no_grad():
+        v = torch.zeros_like(net64.sigma)
+        v[1] = 1.0
+        h = 1e-4
+        net64.sigma += h * v
+        lp = float(((net64(b) - ph.double()) ** 2).mean())
+        net64.sigma -= 2 * h * v
+        lm = float(((net64(b) - ph.double()) ** 2).mean())
+        net64.sigma += h * v
+    fd = (lp - lm) / (2 * h)
+    assert abs(fd - float(net64.sigma.grad[1])) <= 1e-6 + 1e-4 * abs(fd)
+
+
+def test_checkpointing_same_gradients_less_memory():
+    cuda = torch.device("cuda", 0)
+    g = configs.fd_32()
+    b = ops.forward(phantom.phantom((32, 32, 32), 1.0, 6, seed=4).to(cuda), g)
+    grads = []
+    peaks = []
+    for ckpt in (False, True):
+        torch.manual_seed(0)
+        net = unroll.PDHGDTVNet(g, iters=6, checkpointing=ckpt).to(cuda)
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        out = net(b.repeat(2, 1, 1, 1, 1))
+        out.square().mean().backward()
+        torch.cuda.synchronize()
+        peaks.append(torch.cuda.max_memory_allocated() - base)
+        grads.append(torch.cat([p.grad.reshape(-1) for p in net.parameters()]))
+    assert rel_l2(grads[1], grads[0]) <= 1e-4
+    assert peaks[1] < peaks[0]
