@@ -24,6 +24,7 @@
 
 #include "dtv_geom.h"
 #include "dtv_launch.h"
+#include "dtvproj.h"
 
 using namespace dtv;
 
@@ -108,23 +109,33 @@ __device__ __forceinline__ void box_window(const double P0[3], const double d[3]
   if (kb > kend) kb = kend;
 }
 
-__global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol,
-                                                   float* __restrict__ sino, Geo g, int bx,
-                                                   int by, int bz, int nbx, int nby) {
-  extern __shared__ float tile[];  // (bx+1)*(by+1)*(bz+1)
-  __shared__ View sv;
-  __shared__ ViewProj sp;
-  __shared__ double ca[8], cb[8], cw[8];
-  __shared__ int cok[8], rect[4];
+// Per-workgroup LDS state shared by the brick bodies.
+struct BrickSmem {
+  View sv;
+  ViewProj sp;
+  double ca[8], cb[8], cw[8];
+  int cok[8], rect[4];
+  unsigned vmax_bits;
+};
 
-  const int brick = blockIdx.x;
-  const int b = blockIdx.y;
+// A over one brick of one volume.  vb: this volume, sbase: its sinogram
+// (accumulated into with atomics), tile: (bx+1)*(by+1)*(bz+1) floats of LDS.
+__device__ void fwd_brick_body(const float* __restrict__ vb, float* __restrict__ sbase,
+                               const Geo& g, int bx, int by, int bz, int nbx, int nby,
+                               int brick, float* tile, BrickSmem& sm) {
+  View& sv = sm.sv;
+  ViewProj& sp = sm.sp;
+  double* ca = sm.ca;
+  double* cb = sm.cb;
+  double* cw = sm.cw;
+  int* cok = sm.cok;
+  int* rect = sm.rect;
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   // bricks tile floor(p) over [-1, N-1]; the tile holds voxels [o, o+b]
   const int ox = -1 + jx * bx, oy = -1 + jy * by, oz = -1 + jz * bz;
   const int ex = min(bx, g.nx - ox), ey = min(by, g.ny - oy), ez = min(bz, g.nz - oz);
   const int tx = bx + 1, ty = by + 1, tz = bz + 1;
-  const float* vb = vol + (size_t)b * g.nx * g.ny * g.nz;
+  __syncthreads();  // the previous item of a persistent workgroup is done with LDS
   for (int i = threadIdx.x; i < tx * ty * tz; i += blockDim.x) {
     const int lx = i % tx, ly = (i / tx) % ty, lz = i / (tx * ty);
     const int x = ox + lx, y = oy + ly, z = oz + lz;
@@ -155,7 +166,7 @@ __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol
     const int v0 = rect[2], nvr = rect[3] - rect[2] + 1;
     if (nur <= 0 || nvr <= 0) continue;
     const View v = sv;
-    float* srow = sino + ((size_t)b * g.n_views + a) * g.nv * g.nu;
+    float* srow = sbase + (size_t)a * g.nv * g.nu;
     for (int r = threadIdx.x; r < nur * nvr; r += blockDim.x) {
       const int iu = u0 + r % nur, iv = v0 + r / nur;
       double d[3], P0[3];
@@ -200,6 +211,17 @@ __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol
   }
 }
 
+__global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol,
+                                                   float* __restrict__ sino, Geo g, int bx,
+                                                   int by, int bz, int nbx, int nby) {
+  extern __shared__ float tile[];
+  __shared__ BrickSmem sm;
+  const int b = blockIdx.y;
+  fwd_brick_body(vol + (size_t)b * g.nx * g.ny * g.nz,
+                 sino + (size_t)b * g.n_views * g.nv * g.nu, g, bx, by, bz, nbx, nby,
+                 blockIdx.x, tile, sm);
+}
+
 // Upper bound on sum_k w over the samples of one view that touch one voxel c
 // of the box (w = the sample's trilinear weight on c):
 //  * per ray, the weights on c are bounded by a 1-D tent along the ray's
@@ -235,27 +257,29 @@ __device__ __forceinline__ float weight_per_voxel_bound(const double* ca, const 
 
 constexpr int kAdjPerThread = 32;  // owned voxels per lane (bx*by*bz <= 32*block)
 
-__global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sino,
-                                                   float* __restrict__ vol, Geo g, int bx,
-                                                   int by, int bz, int nbx, int nby) {
-  // Per-view accumulators: fixed-point int32 in LDS, rows padded to odd
-  // strides (bx+1, (bx+1)*by+1) so lanes differing in y or z use different
-  // banks.  A view whose rays carry a non-finite value falls back to FP32
-  // LDS atomics in the same buffer so infinities and NaNs propagate.
-  extern __shared__ int iacc[];
-  __shared__ View sv;
-  __shared__ ViewProj sp;
-  __shared__ double ca[8], cb[8], cw[8];
-  __shared__ int cok[8], rect[4];
-  __shared__ unsigned vmax_bits;
-
-  const int brick = blockIdx.x;
-  const int b = blockIdx.y;
+// Matched A^T over one brick of one volume.  sbase: this volume's sinogram,
+// vb: its volume (the brick's voxels are written once), iacc: per-view
+// accumulators, fixed-point int32 in LDS with rows padded to odd strides
+// (bx+1, (bx+1)*by+1) so lanes differing in y or z use different banks.  A
+// view whose rays carry a non-finite value falls back to FP32 LDS atomics
+// in the same buffer so infinities and NaNs propagate.
+__device__ void adj_brick_body(const float* __restrict__ sbase, float* __restrict__ vb,
+                               const Geo& g, int bx, int by, int bz, int nbx, int nby,
+                               int brick, int* iacc, BrickSmem& sm) {
+  View& sv = sm.sv;
+  ViewProj& sp = sm.sp;
+  double* ca = sm.ca;
+  double* cb = sm.cb;
+  double* cw = sm.cw;
+  int* cok = sm.cok;
+  int* rect = sm.rect;
+  unsigned& vmax_bits = sm.vmax_bits;
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   const int x0 = jx * bx, y0 = jy * by, z0 = jz * bz;
   const int ex = min(bx, g.nx - x0), ey = min(by, g.ny - y0), ez = min(bz, g.nz - z0);
   const int sty = bx + 1, stz = (bx + 1) * by + 1;
   const int nown = bx * by * bz;
+  __syncthreads();  // the previous item of a persistent workgroup is done with LDS
   for (int i = threadIdx.x; i < stz * bz; i += blockDim.x) iacc[i] = 0;
   float vacc[kAdjPerThread];
 #pragma unroll
@@ -283,7 +307,7 @@ __global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sin
     const int u0 = rect[0], nur = rect[1] - rect[0] + 1;
     const int v0 = rect[2], nvr = rect[3] - rect[2] + 1;
     if (nur <= 0 || nvr <= 0) continue;
-    const float* srow = sino + ((size_t)b * g.n_views + a) * g.nv * g.nu;
+    const float* srow = sbase + (size_t)a * g.nv * g.nu;
     // pass 1: largest |value| among this view's rays through the brick
     float m = 0.f;
     for (int r = threadIdx.x; r < nur * nvr; r += blockDim.x) {
@@ -385,7 +409,6 @@ __global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sin
       }
     }
   }
-  float* vb = vol + (size_t)b * g.nx * g.ny * g.nz;
 #pragma unroll
   for (int j = 0; j < kAdjPerThread; ++j) {
     const int i = threadIdx.x + j * blockDim.x;
@@ -396,6 +419,92 @@ __global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sin
     }
   }
 }
+
+__global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sino,
+                                                   float* __restrict__ vol, Geo g, int bx,
+                                                   int by, int bz, int nbx, int nby) {
+  extern __shared__ int iacc[];
+  __shared__ BrickSmem sm;
+  const int b = blockIdx.y;
+  adj_brick_body(sino + (size_t)b * g.n_views * g.nv * g.nu,
+                 vol + (size_t)b * g.nx * g.ny * g.nz, g, bx, by, bz, nbx, nby, blockIdx.x,
+                 iacc, sm);
+}
+
+// ---- uneven instances in one launch ----------------------------------------
+
+__device__ Geo instance_geo(const dtv_instance& in) {
+  Geo g{};
+  g.nx = in.nx;
+  g.ny = in.ny;
+  g.nz = in.nz;
+  g.nu = in.nu;
+  g.nv = in.nv;
+  g.n_views = in.n_views;
+  g.axis_mode = in.axis_mode;
+  for (int c = 0; c < 3; ++c) {
+    g.lo[c] = in.box_lo[c];
+    g.hi[c] = in.box_hi[c];
+  }
+  g.margin = in.margin;
+  g.scale = in.scale;
+  g.kind = kTable;
+  g.table = in.views;
+  return g;
+}
+
+// Next work item: blockIdx.x in one-item-per-workgroup mode, else taken
+// from the device counter (persistent workgroups, dynamic balance).
+__device__ __forceinline__ int next_item(int* counter, int* slot, int done) {
+  if (!counter) return done == 0 ? (int)blockIdx.x : 0x7fffffff;
+  __syncthreads();
+  if (threadIdx.x == 0) *slot = atomicAdd(counter, 1);
+  __syncthreads();
+  return *slot;
+}
+
+__global__ __launch_bounds__(256) void k_fwd_brick_multi(const dtv_instance* __restrict__ inst,
+                                                         const dtv_work_item* __restrict__ work,
+                                                         int n_work, int* counter) {
+  extern __shared__ float tile[];
+  __shared__ BrickSmem sm;
+  __shared__ int slot;
+  for (int done = 0;; ++done) {
+    const int w = next_item(counter, &slot, done);
+    if (w >= n_work) break;
+    const dtv_work_item it = work[w];
+    const dtv_instance in = inst[it.instance];
+    const Geo g = instance_geo(in);
+    fwd_brick_body(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, tile, sm);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adj_brick_multi(const dtv_instance* __restrict__ inst,
+                                                         const dtv_work_item* __restrict__ work,
+                                                         int n_work, int* counter) {
+  extern __shared__ int iacc[];
+  __shared__ BrickSmem sm;
+  __shared__ int slot;
+  for (int done = 0;; ++done) {
+    const int w = next_item(counter, &slot, done);
+    if (w >= n_work) break;
+    const dtv_work_item it = work[w];
+    const dtv_instance in = inst[it.instance];
+    const Geo g = instance_geo(in);
+    adj_brick_body(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, iacc, sm);
+  }
+}
+
+// out[i] = -init[i] (or 0) for every instance's output, blockIdx.y = instance
+__global__ void k_init_multi(const dtv_instance* __restrict__ inst) {
+  const dtv_instance in = inst[blockIdx.y];
+  const size_t n = (size_t)in.n_views * in.nu * in.nv;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    in.out[i] = in.init ? -in.init[i] : 0.f;
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
 
 }  // namespace
 
@@ -422,3 +531,100 @@ int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int
 }
 
 }  // namespace dtv
+
+namespace {
+
+int validate_multi(const dtv_instance* ih, int32_t n_inst, const dtv_work_item* wh,
+                   int32_t n_work, int32_t block, bool adjoint, size_t& lds) {
+  if (!ih || !wh) return DTV_ERR_NULL_POINTER;
+  if (n_inst < 1 || n_work < 1 || n_inst > 65535) return DTV_ERR_INVALID_ARGUMENT;
+  if (block < 64 || block > 256 || block % 64) return DTV_ERR_INVALID_ARGUMENT;
+  lds = 0;
+  size_t max_out = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    const dtv_instance& in = ih[i];
+    if (!in.in || !in.out || !in.views) return DTV_ERR_NULL_POINTER;
+    if (in.nx < 1 || in.ny < 1 || in.nz < 1 || in.nu < 1 || in.nv < 1 || in.n_views < 1)
+      return DTV_ERR_INVALID_ARGUMENT;
+    if ((int64_t)in.nx * in.ny * in.nz >= (1ll << 31) || (int64_t)in.nu * in.nv >= (1ll << 31))
+      return DTV_ERR_TOO_LARGE;
+    if (in.axis_mode != kAxisXY && in.axis_mode != kAxisXYZ) return DTV_ERR_INVALID_ARGUMENT;
+    if (in.bx < 2 || in.by < 2 || in.bz < 2 || in.bx > 64 || in.by > 64 || in.bz > 64)
+      return DTV_ERR_INVALID_ARGUMENT;
+    const int ext = adjoint ? 0 : 1;
+    if (in.nbx != (in.nx + ext + in.bx - 1) / in.bx || in.nby != (in.ny + ext + in.by - 1) / in.by ||
+        in.nbz != (in.nz + ext + in.bz - 1) / in.bz)
+      return DTV_ERR_INVALID_ARGUMENT;
+    size_t l;
+    if (adjoint) {
+      if (in.bx * in.by * in.bz > kAdjPerThread * block) return DTV_ERR_INVALID_ARGUMENT;
+      l = (size_t)((in.bx + 1) * in.by + 1) * in.bz * sizeof(int);
+    } else {
+      l = (size_t)(in.bx + 1) * (in.by + 1) * (in.bz + 1) * sizeof(float);
+    }
+    if (l > 65536) return DTV_ERR_INVALID_ARGUMENT;
+    lds = l > lds ? l : lds;
+    const size_t no = (size_t)in.n_views * in.nu * in.nv;
+    max_out = no > max_out ? no : max_out;
+  }
+  for (int w = 0; w < n_work; ++w) {
+    if (wh[w].instance < 0 || wh[w].instance >= n_inst) return DTV_ERR_INVALID_ARGUMENT;
+    const dtv_instance& in = ih[wh[w].instance];
+    if (wh[w].brick < 0 || wh[w].brick >= in.nbx * in.nby * in.nbz) return DTV_ERR_INVALID_ARGUMENT;
+  }
+  (void)max_out;
+  return DTV_OK;
+}
+
+int launch_multi(bool adjoint, const dtv_instance* ih, const dtv_instance* id, int32_t n_inst,
+                 const dtv_work_item* wh, const dtv_work_item* wd, int32_t n_work, int32_t block,
+                 int32_t* counter, int32_t persistent_blocks, hipStream_t st) {
+  size_t lds = 0;
+  int rc = validate_multi(ih, n_inst, wh, n_work, block, adjoint, lds);
+  if (rc) return rc;
+  if (!id || !wd) return DTV_ERR_NULL_POINTER;
+  if (!adjoint) {
+    size_t max_out = 0;
+    for (int i = 0; i < n_inst; ++i) {
+      const size_t no = (size_t)ih[i].n_views * ih[i].nu * ih[i].nv;
+      max_out = no > max_out ? no : max_out;
+    }
+    const unsigned gx = (unsigned)std::min<size_t>((max_out + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_init_multi, dim3(gx, n_inst), dim3(256), 0, st, id);
+    if (hipGetLastError() != hipSuccess) return DTV_ERR_HIP;
+  }
+  const bool persistent = counter && persistent_blocks > 0;
+  if (persistent) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, counter);
+    if (hipGetLastError() != hipSuccess) return DTV_ERR_HIP;
+  }
+  const dim3 grid(persistent ? (unsigned)std::min(persistent_blocks, n_work) : (unsigned)n_work);
+  int* ctr = persistent ? counter : nullptr;
+  if (adjoint)
+    hipLaunchKernelGGL(k_adj_brick_multi, grid, dim3(block), lds, st, id, wd, n_work, ctr);
+  else
+    hipLaunchKernelGGL(k_fwd_brick_multi, grid, dim3(block), lds, st, id, wd, n_work, ctr);
+  return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dtv_cone_forward_multi(const dtv_instance* inst_host, const dtv_instance* inst_dev,
+                           int32_t n_inst, const dtv_work_item* work_host,
+                           const dtv_work_item* work_dev, int32_t n_work, int32_t block,
+                           int32_t* counter, int32_t persistent_blocks, void* stream) {
+  return launch_multi(false, inst_host, inst_dev, n_inst, work_host, work_dev, n_work, block,
+                      counter, persistent_blocks, (hipStream_t)stream);
+}
+
+int dtv_cone_adjoint_multi(const dtv_instance* inst_host, const dtv_instance* inst_dev,
+                           int32_t n_inst, const dtv_work_item* work_host,
+                           const dtv_work_item* work_dev, int32_t n_work, int32_t block,
+                           int32_t* counter, int32_t persistent_blocks, void* stream) {
+  return launch_multi(true, inst_host, inst_dev, n_inst, work_host, work_dev, n_work, block,
+                      counter, persistent_blocks, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+"""Autotuner for the brick kernels' tile shape (brick edge lengths) and ray
+bundle size (lanes per workgroup).
+
+For a geometry it times every candidate launch for A and for the matched
+A^T on the device (median of ``reps`` timed calls after one warm-up, HIP
+events on the current stream) and keeps the fastest.  Results are cached per
+geometry signature in memory and, if ``path`` is given, in a JSON file.
+All candidates compute the same result (to FP32 rounding), so tuning never
+changes numerics beyond that.
+"""
+from __future__ import annotations
+
+import json
+import os
+import threading
+
+import torch
+
+from . import ops
+from .geometry import ConeGeometry
+
+# (bx, by, bz, lanes); the adjoint needs bx*by*bz <= 32 * lanes
+CANDIDATES = [
+    (32, 16, 16, 256), (32, 32, 8, 256), (16, 16, 32, 256), (16, 16, 16, 256),
+    (32, 16, 8, 256), (16, 16, 16, 128), (16, 16, 8, 128), (8, 8, 8, 64),
+]
+
+_cache = {}
+_lock = threading.Lock()
+
+
+def signature(geom: ConeGeometry) -> str:
+    sc = geom.scheme
+    return json.dumps([list(geom.volume_size), list(geom.detector_size), geom.n_views,
+                       sc.axis_mode, round(float(sc.margin), 6)])
+
+
+def launch_of(c):
+    return dict(algorithm=2, brick=tuple(c[:3]), block=c[3])
+
+
+def _time(fn, reps):
+    fn()
+    times = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    times.sort()
+    return times[len(times) // 2]
+
+
+def _load(path):
+    if path and os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def tune(geom: ConeGeometry, device=None, reps: int = 3, candidates=CANDIDATES,
+         path: str | None = None, verbose: bool = False) -> dict:
+    """Fastest ``dict(forward=launch, adjoint=launch, times=...)`` for geom."""
+    key = signature(geom)
+    with _lock:
+        if key in _cache:
+            return _cache[key]
+        disk = _load(path)
+        if key in disk:
+            r = disk[key]
+            res = dict(forward=launch_of(r["forward"]), adjoint=launch_of(r["adjoint"]),
+                       times=r.get("times"))
+            _cache[key] = res
+            return res
+    device = device or torch.device("cuda", torch.cuda.current_device())
+    gen = torch.Generator().manual_seed(0)
+    x = torch.rand(geom.volume_shape(1), generator=gen).to(device)
+    y = torch.rand(geom.sino_shape(1), generator=gen).to(device)
+    times = {"forward": {}, "adjoint": {}}
+    for c in candidates:
+        L = launch_of(c)
+        if c[0] * c[1] * c[2] > 32 * c[3]:
+            continue
+        times["forward"][str(c)] = _time(lambda: ops.forward(x, geom, L), reps)
+        times["adjoint"][str(c)] = _time(lambda: ops.adjoint(y, geom, L), reps)
+        if verbose:
+            print(f"tune {c}: fwd {times['forward'][str(c)]:.3f} ms  "
+                  f"adj {times['adjoint'][str(c)]:.3f} ms", flush=True)
+    best_f = min(times["forward"], key=times["forward"].get)
+    best_a = min(times["adjoint"], key=times["adjoint"].get)
+    bf = tuple(json.loads(best_f.replace("(", "[").replace(")", "]")))
+    ba = tuple(json.loads(best_a.replace("(", "[").replace(")", "]")))
+    res = dict(forward=launch_of(bf), adjoint=launch_of(ba), times=times)
+    with _lock:
+        _cache[key] = res
+        if path:
+            disk = _load(path)
+            disk[key] = dict(forward=list(bf), adjoint=list(ba), times=times)
+            with open(path, "w") as f:
+                json.dump(disk, f, indent=1)
+    return res

@@ -1,0 +1,244 @@
+"""Many volumes of uneven size and view count in one launch per operator.
+
+:class:`MultiProjector` builds the instance and work tables of
+``dtv_cone_forward_multi`` / ``dtv_cone_adjoint_multi`` (include/dtvproj.h)
+for a list of geometries and bound buffers.  The work list holds every
+(instance, brick) pair sorted by estimated cost (views x brick voxels),
+largest first, so the hardware's in-order workgroup dispatch (or the
+persistent queue) balances uneven instances.
+
+:class:`BatchedFistaTV` runs the FISTA-TV unroll over all instances with
+three launches per iteration -- A (initialised with -b, so it yields the
+residual A y - b directly), matched A^T, and the fused TV / non-negativity /
+momentum step over a descriptor table -- with every table in static device
+memory, so :meth:`BatchedFistaTV.capture` can record the whole unroll in one
+hipGraph (torch.cuda.CUDAGraph on ROCm) and :meth:`replay` re-runs it.
+:func:`plain_fista_tv` is the per-volume loop it replaces.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _capi, autotune, ops, tv, unroll
+from .geometry import ConeGeometry
+
+
+class Instance(ctypes.Structure):
+    _fields_ = [
+        ("in_", ctypes.c_void_p), ("out", ctypes.c_void_p), ("init", ctypes.c_void_p),
+        ("views", ctypes.c_void_p),
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("nu", ctypes.c_int32), ("nv", ctypes.c_int32), ("n_views", ctypes.c_int32),
+        ("axis_mode", ctypes.c_int32),
+        ("bx", ctypes.c_int32), ("by", ctypes.c_int32), ("bz", ctypes.c_int32),
+        ("nbx", ctypes.c_int32), ("nby", ctypes.c_int32), ("nbz", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("box_lo", ctypes.c_double * 3), ("box_hi", ctypes.c_double * 3),
+        ("margin", ctypes.c_double), ("scale", ctypes.c_double),
+    ]
+
+
+class WorkItem(ctypes.Structure):
+    _fields_ = [("instance", ctypes.c_int32), ("brick", ctypes.c_int32)]
+
+
+def _sig(lib):
+    for name in ("dtv_cone_forward_multi", "dtv_cone_adjoint_multi"):
+        fn = getattr(lib, name)
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                       ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
+                       ctypes.c_int32, ctypes.c_void_p]
+        fn.restype = ctypes.c_int32
+
+
+class _Bound:
+    """Host + device copies of one instance table and its work list."""
+
+    def __init__(self, host_inst, host_work, device):
+        self.host_inst = host_inst
+        self.host_work = host_work
+        self.dev_inst = torch.frombuffer(bytearray(host_inst), dtype=torch.uint8).to(device)
+        self.dev_work = torch.frombuffer(bytearray(host_work), dtype=torch.uint8).to(device)
+        self.n_inst = len(host_inst)
+        self.n_work = len(host_work)
+
+
+class MultiProjector:
+    """A / matched A^T over uneven instances, one kernel launch per call.
+
+    ``bricks``: per-instance (forward, adjoint) brick shapes; default
+    32x16x16 for both, or ``"tune"`` to run the autotuner per instance."""
+
+    def __init__(self, geoms, bricks=None, block=256, persistent_blocks=0, device=None):
+        self.geoms = list(geoms)
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.block = block
+        self.persistent_blocks = persistent_blocks
+        if bricks == "tune":
+            bricks = []
+            for g in self.geoms:
+                t = autotune.tune(g, self.device)
+                bricks.append((t["forward"]["brick"], t["adjoint"]["brick"]))
+        self.bricks = bricks or [((32, 16, 16), (32, 16, 16))] * len(self.geoms)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self.views = [g.device_views(self.device) for g in self.geoms]
+        _sig(_capi.lib())
+
+    def _table(self, adjoint, ins, outs, inits):
+        n = len(self.geoms)
+        host = (Instance * n)()
+        work = []
+        for i, g in enumerate(self.geoms):
+            e = host[i]
+            b = self.bricks[i][1 if adjoint else 0]
+            X, Y, Z = g.volume_size
+            U, V = g.detector_size
+            e.in_ = ins[i].data_ptr()
+            e.out = outs[i].data_ptr()
+            e.init = inits[i].data_ptr() if (inits is not None and inits[i] is not None) else None
+            e.views = self.views[i].data_ptr()
+            e.nx, e.ny, e.nz, e.nu, e.nv, e.n_views = X, Y, Z, U, V, g.n_views
+            e.axis_mode = g.scheme.axis_mode
+            e.bx, e.by, e.bz = b
+            ext = 0 if adjoint else 1
+            e.nbx = (X + ext + b[0] - 1) // b[0]
+            e.nby = (Y + ext + b[1] - 1) // b[1]
+            e.nbz = (Z + ext + b[2] - 1) // b[2]
+            for c in range(3):
+                e.box_lo[c] = g.scheme.box_lo[c]
+                e.box_hi[c] = g.scheme.box_hi[c]
+            e.margin = g.scheme.margin
+            e.scale = g.scheme.scale
+            nb = e.nbx * e.nby * e.nbz
+            cost = float(g.n_views * b[0] * b[1] * b[2])
+            work += [(cost, i, k) for k in range(nb)]
+        work.sort(key=lambda t: (-t[0], t[1], t[2]))
+        wh = (WorkItem * len(work))()
+        for j, (_, i, k) in enumerate(work):
+            wh[j].instance = i
+            wh[j].brick = k
+        return _Bound(host, wh, self.device)
+
+    def bind_forward(self, volumes, sinos, inits=None):
+        """Bind buffers for out[i] = A volumes[i] - inits[i] (inits optional)."""
+        return self._table(False, volumes, sinos, inits)
+
+    def bind_adjoint(self, sinos, volumes):
+        return self._table(True, sinos, volumes, None)
+
+    def run(self, bound, adjoint: bool):
+        L = _capi.lib()
+        fn = L.dtv_cone_adjoint_multi if adjoint else L.dtv_cone_forward_multi
+        with torch.cuda.device(self.device):
+            rc = fn(ctypes.addressof(bound.host_inst), bound.dev_inst.data_ptr(), bound.n_inst,
+                    ctypes.addressof(bound.host_work), bound.dev_work.data_ptr(), bound.n_work,
+                    self.block, self.counter.data_ptr() if self.persistent_blocks else None,
+                    self.persistent_blocks, ops._stream(self.device))
+        _capi.check(rc, "dtv_cone_adjoint_multi" if adjoint else "dtv_cone_forward_multi")
+
+    def forward(self, volumes, inits=None):
+        """Eager convenience: returns new sinograms (A v_i - init_i)."""
+        sinos = [torch.empty(g.sino_shape(1), device=self.device) for g in self.geoms]
+        self.run(self.bind_forward(volumes, sinos, inits), False)
+        return sinos
+
+    def adjoint(self, sinos):
+        vols = [torch.empty(g.volume_shape(1), device=self.device) for g in self.geoms]
+        self.run(self.bind_adjoint(sinos, vols), True)
+        return vols
+
+
+class BatchedFistaTV:
+    """``iters`` FISTA-TV iterations over uneven instances (3 launches per
+    iteration, all tables static, graph-capturable)."""
+
+    def __init__(self, geoms, sinos, iters=20, lam=1e-3, eps=1e-3, steps=None, bricks=None,
+                 block=256, persistent_blocks=0, mode="neumann"):
+        self.mp = MultiProjector(geoms, bricks, block, persistent_blocks)
+        dev = self.mp.device
+        self.geoms = self.mp.geoms
+        self.iters = iters
+        self.mode = tv.MODES[mode]
+        n = len(self.geoms)
+        if steps is None:
+            steps = [1.0 / (unroll.power_norm(g, device=dev) + lam * 12.0 / eps)
+                     for g in self.geoms]
+        self.b = [s.contiguous() for s in sinos]
+        self.x = [torch.zeros(g.volume_shape(1), device=dev) for g in self.geoms]
+        self.y = [[torch.zeros(g.volume_shape(1), device=dev) for g in self.geoms]
+                  for _ in range(2)]
+        self.r = [torch.empty(g.sino_shape(1), device=dev) for g in self.geoms]
+        self.g = [torch.empty(g.volume_shape(1), device=dev) for g in self.geoms]
+        self.fwd = [self.mp.bind_forward(self.y[p], self.r, self.b) for p in range(2)]
+        self.adj = self.mp.bind_adjoint(self.r, self.g)
+        betas = unroll.fista_momentum(iters)
+        self.max_vox = max(int(np.prod(g.volume_size)) for g in self.geoms)
+        self.tv_tables = []
+        for k in range(iters):
+            p = k % 2
+            items = (tv.TvItem * n)()
+            for i, g in enumerate(self.geoms):
+                X, Y, Z = g.volume_size
+                it = items[i]
+                it.y = self.y[p][i].data_ptr()
+                it.g = self.g[i].data_ptr()
+                it.xprev = self.x[i].data_ptr()
+                it.xnew = self.x[i].data_ptr()
+                it.ynew = self.y[1 - p][i].data_ptr()
+                it.nx, it.ny, it.nz = X, Y, Z
+                it.nonneg = 1
+                it.step, it.lam, it.eps, it.beta = steps[i], lam, eps, betas[k]
+            self.tv_tables.append(
+                torch.frombuffer(bytearray(items), dtype=torch.uint8).to(dev))
+        self.graph = None
+
+    def reset(self):
+        for t in self.x + self.y[0] + self.y[1]:
+            t.zero_()
+
+    def run(self):
+        """Eager unroll (also what gets captured)."""
+        L = _capi.lib()
+        stream = ops._stream(self.mp.device)
+        for k in range(self.iters):
+            p = k % 2
+            self.mp.run(self.fwd[p], adjoint=False)
+            self.mp.run(self.adj, adjoint=True)
+            rc = L.dtv_tv_fista_step_multi(self.tv_tables[k].data_ptr(), len(self.geoms),
+                                           self.max_vox, self.mode, stream)
+            _capi.check(rc, "dtv_tv_fista_step_multi")
+        return self.x
+
+    def capture(self):
+        """Record reset + the whole unroll in one graph (fill kernels only,
+        no memset nodes)."""
+        s = torch.cuda.Stream(self.mp.device)
+        s.wait_stream(torch.cuda.current_stream(self.mp.device))
+        with torch.cuda.stream(s):
+            self.reset()
+            self.run()
+        torch.cuda.current_stream(self.mp.device).wait_stream(s)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.reset()
+            self.run()
+        return self.graph
+
+    def replay(self):
+        if self.graph is None:
+            self.capture()
+        self.graph.replay()
+        return self.x
+
+
+def plain_fista_tv(geoms, sinos, iters=20, lam=1e-3, eps=1e-3, steps=None):
+    """The per-volume path: a Python loop over instances, each running the
+    unfused FISTA-TV iteration on the default single-instance kernels."""
+    out = []
+    for i, g in enumerate(geoms):
+        step = None if steps is None else steps[i]
+        out.append(unroll.fista_tv(sinos[i], g, iters, lam, eps, step=step, fused=False))
+    return out

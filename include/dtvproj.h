@@ -136,6 +136,51 @@ int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,
                         const double* views, int32_t n_views, const dtv_scheme* scheme,
                         const dtv_launch* launch, void* stream);
 
+/* ---- Uneven instances in one launch ----------------------------------- */
+
+/* One volume/sinogram pair of a multi-instance call (canonical view table
+ * geometry).  Forward: in = volume, out = sinogram (initialised to -init, or
+ * 0 when init is NULL, then accumulated: out = A in - init).  Adjoint: in =
+ * sinogram, out = volume (fully overwritten).  bx..bz is this instance's
+ * brick shape, nbx..nbz its brick grid (forward bricks tile [-1, N-1]:
+ * ceil((N+1)/b); adjoint bricks tile [0, N-1]: ceil(N/b)). */
+typedef struct dtv_instance {
+  const float* in;
+  float* out;
+  const float* init;
+  const double* views;
+  int32_t nx, ny, nz, nu, nv, n_views;
+  int32_t axis_mode;
+  int32_t bx, by, bz;
+  int32_t nbx, nby, nbz;
+  int32_t reserved;
+  double box_lo[3], box_hi[3];
+  double margin, scale;
+} dtv_instance;
+
+/* One brick of one instance. */
+typedef struct dtv_work_item {
+  int32_t instance;
+  int32_t brick;
+} dtv_work_item;
+
+/* A (resp. matched A^T) over every (instance, brick) of ``work`` in one
+ * kernel launch.  inst_host / work_host are host copies of the device
+ * tables inst_dev / work_dev (validated and sized on the host; the kernel
+ * reads the device copies, so the call can be captured in a graph).  With
+ * counter != NULL (a device int) and persistent_blocks > 0, that many
+ * workgroups pull items from a queue (dynamic balance); otherwise one
+ * workgroup per item, in list order.  No reference counterpart: the
+ * reference launches per view and per batch item (forwardKernel.cu:115-141). */
+int dtv_cone_forward_multi(const dtv_instance* inst_host, const dtv_instance* inst_dev,
+                           int32_t n_inst, const dtv_work_item* work_host,
+                           const dtv_work_item* work_dev, int32_t n_work, int32_t block,
+                           int32_t* counter, int32_t persistent_blocks, void* stream);
+int dtv_cone_adjoint_multi(const dtv_instance* inst_host, const dtv_instance* inst_dev,
+                           int32_t n_inst, const dtv_work_item* work_host,
+                           const dtv_work_item* work_dev, int32_t n_work, int32_t block,
+                           int32_t* counter, int32_t persistent_blocks, void* stream);
+
 /* ---- Fused reconstruction update ---------------------------------------- */
 
 /* One instance of the fused FISTA / smoothed-TV step (device pointers). */
