@@ -184,9 +184,18 @@ def main():
     else:
         dom, dom_t, dom_b = "k_fwd (A)", t_fwd, acct["forward"]
     achieved = dom_b / dom_t / 1e9
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by a copy kernel)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic_sparse_256.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            pmc = json.load(f)
+        key = "k_adj_brick" if dom.startswith("k_adj") else "k_fwd_brick"
+        traffic = pmc.get(key, {}).get("traffic_bytes")
     pair_gbs = acct["pair"] / (t_fwd + t_adj) / 1e9
     roof = dict(bound="hbm", achieved=achieved, peak=roofline.HBM_PEAK_GBS, unit="GB/s",
-                frac=achieved / roofline.HBM_PEAK_GBS, traffic=None, kernel=dom,
+                frac=achieved / roofline.HBM_PEAK_GBS, traffic=traffic, kernel=dom,
                 pair_achieved=pair_gbs, pair_frac=pair_gbs / roofline.HBM_PEAK_GBS,
                 forward_ms=t_fwd * 1e3, adjoint_ms=t_adj * 1e3,
                 algorithmic_bytes=dict(forward=acct["forward"], adjoint=acct["adjoint"],
