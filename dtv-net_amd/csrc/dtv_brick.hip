@@ -28,6 +28,12 @@
 
 using namespace dtv;
 
+// Brick shapes compiled with constant dimensions (the autotuner's candidates);
+// any other shape runs the run-time-shape kernels.
+#define DTV_BRICK_SHAPES(X) \
+  X(32, 16, 16) X(32, 32, 8) X(16, 16, 32) X(16, 16, 16) X(32, 16, 8) X(16, 16, 8) X(8, 8, 8)
+
+
 namespace {
 
 // Rectangle of detector pixels whose rays can pass through the box
@@ -120,9 +126,16 @@ struct BrickSmem {
 
 // A over one brick of one volume.  vb: this volume, sbase: its sinogram
 // (accumulated into with atomics), tile: (bx+1)*(by+1)*(bz+1) floats of LDS.
-__device__ void fwd_brick_body(const float* __restrict__ vb, float* __restrict__ sbase,
-                               const Geo& g, int bx, int by, int bz, int nbx, int nby,
-                               int brick, float* tile, BrickSmem& sm) {
+//
+// CBX/CBY/CBZ: the brick shape when known at compile time (0 = take the
+// run-time value); the instantiated shapes turn every index computation into
+// shifts and immediates.
+template <int CBX, int CBY, int CBZ>
+__device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
+                                               float* __restrict__ sbase, const Geo& g,
+                                               int bx_, int by_, int bz_, int nbx, int nby,
+                                               int brick, float* tile, BrickSmem& sm) {
+  const int bx = CBX ? CBX : bx_, by = CBY ? CBY : by_, bz = CBZ ? CBZ : bz_;
   View& sv = sm.sv;
   ViewProj& sp = sm.sp;
   double* ca = sm.ca;
@@ -211,13 +224,14 @@ __device__ void fwd_brick_body(const float* __restrict__ vb, float* __restrict__
   }
 }
 
+template <int CBX, int CBY, int CBZ>
 __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol,
                                                    float* __restrict__ sino, Geo g, int bx,
                                                    int by, int bz, int nbx, int nby) {
   extern __shared__ float tile[];
   __shared__ BrickSmem sm;
   const int b = blockIdx.y;
-  fwd_brick_body(vol + (size_t)b * g.nx * g.ny * g.nz,
+  fwd_brick_body<CBX, CBY, CBZ>(vol + (size_t)b * g.nx * g.ny * g.nz,
                  sino + (size_t)b * g.n_views * g.nv * g.nu, g, bx, by, bz, nbx, nby,
                  blockIdx.x, tile, sm);
 }
@@ -263,9 +277,12 @@ constexpr int kAdjPerThread = 32;  // owned voxels per lane (bx*by*bz <= 32*bloc
 // (bx+1, (bx+1)*by+1) so lanes differing in y or z use different banks.  A
 // view whose rays carry a non-finite value falls back to FP32 LDS atomics
 // in the same buffer so infinities and NaNs propagate.
-__device__ void adj_brick_body(const float* __restrict__ sbase, float* __restrict__ vb,
-                               const Geo& g, int bx, int by, int bz, int nbx, int nby,
-                               int brick, int* iacc, BrickSmem& sm) {
+template <int CBX, int CBY, int CBZ>
+__device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
+                                               float* __restrict__ vb, const Geo& g, int bx_,
+                                               int by_, int bz_, int nbx, int nby, int brick,
+                                               int* iacc, BrickSmem& sm) {
+  const int bx = CBX ? CBX : bx_, by = CBY ? CBY : by_, bz = CBZ ? CBZ : bz_;
   View& sv = sm.sv;
   ViewProj& sp = sm.sp;
   double* ca = sm.ca;
@@ -420,13 +437,16 @@ __device__ void adj_brick_body(const float* __restrict__ sbase, float* __restric
   }
 }
 
-__global__ __launch_bounds__(256) void k_adj_brick(const float* __restrict__ sino,
+// at least 3 waves per SIMD (<= 168 VGPRs): the hoisted flush indices must
+// not cost occupancy
+template <int CBX, int CBY, int CBZ>
+__global__ __launch_bounds__(256, 3) void k_adj_brick(const float* __restrict__ sino,
                                                    float* __restrict__ vol, Geo g, int bx,
                                                    int by, int bz, int nbx, int nby) {
   extern __shared__ int iacc[];
   __shared__ BrickSmem sm;
   const int b = blockIdx.y;
-  adj_brick_body(sino + (size_t)b * g.n_views * g.nv * g.nu,
+  adj_brick_body<CBX, CBY, CBZ>(sino + (size_t)b * g.n_views * g.nv * g.nu,
                  vol + (size_t)b * g.nx * g.ny * g.nz, g, bx, by, bz, nbx, nby, blockIdx.x,
                  iacc, sm);
 }
@@ -463,6 +483,7 @@ __device__ __forceinline__ int next_item(int* counter, int* slot, int done) {
   return *slot;
 }
 
+template <int CBX, int CBY, int CBZ>
 __global__ __launch_bounds__(256) void k_fwd_brick_multi(const dtv_instance* __restrict__ inst,
                                                          const dtv_work_item* __restrict__ work,
                                                          int n_work, int* counter) {
@@ -475,11 +496,12 @@ __global__ __launch_bounds__(256) void k_fwd_brick_multi(const dtv_instance* __r
     const dtv_work_item it = work[w];
     const dtv_instance in = inst[it.instance];
     const Geo g = instance_geo(in);
-    fwd_brick_body(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, tile, sm);
+    fwd_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, tile, sm);
   }
 }
 
-__global__ __launch_bounds__(256) void k_adj_brick_multi(const dtv_instance* __restrict__ inst,
+template <int CBX, int CBY, int CBZ>
+__global__ __launch_bounds__(256, 3) void k_adj_brick_multi(const dtv_instance* __restrict__ inst,
                                                          const dtv_work_item* __restrict__ work,
                                                          int n_work, int* counter) {
   extern __shared__ int iacc[];
@@ -491,7 +513,7 @@ __global__ __launch_bounds__(256) void k_adj_brick_multi(const dtv_instance* __r
     const dtv_work_item it = work[w];
     const dtv_instance in = inst[it.instance];
     const Geo g = instance_geo(in);
-    adj_brick_body(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, iacc, sm);
+    adj_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, iacc, sm);
   }
 }
 
@@ -515,8 +537,13 @@ int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int
   const int nbx = (g.nx + 1 + bx - 1) / bx, nby = (g.ny + 1 + by - 1) / by,
             nbz = (g.nz + 1 + bz - 1) / bz;
   const size_t lds = (size_t)(bx + 1) * (by + 1) * (bz + 1) * sizeof(float);
-  hipLaunchKernelGGL(k_fwd_brick, dim3(nbx * nby * nbz, batch), dim3(block), lds, st, vol, sino,
-                     g, bx, by, bz, nbx, nby);
+  auto kern = &k_fwd_brick<0, 0, 0>;
+#define DTV_PICK(a, b, c) \
+  if (bx == a && by == b && bz == c) kern = &k_fwd_brick<a, b, c>;
+  DTV_BRICK_SHAPES(DTV_PICK)
+#undef DTV_PICK
+  hipLaunchKernelGGL(kern, dim3(nbx * nby * nbz, batch), dim3(block), lds, st, vol, sino, g, bx,
+                     by, bz, nbx, nby);
   return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
@@ -525,8 +552,13 @@ int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int
   const int nbx = (g.nx + bx - 1) / bx, nby = (g.ny + by - 1) / by, nbz = (g.nz + bz - 1) / bz;
   if (bx * by * bz > kAdjPerThread * block) return 1;
   const size_t lds = (size_t)((bx + 1) * by + 1) * bz * sizeof(int);
-  hipLaunchKernelGGL(k_adj_brick, dim3(nbx * nby * nbz, batch), dim3(block), lds, st, sino, vol,
-                     g, bx, by, bz, nbx, nby);
+  auto kern = &k_adj_brick<0, 0, 0>;
+#define DTV_PICK(a, b, c) \
+  if (bx == a && by == b && bz == c) kern = &k_adj_brick<a, b, c>;
+  DTV_BRICK_SHAPES(DTV_PICK)
+#undef DTV_PICK
+  hipLaunchKernelGGL(kern, dim3(nbx * nby * nbz, batch), dim3(block), lds, st, sino, vol, g, bx,
+                     by, bz, nbx, nby);
   return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
@@ -600,10 +632,26 @@ int launch_multi(bool adjoint, const dtv_instance* ih, const dtv_instance* id, i
   }
   const dim3 grid(persistent ? (unsigned)std::min(persistent_blocks, n_work) : (unsigned)n_work);
   int* ctr = persistent ? counter : nullptr;
-  if (adjoint)
-    hipLaunchKernelGGL(k_adj_brick_multi, grid, dim3(block), lds, st, id, wd, n_work, ctr);
-  else
-    hipLaunchKernelGGL(k_fwd_brick_multi, grid, dim3(block), lds, st, id, wd, n_work, ctr);
+  // one compiled shape when every instance uses the same listed brick shape
+  bool same = true;
+  for (int i = 1; i < n_inst; ++i)
+    same = same && ih[i].bx == ih[0].bx && ih[i].by == ih[0].by && ih[i].bz == ih[0].bz;
+  const int bx = ih[0].bx, by = ih[0].by, bz = ih[0].bz;
+  if (adjoint) {
+    auto kern = &k_adj_brick_multi<0, 0, 0>;
+#define DTV_PICK(a, b, c) \
+    if (same && bx == a && by == b && bz == c) kern = &k_adj_brick_multi<a, b, c>;
+    DTV_BRICK_SHAPES(DTV_PICK)
+#undef DTV_PICK
+    hipLaunchKernelGGL(kern, grid, dim3(block), lds, st, id, wd, n_work, ctr);
+  } else {
+    auto kern = &k_fwd_brick_multi<0, 0, 0>;
+#define DTV_PICK(a, b, c) \
+    if (same && bx == a && by == b && bz == c) kern = &k_fwd_brick_multi<a, b, c>;
+    DTV_BRICK_SHAPES(DTV_PICK)
+#undef DTV_PICK
+    hipLaunchKernelGGL(kern, grid, dim3(block), lds, st, id, wd, n_work, ctr);
+  }
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
 }
 

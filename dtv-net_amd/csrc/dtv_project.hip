@@ -388,15 +388,26 @@ int run_zero(float* p, size_t n, hipStream_t st) {
   return launch_status();
 }
 
-// Resolved kernel choice (defaults: 32x16x16 bricks, 256 lanes).
+// Resolved kernel choice.  Defaults (measured fastest on the 256^3 / 60-view
+// workload): A with 32x32x8 bricks, A^T with 16x16x32 bricks, 256 lanes.
 struct Launch {
   int algo = DTV_ALGO_BRICK;
-  int bx = 32, by = 16, bz = 16;
+  int bx = 32, by = 32, bz = 8;
   int block = 256;
 };
 
-int resolve_launch(const dtv_launch* l, Launch& out) {
-  out = Launch();
+Launch default_launch(bool adjoint) {
+  Launch L;
+  if (adjoint) {
+    L.bx = 16;
+    L.by = 16;
+    L.bz = 32;
+  }
+  return L;
+}
+
+int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint) {
+  out = default_launch(adjoint);
   if (!l) return DTV_OK;
   if (l->algorithm != DTV_ALGO_DEFAULT) {
     if (l->algorithm != DTV_ALGO_RAY && l->algorithm != DTV_ALGO_BRICK)
@@ -421,7 +432,7 @@ int resolve_launch(const dtv_launch* l, Launch& out) {
 }
 
 int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st,
-                const Launch& L = Launch()) {
+                const Launch& L = default_launch(false)) {
   if (L.algo == DTV_ALGO_BRICK) {
     int rc = run_zero(sino, (size_t)batch * g.n_views * g.nu * g.nv, st);
     if (rc) return rc;
@@ -435,7 +446,7 @@ int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipS
 }
 
 int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st,
-                const Launch& L = Launch()) {
+                const Launch& L = default_launch(true)) {
   if (L.algo == DTV_ALGO_BRICK)
     return launch_adj_brick(sino, vol, batch, g, L.bx, L.by, L.bz, L.block, st);
   int rc = run_zero(vol, (size_t)batch * g.nx * g.ny * g.nz, st);
@@ -619,7 +630,7 @@ int dtv_cone_forward_ex(const float* volume, float* sino, int32_t batch,
   rc = fill_table_geo(g, volume_size, detector_size, views, n_views, scheme);
   if (rc) return rc;
   Launch L;
-  rc = resolve_launch(launch, L);
+  rc = resolve_launch(launch, L, false);
   if (rc) return rc;
   return run_forward(volume, sino, batch, g, (hipStream_t)stream, L);
 }
@@ -635,7 +646,7 @@ int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,
   rc = fill_table_geo(g, volume_size, detector_size, views, n_views, scheme);
   if (rc) return rc;
   Launch L;
-  rc = resolve_launch(launch, L);
+  rc = resolve_launch(launch, L, true);
   if (rc) return rc;
   return run_adjoint(sino, volume, batch, g, (hipStream_t)stream, L);
 }
