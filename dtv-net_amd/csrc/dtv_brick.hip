@@ -84,36 +84,28 @@ __device__ void brick_rect(const Geo& g, const View& v, const ViewProj& vp, cons
   __syncthreads();
 }
 
-// Sample window of a ray inside the box p in [lo, hi) on every axis,
-// computed in FP32 from the FP64 start point and widened by one sample at
-// each end (every sample is then membership-checked exactly).
-__device__ __forceinline__ void box_window(const double P0[3], const double d[3],
-                                           const double lo[3], const double hi[3], int kend,
-                                           int& ka, int& kb) {
-  float klo = 0.f, khi = (float)kend;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const float dc = (float)d[c];
-    const float l = (float)(lo[c] - P0[c]);
-    const float h = (float)(hi[c] - P0[c]);
-    if (dc != 0.f) {
-      const float r = 1.0f / dc;
-      const float t0 = l * r, t1 = h * r;
-      klo = fmaxf(klo, fminf(t0, t1));
-      khi = fminf(khi, fmaxf(t0, t1));
-    } else if (!(l <= 0.f && h > 0.f)) {
-      khi = -2.f;
-    }
-  }
-  if (!(khi >= klo - 2.f)) {
-    ka = kb = 0;
-    return;
-  }
-  ka = (int)floorf(klo) - 1;
-  kb = (int)ceilf(khi) + 2;
-  if (ka < 0) ka = 0;
-  if (kb > kend) kb = kend;
-}
+// ---- ray scheduling ----------------------------------------------------------
+//
+// Per (brick, view) the rectangle of candidate rays is taken in segments of
+// kSchedR rays per lane.  Pass 1 tests each candidate's line against the
+// brick's sample box in FP32 with half a voxel of slack (no ray that has a
+// sample in the box can fail it) and estimates its chord in samples; the
+// hits are counting-sorted by chord, longest first, into an LDS list.  Pass
+// 2 marches the list: each wave gets rays of similar length, and rays that
+// miss the brick cost no wave time.  Pass 2 solves every ray's run of
+// samples in the brick exactly (exact_window), so its sample loop carries no
+// membership test.
+
+constexpr int kSchedR = 8;       // candidate rays per lane per segment
+constexpr int kSchedBins = 64;   // chord classes, one per sample (the last open-ended)
+constexpr int kSchedCap = kSchedR * 256;
+
+struct RaySched {
+  unsigned hist[kSchedBins];     // hits per class; cleared by the scan
+  unsigned offs[kSchedBins];     // first list slot of each class
+  unsigned n;                    // hits in this segment
+  unsigned short list[kSchedCap];  // segment-relative candidate indices
+};
 
 // Per-workgroup LDS state shared by the brick bodies.
 struct BrickSmem {
@@ -122,7 +114,153 @@ struct BrickSmem {
   double ca[8], cb[8], cw[8];
   int cok[8], rect[4];
   unsigned vmax_bits;
+  float fs[3], fd0[3], fu[3], fv[3];  // the view in FP32 for pass 1
 };
+
+__device__ __forceinline__ void stage_view(const Geo& g, int a, BrickSmem& sm) {
+  View v;
+  load_view(g, a, v);
+  sm.sv = v;
+  ViewProj p;
+  view_proj(v, p);
+  sm.sp = p;
+  for (int c = 0; c < 3; ++c) {
+    sm.fs[c] = (float)v.S[c];
+    sm.fd0[c] = (float)v.D0[c];
+    sm.fu[c] = (float)v.U[c];
+    sm.fv[c] = (float)v.V[c];
+  }
+  sm.vmax_bits = 0u;
+}
+
+// q = i / n, r = i % n for 0 <= i < 2^24 (float reciprocal rn = 1/n, one
+// correction step: the estimate is within one of the quotient)
+__device__ __forceinline__ void divmod(int i, int n, float rn, int& q, int& r) {
+  q = (int)((float)i * rn);
+  r = i - q * n;
+  if (r < 0) {
+    --q;
+    r += n;
+  } else if (r >= n) {
+    ++q;
+    r -= n;
+  }
+}
+
+// Length in samples (one per voxel of path) of the pixel (iu, iv)'s line
+// inside the box [blo, bhi], or -1 when the line misses it.  FP32; the
+// callers' boxes carry half a voxel of slack, far above the rounding here.
+__device__ __forceinline__ float chord_samples(const BrickSmem& sm, int iu, int iv,
+                                               const float blo[3], const float bhi[3]) {
+  float tin = -3.0e38f, tout = 3.0e38f, rr = 0.f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float rc = sm.fd0[c] + (float)iu * sm.fu[c] + (float)iv * sm.fv[c] - sm.fs[c];
+    rr += rc * rc;
+    rc = rc != 0.f ? rc : 1e-30f;  // parallel: huge parameters, right sign of the test
+    const float ir = __builtin_amdgcn_rcpf(rc);
+    const float t0 = (blo[c] - sm.fs[c]) * ir, t1 = (bhi[c] - sm.fs[c]) * ir;
+    tin = fmaxf(tin, fminf(t0, t1));
+    tout = fminf(tout, fmaxf(t0, t1));
+  }
+  return tout >= tin ? (tout - tin) * sqrtf(rr) : -1.f;
+}
+
+// Pass-1 bookkeeping of one hit: class and rank within the class.
+__device__ __forceinline__ unsigned sched_hit(RaySched& rs, float chord) {
+  const unsigned key = (unsigned)min((int)chord, kSchedBins - 1);
+  return (key << 16) | atomicAdd(&rs.hist[key], 1u);
+}
+
+// Counting sort of the segment's hits (slot[j] = class << 16 | rank, or ~0)
+// into rs.list, longest class first; rs.n = number of hits.
+__device__ __forceinline__ void sched_sort(RaySched& rs, const unsigned (&slot)[kSchedR]) {
+  __syncthreads();
+  if (threadIdx.x < kSchedBins) {
+    const int l = threadIdx.x;
+    const int key = kSchedBins - 1 - l;
+    const unsigned h = rs.hist[key];
+    unsigned incl = h;
+#pragma unroll
+    for (int off = 1; off < kSchedBins; off <<= 1) {
+      const unsigned o = __shfl_up(incl, off, kSchedBins);
+      if (l >= off) incl += o;
+    }
+    rs.offs[key] = incl - h;
+    rs.hist[key] = 0u;
+    if (l == kSchedBins - 1) rs.n = incl;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSchedR; ++j)
+    if (slot[j] != ~0u)
+      rs.list[rs.offs[slot[j] >> 16] + (slot[j] & 0xffffu)] =
+          (unsigned short)(threadIdx.x + j * blockDim.x);
+  __syncthreads();
+}
+
+// Candidate index -> pixel.  Forward: row-major over the rectangle.
+// Adjoint: the rectangle split into its nine stride-3 sub-lattices, so rays
+// that end up side by side in a wave are mostly three pixels apart and
+// rarely add into one voxel in the same instruction.
+struct RectMap {
+  int u0, v0, nur, nvr, mu, mv, w;  // w = row length of the index space
+  float rw;
+  int n;                            // index space size
+  __device__ void init(const int* rect, bool interleave) {
+    u0 = rect[0];
+    v0 = rect[2];
+    nur = rect[1] - rect[0] + 1;
+    nvr = rect[3] - rect[2] + 1;
+    mu = (nur + 2) / 3;
+    mv = (nvr + 2) / 3;
+    w = interleave ? 3 * mu : nur;
+    n = interleave ? 9 * mu * mv : nur * nvr;
+    rw = 1.f / (float)w;
+  }
+  // false for the padding of the interleaved space
+  __device__ __forceinline__ bool pixel(int i, bool interleave, int& iu, int& iv) const {
+    int cv, cu;
+    divmod(i, w, rw, cv, cu);
+    if (!interleave) {
+      iu = u0 + cu;
+      iv = v0 + cv;
+      return true;
+    }
+    const int su = (cu >= mu) + (cu >= 2 * mu), sv = (cv >= mv) + (cv >= 2 * mv);
+    const int du = 3 * (cu - su * mu) + su, dv = 3 * (cv - sv * mv) + sv;
+    iu = u0 + du;
+    iv = v0 + dv;
+    return du < nur && dv < nvr;
+  }
+};
+
+// Exact run [ka, kb) of the ray's samples whose floor index lies in
+// [flo, fhi], and the fixed-point position of sample ka relative to the
+// tile origin org (in units of 2^-32 voxel) with the fixed-point step.
+__device__ __forceinline__ bool ray_run(const Geo& g, const View& v, int iu, int iv,
+                                        const int flo[3], const int fhi[3],
+                                        const double org[3], int& ka, int& kb, int64_t X[3],
+                                        int64_t D[3]) {
+  double d[3], P0[3];
+  int kend;
+  ray_core(g, v, iu, iv, d, P0, kend);
+  double Pd[3], Dd[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    Pd[c] = rint(P0[c] * kFix);  // == to_fix(P0[c]), kept as an integral double
+    Dd[c] = rint(d[c] * kFix);
+  }
+  exact_window(Pd, Dd, flo, fhi, kend, ka, kb);
+  if (kb <= ka) return false;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    // integers below 2^53 throughout: exact
+    X[c] = (int64_t)(Pd[c] + (double)ka * Dd[c] - org[c]);
+    D[c] = (int64_t)Dd[c];
+  }
+  return true;
+}
 
 // A over one brick of one volume.  vb: this volume, sbase: its sinogram
 // (accumulated into with atomics), tile: (bx+1)*(by+1)*(bz+1) floats of LDS.
@@ -134,15 +272,9 @@ template <int CBX, int CBY, int CBZ>
 __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
                                                float* __restrict__ sbase, const Geo& g,
                                                int bx_, int by_, int bz_, int nbx, int nby,
-                                               int brick, float* tile, BrickSmem& sm) {
+                                               int brick, float* tile, BrickSmem& sm,
+                                               RaySched& rs) {
   const int bx = CBX ? CBX : bx_, by = CBY ? CBY : by_, bz = CBZ ? CBZ : bz_;
-  View& sv = sm.sv;
-  ViewProj& sp = sm.sp;
-  double* ca = sm.ca;
-  double* cb = sm.cb;
-  double* cw = sm.cw;
-  int* cok = sm.cok;
-  int* rect = sm.rect;
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   // bricks tile floor(p) over [-1, N-1]; the tile holds voxels [o, o+b]
   const int ox = -1 + jx * bx, oy = -1 + jy * by, oz = -1 + jz * bz;
@@ -158,50 +290,54 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
       val = vb[((size_t)z * g.ny + y) * g.nx + x];
     tile[i] = val;
   }
+  // samples with floor(p) in [o, o+e-1] belong to this brick
+  const int flo[3] = {ox, oy, oz}, fhi[3] = {ox + ex - 1, oy + ey - 1, oz + ez - 1};
   const double lo[3] = {(double)ox, (double)oy, (double)oz};
   const double hi[3] = {(double)(ox + ex), (double)(oy + ey), (double)(oz + ez)};
+  const float blo[3] = {ox - 0.5f, oy - 0.5f, oz - 0.5f};
+  const float bhi[3] = {ox + ex + 0.5f, oy + ey + 0.5f, oz + ez + 0.5f};
+  const double org[3] = {ox * kFix, oy * kFix, oz * kFix};
   const float scale = (float)g.scale;
   const int sty = tx, stz = tx * ty;
 
   for (int a = 0; a < g.n_views; ++a) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-      View v;
-      load_view(g, a, v);
-      sv = v;
-      ViewProj p;
-      view_proj(v, p);
-      sp = p;
-    }
+    if (threadIdx.x == 0) stage_view(g, a, sm);
     __syncthreads();
-    brick_rect(g, sv, sp, lo, hi, ca, cb, cw, cok, rect);
-    const int u0 = rect[0], nur = rect[1] - rect[0] + 1;
-    const int v0 = rect[2], nvr = rect[3] - rect[2] + 1;
-    if (nur <= 0 || nvr <= 0) continue;
-    const View v = sv;
+    brick_rect(g, sm.sv, sm.sp, lo, hi, sm.ca, sm.cb, sm.cw, sm.cok, sm.rect);
+    RectMap rm;
+    rm.init(sm.rect, false);
+    if (rm.nur <= 0 || rm.nvr <= 0) continue;
     float* srow = sbase + (size_t)a * g.nv * g.nu;
-    for (int r = threadIdx.x; r < nur * nvr; r += blockDim.x) {
-      const int iu = u0 + r % nur, iv = v0 + r / nur;
-      double d[3], P0[3];
-      int kend;
-      ray_core(g, v, iu, iv, d, P0, kend);
-      int ka, kb;
-      box_window(P0, d, lo, hi, kend, ka, kb);
-      if (kb <= ka) continue;
-      const int64_t Dx = to_fix(d[0]), Dy = to_fix(d[1]), Dz = to_fix(d[2]);
-      int64_t px = to_fix(P0[0]) + (int64_t)ka * Dx;
-      int64_t py = to_fix(P0[1]) + (int64_t)ka * Dy;
-      int64_t pz = to_fix(P0[2]) + (int64_t)ka * Dz;
-      float acc = 0.f;
-      bool hit = false;
-      for (int k = ka; k < kb; ++k) {
-        const int ix = (int)(px >> 32) - ox, iy = (int)(py >> 32) - oy,
-                  iz = (int)(pz >> 32) - oz;
-        if ((unsigned)ix < (unsigned)ex && (unsigned)iy < (unsigned)ey &&
-            (unsigned)iz < (unsigned)ez) {
-          const float fx = (float)(uint32_t)px * kInvFix;
-          const float fy = (float)(uint32_t)py * kInvFix;
-          const float fz = (float)(uint32_t)pz * kInvFix;
+    for (int seg = 0; seg < rm.n; seg += kSchedR * blockDim.x) {
+      unsigned slot[kSchedR];
+#pragma unroll
+      for (int j = 0; j < kSchedR; ++j) {
+        // one candidate at a time: interleaving the unrolled candidates would
+        // only raise register pressure
+        asm volatile("" ::: "memory");
+        const int i = seg + threadIdx.x + j * blockDim.x;
+        slot[j] = ~0u;
+        int iu, iv;
+        if (i < rm.n && rm.pixel(i, false, iu, iv)) {
+          const float ch = chord_samples(sm, iu, iv, blo, bhi);
+          if (ch >= 0.f) slot[j] = sched_hit(rs, ch);
+        }
+      }
+      sched_sort(rs, slot);
+      const int nhit = (int)rs.n;
+      for (int p = threadIdx.x; p < nhit; p += blockDim.x) {
+        int iu, iv;
+        rm.pixel(seg + rs.list[p], false, iu, iv);
+        int ka, kb;
+        int64_t X[3], D[3];
+        if (!ray_run(g, sm.sv, iu, iv, flo, fhi, org, ka, kb, X, D)) continue;
+        float acc = 0.f;
+        for (int k = ka; k < kb; ++k) {
+          const int ix = (int)(X[0] >> 32), iy = (int)(X[1] >> 32), iz = (int)(X[2] >> 32);
+          const float fx = (float)(uint32_t)X[0] * kInvFix;
+          const float fy = (float)(uint32_t)X[1] * kInvFix;
+          const float fz = (float)(uint32_t)X[2] * kInvFix;
           const float* t0 = tile + iz * stz + iy * sty + ix;
           const float c000 = t0[0], c100 = t0[1], c010 = t0[sty], c110 = t0[sty + 1];
           const float c001 = t0[stz], c101 = t0[stz + 1], c011 = t0[stz + sty],
@@ -213,13 +349,12 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
           const float b0 = a0 + fy * (a1 - a0);
           const float b1 = a2 + fy * (a3 - a2);
           acc += b0 + fz * (b1 - b0);
-          hit = true;
+          X[0] += D[0];
+          X[1] += D[1];
+          X[2] += D[2];
         }
-        px += Dx;
-        py += Dy;
-        pz += Dz;
+        unsafeAtomicAdd(srow + (size_t)iv * g.nu + iu, acc * scale);
       }
-      if (hit) unsafeAtomicAdd(srow + (size_t)iv * g.nu + iu, acc * scale);
     }
   }
 }
@@ -230,10 +365,12 @@ __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol
                                                    int by, int bz, int nbx, int nby) {
   extern __shared__ float tile[];
   __shared__ BrickSmem sm;
+  __shared__ RaySched rs;
+  if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
   const int b = blockIdx.y;
   fwd_brick_body<CBX, CBY, CBZ>(vol + (size_t)b * g.nx * g.ny * g.nz,
                  sino + (size_t)b * g.n_views * g.nv * g.nu, g, bx, by, bz, nbx, nby,
-                 blockIdx.x, tile, sm);
+                 blockIdx.x, tile, sm, rs);
 }
 
 // Upper bound on sum_k w over the samples of one view that touch one voxel c
@@ -271,156 +408,160 @@ __device__ __forceinline__ float weight_per_voxel_bound(const double* ca, const 
 
 constexpr int kAdjPerThread = 32;  // owned voxels per lane (bx*by*bz <= 32*block)
 
+// Nearest integer, halves rounded up (v_cvt_rpi_i32_f32: one instruction
+// where the compiler would emit a round and a convert); deterministic, which
+// is all the fixed-point accumulation needs.
+__device__ __forceinline__ int round_to_int(float x) {
+  int r;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 // Matched A^T over one brick of one volume.  sbase: this volume's sinogram,
 // vb: its volume (the brick's voxels are written once), iacc: per-view
-// accumulators, fixed-point int32 in LDS with rows padded to odd strides
-// (bx+1, (bx+1)*by+1) so lanes differing in y or z use different banks.  A
-// view whose rays carry a non-finite value falls back to FP32 LDS atomics
+// accumulators, fixed-point int32 in LDS over the brick plus a one-voxel
+// halo, so every tap of every sample in a ray's run is an unconditional
+// atomic (the halo is never read back).  Rows are padded to odd strides
+// (bx+3, (bx+3)*(by+2)+1) so lanes differing in y or z use different banks.
+// A view whose rays carry a non-finite value falls back to FP32 LDS atomics
 // in the same buffer so infinities and NaNs propagate.
 template <int CBX, int CBY, int CBZ>
 __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
                                                float* __restrict__ vb, const Geo& g, int bx_,
                                                int by_, int bz_, int nbx, int nby, int brick,
-                                               int* iacc, BrickSmem& sm) {
+                                               int* iacc, BrickSmem& sm, RaySched& rs) {
   const int bx = CBX ? CBX : bx_, by = CBY ? CBY : by_, bz = CBZ ? CBZ : bz_;
-  View& sv = sm.sv;
-  ViewProj& sp = sm.sp;
-  double* ca = sm.ca;
-  double* cb = sm.cb;
-  double* cw = sm.cw;
-  int* cok = sm.cok;
-  int* rect = sm.rect;
-  unsigned& vmax_bits = sm.vmax_bits;
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   const int x0 = jx * bx, y0 = jy * by, z0 = jz * bz;
   const int ex = min(bx, g.nx - x0), ey = min(by, g.ny - y0), ez = min(bz, g.nz - z0);
-  const int sty = bx + 1, stz = (bx + 1) * by + 1;
+  const int sty = bx + 3, stz = (bx + 3) * (by + 2) + 1;
   const int nown = bx * by * bz;
   __syncthreads();  // the previous item of a persistent workgroup is done with LDS
-  for (int i = threadIdx.x; i < stz * bz; i += blockDim.x) iacc[i] = 0;
+  for (int i = threadIdx.x; i < stz * (bz + 2); i += blockDim.x) iacc[i] = 0;
   float vacc[kAdjPerThread];
 #pragma unroll
   for (int j = 0; j < kAdjPerThread; ++j) vacc[j] = 0.f;
   // samples with floor(p) in [x0-1, x0+ex-1] touch owned voxels
+  const int flo[3] = {x0 - 1, y0 - 1, z0 - 1}, fhi[3] = {x0 + ex - 1, y0 + ey - 1, z0 + ez - 1};
   const double lo[3] = {(double)(x0 - 1), (double)(y0 - 1), (double)(z0 - 1)};
   const double hi[3] = {(double)(x0 + ex), (double)(y0 + ey), (double)(z0 + ez)};
+  const float blo[3] = {x0 - 1.5f, y0 - 1.5f, z0 - 1.5f};
+  const float bhi[3] = {x0 + ex + 0.5f, y0 + ey + 0.5f, z0 + ez + 0.5f};
+  // tile origin = the halo corner (x0-1, y0-1, z0-1)
+  const double org[3] = {(x0 - 1) * kFix, (y0 - 1) * kFix, (z0 - 1) * kFix};
   const float emin = (float)(min(ex, min(ey, ez)) + 1);
   const float gscale = (float)g.scale;
   float* facc = reinterpret_cast<float*>(iacc);
 
   for (int a = 0; a < g.n_views; ++a) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-      View v;
-      load_view(g, a, v);
-      sv = v;
-      ViewProj p;
-      view_proj(v, p);
-      sp = p;
-      vmax_bits = 0u;
-    }
+    if (threadIdx.x == 0) stage_view(g, a, sm);
     __syncthreads();
-    brick_rect(g, sv, sp, lo, hi, ca, cb, cw, cok, rect);
-    const int u0 = rect[0], nur = rect[1] - rect[0] + 1;
-    const int v0 = rect[2], nvr = rect[3] - rect[2] + 1;
-    if (nur <= 0 || nvr <= 0) continue;
+    brick_rect(g, sm.sv, sm.sp, lo, hi, sm.ca, sm.cb, sm.cw, sm.cok, sm.rect);
+    RectMap rm;
+    rm.init(sm.rect, true);
+    if (rm.nur <= 0 || rm.nvr <= 0) continue;
     const float* srow = sbase + (size_t)a * g.nv * g.nu;
-    // pass 1: largest |value| among this view's rays through the brick
+    // largest |value| among this view's rays through the brick
     float m = 0.f;
-    for (int r = threadIdx.x; r < nur * nvr; r += blockDim.x) {
-      const int iu = u0 + r % nur, iv = v0 + r / nur;
-      m = fmaxf(m, fabsf(srow[(size_t)iv * g.nu + iu] * gscale));
-      if (!(fabsf(srow[(size_t)iv * g.nu + iu]) <= 3.0e38f)) m = __int_as_float(0x7f800000);
+    for (int r = threadIdx.x; r < rm.nur * rm.nvr; r += blockDim.x) {
+      int dv, du;
+      divmod(r, rm.nur, 1.f / (float)rm.nur, dv, du);
+      const float s = srow[(size_t)(rm.v0 + dv) * g.nu + rm.u0 + du];
+      m = fmaxf(m, fabsf(s * gscale));
+      if (!(fabsf(s) <= 3.0e38f)) m = __int_as_float(0x7f800000);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(&vmax_bits, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0) atomicMax(&sm.vmax_bits, __float_as_uint(m));
     __syncthreads();
-    const float vmax = __uint_as_float(vmax_bits);
+    const float vmax = __uint_as_float(sm.vmax_bits);
     if (vmax == 0.f) continue;  // uniform: every contribution is zero
     const bool fixed = vmax <= 3.0e38f;
     // power-of-two scale so that one voxel's sum over this view stays below
     // 2^30 in magnitude
     float qscale = 1.f, qinv = 1.f;
     if (fixed) {
-      const float bound = weight_per_voxel_bound(ca, cb, cw, cok, emin, nur, nvr) * vmax;
+      const float bound =
+          weight_per_voxel_bound(sm.ca, sm.cb, sm.cw, sm.cok, emin, rm.nur, rm.nvr) * vmax;
       int e;
       frexpf(bound, &e);  // bound < 2^e
       qscale = ldexpf(1.f, 30 - e);
       qinv = ldexpf(1.f, e - 30);
     }
-    const View v = sv;
-    // pass 2: scatter.  Lanes take every third ray along u and v so rays
-    // sharing a wave rarely add into the same voxel in one instruction.
-    const int mu = (nur + 2) / 3, mv = (nvr + 2) / 3;
-    for (int r = threadIdx.x; r < 9 * mu * mv; r += blockDim.x) {
-      const int cu = r % (3 * mu), cv = r / (3 * mu);
-      const int iu = u0 + 3 * (cu % mu) + cu / mu, iv = v0 + 3 * (cv % mv) + cv / mv;
-      if (iu > rect[1] || iv > rect[3]) continue;
-      const float val = srow[(size_t)iv * g.nu + iu] * gscale * qscale;
-      if (val == 0.f) continue;
-      double d[3], P0[3];
-      int kend;
-      ray_core(g, v, iu, iv, d, P0, kend);
-      int ka, kb;
-      box_window(P0, d, lo, hi, kend, ka, kb);
-      if (kb <= ka) continue;
-      const int64_t Dx = to_fix(d[0]), Dy = to_fix(d[1]), Dz = to_fix(d[2]);
-      int64_t px = to_fix(P0[0]) + (int64_t)ka * Dx;
-      int64_t py = to_fix(P0[1]) + (int64_t)ka * Dy;
-      int64_t pz = to_fix(P0[2]) + (int64_t)ka * Dz;
-      for (int k = ka; k < kb; ++k) {
-        const int ix = (int)(px >> 32) - x0, iy = (int)(py >> 32) - y0,
-                  iz = (int)(pz >> 32) - z0;
-        // floor(p) relative to the brick must lie in [-1, e-1]
-        if ((unsigned)(ix + 1) <= (unsigned)ex && (unsigned)(iy + 1) <= (unsigned)ey &&
-            (unsigned)(iz + 1) <= (unsigned)ez) {
-          const float fx = (float)(uint32_t)px * kInvFix;
-          const float fy = (float)(uint32_t)py * kInvFix;
-          const float fz = (float)(uint32_t)pz * kInvFix;
-          const bool mx0 = ix >= 0, mx1 = ix + 1 < ex;
-          const bool my0 = iy >= 0, my1 = iy + 1 < ey;
-          const bool mz0 = iz >= 0, mz1 = iz + 1 < ez;
-          const float wz0 = (1.f - fz) * val, wz1 = fz * val;
-          const float w00 = (1.f - fy) * wz0, w10 = fy * wz0;
-          const float w01 = (1.f - fy) * wz1, w11 = fy * wz1;
-          const float gx = 1.f - fx;
+    const float vs = gscale * qscale;
+    for (int seg = 0; seg < rm.n; seg += kSchedR * blockDim.x) {
+      unsigned slot[kSchedR];
+#pragma unroll
+      for (int j = 0; j < kSchedR; ++j) {
+        // one candidate at a time: interleaving the unrolled candidates would
+        // only raise register pressure
+        asm volatile("" ::: "memory");
+        const int i = seg + threadIdx.x + j * blockDim.x;
+        slot[j] = ~0u;
+        int iu, iv;
+        if (i < rm.n && rm.pixel(i, true, iu, iv) && srow[(size_t)iv * g.nu + iu] != 0.f) {
+          const float ch = chord_samples(sm, iu, iv, blo, bhi);
+          if (ch >= 0.f) slot[j] = sched_hit(rs, ch);
+        }
+      }
+      sched_sort(rs, slot);
+      const int nhit = (int)rs.n;
+      for (int p = threadIdx.x; p < nhit; p += blockDim.x) {
+        int iu, iv;
+        rm.pixel(seg + rs.list[p], true, iu, iv);
+        const float val = srow[(size_t)iv * g.nu + iu] * vs;
+        int ka, kb;
+        int64_t X[3], D[3];
+        if (!ray_run(g, sm.sv, iu, iv, flo, fhi, org, ka, kb, X, D)) continue;
+        for (int k = ka; k < kb; ++k) {
+          const int ix = (int)(X[0] >> 32), iy = (int)(X[1] >> 32), iz = (int)(X[2] >> 32);
+          const float fx = (float)(uint32_t)X[0] * kInvFix;
+          const float fy = (float)(uint32_t)X[1] * kInvFix;
+          const float fz = (float)(uint32_t)X[2] * kInvFix;
+          const float wz1 = fz * val, wz0 = val - wz1;
+          const float w10 = fy * wz0, w00 = wz0 - w10;
+          const float w11 = fy * wz1, w01 = wz1 - w11;
+          const float t001 = fx * w00, t000 = w00 - t001;
+          const float t011 = fx * w10, t010 = w10 - t011;
+          const float t101 = fx * w01, t100 = w01 - t101;
+          const float t111 = fx * w11, t110 = w11 - t111;
           const int base = iz * stz + iy * sty + ix;
           if (fixed) {
-            if (mz0 && my0 && mx0) atomicAdd(iacc + base, __float2int_rn(gx * w00));
-            if (mz0 && my0 && mx1) atomicAdd(iacc + base + 1, __float2int_rn(fx * w00));
-            if (mz0 && my1 && mx0) atomicAdd(iacc + base + sty, __float2int_rn(gx * w10));
-            if (mz0 && my1 && mx1) atomicAdd(iacc + base + sty + 1, __float2int_rn(fx * w10));
-            if (mz1 && my0 && mx0) atomicAdd(iacc + base + stz, __float2int_rn(gx * w01));
-            if (mz1 && my0 && mx1) atomicAdd(iacc + base + stz + 1, __float2int_rn(fx * w01));
-            if (mz1 && my1 && mx0) atomicAdd(iacc + base + stz + sty, __float2int_rn(gx * w11));
-            if (mz1 && my1 && mx1)
-              atomicAdd(iacc + base + stz + sty + 1, __float2int_rn(fx * w11));
+            atomicAdd(iacc + base, round_to_int(t000));
+            atomicAdd(iacc + base + 1, round_to_int(t001));
+            atomicAdd(iacc + base + sty, round_to_int(t010));
+            atomicAdd(iacc + base + sty + 1, round_to_int(t011));
+            atomicAdd(iacc + base + stz, round_to_int(t100));
+            atomicAdd(iacc + base + stz + 1, round_to_int(t101));
+            atomicAdd(iacc + base + stz + sty, round_to_int(t110));
+            atomicAdd(iacc + base + stz + sty + 1, round_to_int(t111));
           } else {
-            if (mz0 && my0 && mx0) atomicAdd(facc + base, gx * w00);
-            if (mz0 && my0 && mx1) atomicAdd(facc + base + 1, fx * w00);
-            if (mz0 && my1 && mx0) atomicAdd(facc + base + sty, gx * w10);
-            if (mz0 && my1 && mx1) atomicAdd(facc + base + sty + 1, fx * w10);
-            if (mz1 && my0 && mx0) atomicAdd(facc + base + stz, gx * w01);
-            if (mz1 && my0 && mx1) atomicAdd(facc + base + stz + 1, fx * w01);
-            if (mz1 && my1 && mx0) atomicAdd(facc + base + stz + sty, gx * w11);
-            if (mz1 && my1 && mx1) atomicAdd(facc + base + stz + sty + 1, fx * w11);
+            atomicAdd(facc + base, t000);
+            atomicAdd(facc + base + 1, t001);
+            atomicAdd(facc + base + sty, t010);
+            atomicAdd(facc + base + sty + 1, t011);
+            atomicAdd(facc + base + stz, t100);
+            atomicAdd(facc + base + stz + 1, t101);
+            atomicAdd(facc + base + stz + sty, t110);
+            atomicAdd(facc + base + stz + sty + 1, t111);
           }
+          X[0] += D[0];
+          X[1] += D[1];
+          X[2] += D[2];
         }
-        px += Dx;
-        py += Dy;
-        pz += Dz;
       }
     }
     __syncthreads();
-    // flush this view into the lane's FP32 registers (fixed view order:
-    // the result is bitwise reproducible) and clear the accumulators
+    // flush this view's owned voxels into the lane's FP32 registers (fixed
+    // view order: the result is bitwise reproducible) and clear them; the
+    // halo keeps whatever it gathered and is never read
 #pragma unroll
     for (int j = 0; j < kAdjPerThread; ++j) {
       const int i = threadIdx.x + j * blockDim.x;
       if (i < nown) {
         const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
-        const int li = lz * stz + ly * sty + lx;
+        const int li = (lz + 1) * stz + (ly + 1) * sty + lx + 1;
         vacc[j] += fixed ? (float)iacc[li] * qinv : facc[li];
         iacc[li] = 0;
       }
@@ -445,10 +586,12 @@ __global__ __launch_bounds__(256, 3) void k_adj_brick(const float* __restrict__ 
                                                    int by, int bz, int nbx, int nby) {
   extern __shared__ int iacc[];
   __shared__ BrickSmem sm;
+  __shared__ RaySched rs;
+  if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
   const int b = blockIdx.y;
   adj_brick_body<CBX, CBY, CBZ>(sino + (size_t)b * g.n_views * g.nv * g.nu,
                  vol + (size_t)b * g.nx * g.ny * g.nz, g, bx, by, bz, nbx, nby, blockIdx.x,
-                 iacc, sm);
+                 iacc, sm, rs);
 }
 
 // ---- uneven instances in one launch ----------------------------------------
@@ -489,14 +632,16 @@ __global__ __launch_bounds__(256) void k_fwd_brick_multi(const dtv_instance* __r
                                                          int n_work, int* counter) {
   extern __shared__ float tile[];
   __shared__ BrickSmem sm;
+  __shared__ RaySched rs;
   __shared__ int slot;
+  if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
   for (int done = 0;; ++done) {
     const int w = next_item(counter, &slot, done);
     if (w >= n_work) break;
     const dtv_work_item it = work[w];
     const dtv_instance in = inst[it.instance];
     const Geo g = instance_geo(in);
-    fwd_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, tile, sm);
+    fwd_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, tile, sm, rs);
   }
 }
 
@@ -506,14 +651,16 @@ __global__ __launch_bounds__(256, 3) void k_adj_brick_multi(const dtv_instance* 
                                                          int n_work, int* counter) {
   extern __shared__ int iacc[];
   __shared__ BrickSmem sm;
+  __shared__ RaySched rs;
   __shared__ int slot;
+  if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
   for (int done = 0;; ++done) {
     const int w = next_item(counter, &slot, done);
     if (w >= n_work) break;
     const dtv_work_item it = work[w];
     const dtv_instance in = inst[it.instance];
     const Geo g = instance_geo(in);
-    adj_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, iacc, sm);
+    adj_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, iacc, sm, rs);
   }
 }
 
@@ -536,7 +683,7 @@ int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int
                      int bz, int block, hipStream_t st) {
   const int nbx = (g.nx + 1 + bx - 1) / bx, nby = (g.ny + 1 + by - 1) / by,
             nbz = (g.nz + 1 + bz - 1) / bz;
-  const size_t lds = (size_t)(bx + 1) * (by + 1) * (bz + 1) * sizeof(float);
+  const size_t lds = fwd_tile_bytes(bx, by, bz);
   auto kern = &k_fwd_brick<0, 0, 0>;
 #define DTV_PICK(a, b, c) \
   if (bx == a && by == b && bz == c) kern = &k_fwd_brick<a, b, c>;
@@ -551,7 +698,7 @@ int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int
                      int bz, int block, hipStream_t st) {
   const int nbx = (g.nx + bx - 1) / bx, nby = (g.ny + by - 1) / by, nbz = (g.nz + bz - 1) / bz;
   if (bx * by * bz > kAdjPerThread * block) return 1;
-  const size_t lds = (size_t)((bx + 1) * by + 1) * bz * sizeof(int);
+  const size_t lds = adj_tile_bytes(bx, by, bz);
   auto kern = &k_adj_brick<0, 0, 0>;
 #define DTV_PICK(a, b, c) \
   if (bx == a && by == b && bz == c) kern = &k_adj_brick<a, b, c>;
@@ -590,11 +737,11 @@ int validate_multi(const dtv_instance* ih, int32_t n_inst, const dtv_work_item* 
     size_t l;
     if (adjoint) {
       if (in.bx * in.by * in.bz > kAdjPerThread * block) return DTV_ERR_INVALID_ARGUMENT;
-      l = (size_t)((in.bx + 1) * in.by + 1) * in.bz * sizeof(int);
+      l = adj_tile_bytes(in.bx, in.by, in.bz);
     } else {
-      l = (size_t)(in.bx + 1) * (in.by + 1) * (in.bz + 1) * sizeof(float);
+      l = fwd_tile_bytes(in.bx, in.by, in.bz);
     }
-    if (l > 65536) return DTV_ERR_INVALID_ARGUMENT;
+    if (l > kMaxTileBytes) return DTV_ERR_INVALID_ARGUMENT;
     lds = l > lds ? l : lds;
     const size_t no = (size_t)in.n_views * in.nu * in.nv;
     max_out = no > max_out ? no : max_out;

@@ -7,6 +7,18 @@
 
 namespace dtv {
 
+// LDS of the brick kernels' tiles: A holds the brick's voxels [o, o+b]
+// (FP32), A^T its owned voxels plus a one-voxel halo as int32 with odd row
+// strides.  Both stay below kMaxTileBytes so that, with the per-workgroup
+// scheduling state (~5 KiB), a workgroup fits in 64 KiB of LDS.
+inline size_t fwd_tile_bytes(int bx, int by, int bz) {
+  return (size_t)(bx + 1) * (by + 1) * (bz + 1) * 4;
+}
+inline size_t adj_tile_bytes(int bx, int by, int bz) {
+  return (size_t)((bx + 3) * (by + 2) + 1) * (bz + 2) * 4;
+}
+constexpr size_t kMaxTileBytes = 56 * 1024;
+
 int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int bx, int by,
                      int bz, int block, hipStream_t st);
 int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int bx, int by,

@@ -417,8 +417,8 @@ int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint) {
   if (l->brick[0] || l->brick[1] || l->brick[2]) {
     for (int i = 0; i < 3; ++i)
       if (l->brick[i] < 2 || l->brick[i] > 64) return DTV_ERR_INVALID_ARGUMENT;
-    // the forward tile holds (b+1)^3 floats, the adjoint b^3; keep both in 64 KiB
-    if ((size_t)(l->brick[0] + 1) * (l->brick[1] + 1) * (l->brick[2] + 1) * 4 > 65536)
+    if (fwd_tile_bytes(l->brick[0], l->brick[1], l->brick[2]) > kMaxTileBytes ||
+        adj_tile_bytes(l->brick[0], l->brick[1], l->brick[2]) > kMaxTileBytes)
       return DTV_ERR_INVALID_ARGUMENT;
     out.bx = l->brick[0];
     out.by = l->brick[1];

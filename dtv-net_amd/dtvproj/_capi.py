@@ -76,6 +76,10 @@ _SIGS = {
                             ctypes.POINTER(DtvScheme), c_void_p, c_void_p],
     "dtv_cone_adjoint_ex": [c_void_p, c_void_p, c_int32, c_int32_p, c_int32_p, c_void_p, c_int32,
                             ctypes.POINTER(DtvScheme), c_void_p, c_void_p],
+    "dtv_cone_forward_multi": [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32,
+                               c_void_p, c_int32, c_void_p],
+    "dtv_cone_adjoint_multi": [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32,
+                               c_void_p, c_int32, c_void_p],
     "dtv_tv_fista_step": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32_p,
                           ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                           c_int32, c_int32, c_void_p],

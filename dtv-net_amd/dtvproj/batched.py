@@ -45,15 +45,6 @@ class WorkItem(ctypes.Structure):
     _fields_ = [("instance", ctypes.c_int32), ("brick", ctypes.c_int32)]
 
 
-def _sig(lib):
-    for name in ("dtv_cone_forward_multi", "dtv_cone_adjoint_multi"):
-        fn = getattr(lib, name)
-        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
-                       ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
-                       ctypes.c_int32, ctypes.c_void_p]
-        fn.restype = ctypes.c_int32
-
-
 class _Bound:
     """Host + device copies of one instance table and its work list."""
 
@@ -86,7 +77,6 @@ class MultiProjector:
         self.bricks = bricks or [((32, 32, 8), (16, 16, 32))] * len(self.geoms)
         self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.views = [g.device_views(self.device) for g in self.geoms]
-        _sig(_capi.lib())
 
     def _table(self, adjoint, ins, outs, inits):
         n = len(self.geoms)
