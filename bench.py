@@ -105,6 +105,29 @@ def batched_unroll(log):
                 iterations=iters)
 
 
+def ref_backprojector(sino, steps, log):
+    """Reference backprojector B_ref (Standard scheme, backwardKernel.cu:22-53) at the same
+    sizes: 60 views, 384x384 detector, 256^3 volume, SID 750 / SDD 1375 pixel units (the
+    config's 600 / 1100 mm at a 0.8 mm pixel).  Voxel-driven, 16 B per voxel-view."""
+    import torch
+    from dtvproj import geometry as G, ops
+    g = G.standard(G.uniform_angles(60), (256, 256, 256), (384, 384), 750.0, 1375.0)
+    ops.bp_ref(sino, g)
+    stream = torch.cuda.current_stream(sino.device)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        ops.bp_ref(sino, g)
+    e1.record(stream)
+    e1.synchronize()
+    t = e0.elapsed_time(e1) / steps
+    vv = 256 ** 3 * 60
+    log(f"reference backprojector: {t:.3f} ms ({vv * 16 / t / 1e6:.0f} GB/s algorithmic)")
+    return dict(ms=t, voxel_views=vv, algorithmic_gbs=vv * 16 / t / 1e6,
+                geometry="Standard scheme, 60 views, 384x384, 256^3, SID 750 / SDD 1375 px")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +224,8 @@ def main():
                 algorithmic_bytes=dict(forward=acct["forward"], adjoint=acct["adjoint"],
                                        samples=acct["samples"]))
 
+    bref = ref_backprojector(s, max(1, min(args.steps, 10)), log)
+
     cpu = None
     if not args.no_cpu_baseline:
         views = list(range(0, 60, max(1, 60 // max(1, args.cpu_views))))[:args.cpu_views]
@@ -226,6 +251,7 @@ def main():
                    "batch": 1, "views": 60, "volume": [256, 256, 256], "detector": [384, 384]},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "reference_backprojector": bref,
         "batched_unroll": bat,
     }
     print(json.dumps(line), flush=True)

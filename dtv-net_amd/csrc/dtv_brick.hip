@@ -263,7 +263,8 @@ __device__ __forceinline__ bool ray_run(const Geo& g, const View& v, int iu, int
 }
 
 // A over one brick of one volume.  vb: this volume, sbase: its sinogram
-// (accumulated into with atomics), tile: (bx+1)*(by+1)*(bz+1) floats of LDS.
+// (accumulated into with atomics), tile: the (bx+1)*(by+1)*(bz+1) voxels
+// [o, o+b] in LDS (FP32, z stride fwd_tile_stz).
 //
 // CBX/CBY/CBZ: the brick shape when known at compile time (0 = take the
 // run-time value); the instantiated shapes turn every index computation into
@@ -280,6 +281,7 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
   const int ox = -1 + jx * bx, oy = -1 + jy * by, oz = -1 + jz * bz;
   const int ex = min(bx, g.nx - ox), ey = min(by, g.ny - oy), ez = min(bz, g.nz - oz);
   const int tx = bx + 1, ty = by + 1, tz = bz + 1;
+  const int sty = tx, stz = fwd_tile_stz(bx, by);
   __syncthreads();  // the previous item of a persistent workgroup is done with LDS
   for (int i = threadIdx.x; i < tx * ty * tz; i += blockDim.x) {
     const int lx = i % tx, ly = (i / tx) % ty, lz = i / (tx * ty);
@@ -288,7 +290,7 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
     if ((unsigned)x < (unsigned)g.nx && (unsigned)y < (unsigned)g.ny &&
         (unsigned)z < (unsigned)g.nz)
       val = vb[((size_t)z * g.ny + y) * g.nx + x];
-    tile[i] = val;
+    tile[lz * stz + ly * sty + lx] = val;
   }
   // samples with floor(p) in [o, o+e-1] belong to this brick
   const int flo[3] = {ox, oy, oz}, fhi[3] = {ox + ex - 1, oy + ey - 1, oz + ez - 1};
@@ -298,7 +300,6 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
   const float bhi[3] = {ox + ex + 0.5f, oy + ey + 0.5f, oz + ez + 0.5f};
   const double org[3] = {ox * kFix, oy * kFix, oz * kFix};
   const float scale = (float)g.scale;
-  const int sty = tx, stz = tx * ty;
 
   for (int a = 0; a < g.n_views; ++a) {
     __syncthreads();

@@ -8,11 +8,17 @@
 namespace dtv {
 
 // LDS of the brick kernels' tiles: A holds the brick's voxels [o, o+b]
-// (FP32), A^T its owned voxels plus a one-voxel halo as int32 with odd row
+// (FP32, z stride padded), A^T its owned voxels plus a one-voxel halo as int32 with odd row
 // strides.  Both stay below kMaxTileBytes so that, with the per-workgroup
 // scheduling state (~5 KiB), a workgroup fits in 64 KiB of LDS.
+// A's z stride is padded to 17 (mod 32) so that rays one detector row apart,
+// which sit one z plane apart in the tile, fall on far-apart banks.
+__host__ __device__ inline int fwd_tile_stz(int bx, int by) {
+  const int p = (bx + 1) * (by + 1);
+  return p + ((17 - p) & 31);
+}
 inline size_t fwd_tile_bytes(int bx, int by, int bz) {
-  return (size_t)(bx + 1) * (by + 1) * (bz + 1) * 4;
+  return (size_t)fwd_tile_stz(bx, by) * (bz + 1) * 4;
 }
 inline size_t adj_tile_bytes(int bx, int by, int bz) {
   return (size_t)((bx + 3) * (by + 2) + 1) * (bz + 2) * 4;

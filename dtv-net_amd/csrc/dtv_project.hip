@@ -389,7 +389,7 @@ int run_zero(float* p, size_t n, hipStream_t st) {
 }
 
 // Resolved kernel choice.  Defaults (measured fastest on the 256^3 / 60-view
-// workload): A with 32x32x8 bricks, A^T with 16x16x32 bricks, 256 lanes.
+// workload): A with 32x32x8 bricks, A^T with 32x16x16 bricks, 256 lanes.
 struct Launch {
   int algo = DTV_ALGO_BRICK;
   int bx = 32, by = 32, bz = 8;
@@ -399,9 +399,9 @@ struct Launch {
 Launch default_launch(bool adjoint) {
   Launch L;
   if (adjoint) {
-    L.bx = 16;
+    L.bx = 32;
     L.by = 16;
-    L.bz = 32;
+    L.bz = 16;
   }
   return L;
 }
