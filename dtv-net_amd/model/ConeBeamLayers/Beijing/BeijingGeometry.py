@@ -24,6 +24,7 @@ import numpy as np
 import torch
 
 import JITBeijingGeometry as projector
+from dtvproj import fbp as _fbp
 from dtvproj import geometry as _G
 from options import beijingAngleNum, beijingPlanes, beijingSubDetectorSize, beijingVolumeSize, \
     beijingParameterRoot, beijingSID, beijingSDD, sampleInterval
@@ -138,11 +139,10 @@ class BeijingGeometryWithFBP(torch.nn.Module):
         U, V = beijingSubDetectorSize
         B = x.shape[0]
         residual = ForwardProjection.apply(x) - p
-        residual = residual.view(B, 1, beijingAngleNum, V, U * beijingPlanes)
-        residual = residual * self.cosWeight
-        residual = residual.view(B, 1, beijingAngleNum * V, U * beijingPlanes)
-        residual = torch.nn.functional.conv2d(residual, self.ramp, stride=1,
-                                              padding=(0, int(U * beijingPlanes / 2)))
+        # rows of the tiled detector (all planes side by side, the public
+        # layout): cosine weight and Ram-Lak row convolution in one pass
+        residual = residual.reshape(B, 1, beijingAngleNum, V, U * beijingPlanes)
+        residual = _fbp.row_filter(residual, self.ramp, self.cosWeight[0, 0, 0])
         residual = residual.view(B, 1, beijingAngleNum * beijingPlanes, V, U)
         return x - self.lamb * BackProjection.apply(residual)
 

@@ -18,6 +18,7 @@ import numpy as np
 import torch
 
 import JITStandardGeometry as projector
+from dtvproj import fbp as _fbp
 from options import standardAngleNum, standardDetectorSize, standardVolumeSize, standardSID, \
     standardSDD
 
@@ -85,12 +86,9 @@ class StandardGeometryWithFBP(torch.nn.Module):
 
     def forward(self, x, p):
         residual = ForwardProjection.apply(x) - p
-        residual = residual * self.cosWeight
-        b, c, a, h, w = residual.size()
-        residual = residual.view(b, c, a * h, w)
-        residual = torch.nn.functional.conv2d(residual, self.ramp, stride=1,
-                                              padding=(0, int(int(detectorSize[0]) / 2)))
-        residual = residual.view(b, c, a, h, w)
+        # cosine weight and the Ram-Lak row convolution in one gfx950 pass
+        # (the reference's residual * cosWeight, then conv2d with padding U/2)
+        residual = _fbp.row_filter(residual, self.ramp, self.cosWeight[0, 0, 0])
         return self.lamb * BackProjection.apply(residual)
 
     def __cosWeight__(self):

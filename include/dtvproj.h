@@ -211,6 +211,21 @@ int dtv_tv_fista_step(const float* y, const float* g, const float* xprev, float*
 int dtv_tv_fista_step_multi(const dtv_tv_item* items, int32_t n_items, int64_t max_voxels,
                             int32_t mode, void* stream);
 
+/* ---- FBP front end ------------------------------------------------------ */
+
+/* Cosine weighting fused with the per-row filtering of the FBP modules:
+ *   output[r][x] = sum_k taps[k] * weight[r % weight_rows][j] * input[r][j],
+ *   j = x + k - (n_taps - 1) / 2, terms with j outside [0, width) dropped,
+ * i.e. ``conv2d(input * cosWeight, ramp, padding=(0, width/2))`` of
+ * StandardGeometry.py:55-65 and BeijingGeometry.py:60-63 (a cross-
+ * correlation; n_taps odd, output width = width).  input/output: device
+ * float32 [rows][width], distinct; weight: device [weight_rows][width] or
+ * NULL for none; taps: device float32 [n_taps].  Needs
+ * (width + 2*roundup8(n_taps) + 8) * 4 bytes <= 64 KiB. */
+int dtv_row_filter(const float* input, float* output, int64_t rows, int32_t width,
+                   const float* weight, int32_t weight_rows, const float* taps,
+                   int32_t n_taps, void* stream);
+
 /* ---- Host-only helpers (no GPU work) ---------------------------------- */
 
 /* Canonical view table and scheme of the Standard geometry, computed on the

@@ -1,0 +1,139 @@
+"""Golden fixtures (tests/golden/oracle_small.npz, written by
+tests/golden/make_golden.py from the FP64 oracle).
+
+CPU: the oracle still reproduces every stored output from the stored inputs
+(regression freeze), and the stored pairs satisfy the identities they must:
+<A x, y> = <x, A^T y> for every scheme, and the reference backprojectors are
+*not* the adjoint (the survey's central quirk).  GPU (``-m gpu``): the HIP
+path meets the stored outputs to the parity bar, for both kernel families."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIX = os.path.join(ROOT, "tests", "golden", "oracle_small.npz")
+
+
+@pytest.fixture(scope="module")
+def fx():
+    with np.load(FIX) as f:
+        return {k: f[k] for k in f.files}
+
+
+def _gen():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "make_golden", os.path.join(ROOT, "tests", "golden", "make_golden.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_oracle_reproduces_fixtures(fx):
+    fresh = _gen().build()
+    assert set(fresh) == set(fx)
+    for k, v in fresh.items():
+        ref = fx[k]
+        assert v.shape == ref.shape, k
+        scale = max(np.abs(ref).max(), 1e-300)
+        assert np.abs(v - ref).max() <= 1e-12 * scale, k
+
+
+@pytest.mark.parametrize("p", ["std", "bj", "circ"])
+def test_fixture_adjoint_identity(fx, p):
+    lhs = np.vdot(fx[f"{p}_forward"], fx[f"{p}_sino"])
+    rhs = np.vdot(fx[f"{p}_vol"], fx[f"{p}_adjoint"])
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+
+
+@pytest.mark.parametrize("p", ["std", "bj"])
+def test_fixture_reference_backprojector_is_not_the_adjoint(fx, p):
+    a, b = fx[f"{p}_adjoint"], fx[f"{p}_bp_ref"]
+    assert np.linalg.norm(a - b) > 0.1 * np.linalg.norm(a)
+
+
+def test_fixture_tv_transpose_pairs(fx):
+    v = fx["tv_vol"]
+    for mode in ("neumann", "reference"):
+        for a in "xyz":
+            # <D v, v> = <v, D^T v>
+            assert np.isclose(np.vdot(fx[f"tv_{mode}_D{a}"], v), np.vdot(v, fx[f"tv_{mode}_DT{a}"]),
+                              rtol=1e-12, atol=1e-12)
+
+
+# ---- GPU: the HIP path against the fixtures --------------------------------
+
+TOL = 1e-5
+LAUNCHES = [None, dict(algorithm=1), dict(algorithm=2, brick=(8, 8, 8), block=64)]
+
+
+def _r(a):
+    return torch.from_numpy(np.ascontiguousarray(a))
+
+
+def _t(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev, torch.float32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("launch", LAUNCHES)
+def test_gpu_standard_fixture(cuda, fx, launch):
+    import JITStandardGeometry as J
+    from dtvproj import geometry as G, ops
+    STD = _gen().STD
+    print(f"[case] golden standard launch={launch}", flush=True)
+    vs, ds = STD["vol"], STD["det"]
+    ang = fx["std_angles"]
+    g = G.standard(ang, vs, ds, STD["sid"], STD["sdd"])
+    assert rel_l2(ops.forward(_t(fx["std_vol"], cuda), g, launch), _r(fx["std_forward"])) <= TOL
+    assert rel_l2(ops.adjoint(_t(fx["std_sino"], cuda), g, launch), _r(fx["std_adjoint"])) <= TOL
+    args = (torch.from_numpy(ang).to(cuda), torch.tensor(vs).to(cuda), torch.tensor(ds).to(cuda),
+            STD["sid"], STD["sdd"], 0)
+    assert rel_l2(J.forward(_t(fx["std_vol"], cuda), *args), _r(fx["std_forward"])) <= TOL
+    assert rel_l2(J.backward(_t(fx["std_sino"], cuda), *args), _r(fx["std_bp_ref"])) <= TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("launch", LAUNCHES)
+def test_gpu_beijing_fixture(cuda, fx, launch):
+    import JITBeijingGeometry as J
+    from dtvproj import geometry as G, ops
+    BJ = _gen().BJ
+    print(f"[case] golden beijing launch={launch}", flush=True)
+    vs, ds = BJ["vol"], BJ["det"]
+    P = fx["bj_vectors"]
+    g = G.beijing(P, vs, ds)
+    assert rel_l2(ops.forward(_t(fx["bj_vol"], cuda), g, launch), _r(fx["bj_forward"])) <= TOL
+    assert rel_l2(ops.adjoint(_t(fx["bj_sino"], cuda), g, launch), _r(fx["bj_adjoint"])) <= TOL
+    args = (torch.tensor(vs, dtype=torch.int32).to(cuda), torch.tensor(ds, dtype=torch.int32).to(cuda),
+            torch.from_numpy(P).to(cuda), 0)
+    assert rel_l2(J.backward(_t(fx["bj_sino"], cuda), *args), _r(fx["bj_bp_ref"])) <= TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("launch", LAUNCHES)
+def test_gpu_circular_fixture(cuda, fx, launch):
+    from dtvproj import geometry as G, ops
+    CIRC = _gen().CIRC
+    print(f"[case] golden circular launch={launch}", flush=True)
+    g = G.circular(fx["circ_angles"], CIRC["vol"], CIRC["voxel"], CIRC["det"], CIRC["pitch"],
+                   CIRC["sod"], CIRC["sdd"])
+    assert rel_l2(ops.forward(_t(fx["circ_vol"], cuda), g, launch), _r(fx["circ_forward"])) <= TOL
+    assert rel_l2(ops.adjoint(_t(fx["circ_sino"], cuda), g, launch), _r(fx["circ_adjoint"])) <= TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["neumann", "reference"])
+def test_gpu_tv_fixture(cuda, fx, mode):
+    from dtvproj import tv
+    TV = _gen().TV
+    print(f"[case] golden tv mode={mode}", flush=True)
+    v = _t(fx["tv_vol"], cuda)
+    for a in "xyz":
+        assert rel_l2(tv.D(v, a, mode), _r(fx[f"tv_{mode}_D{a}"])) <= TOL
+        assert rel_l2(tv.DT(v, a, mode), _r(fx[f"tv_{mode}_DT{a}"])) <= TOL
+    assert rel_l2(tv.tv_eps_grad(v, TV["eps"], mode), _r(fx[f"tv_{mode}_grad"])) <= TOL

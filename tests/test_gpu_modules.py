@@ -1,0 +1,189 @@
+"""GPU: the module layer of the drop-in (reference modules and loaders) on
+the gfx950 kernels, against compositions built from the FP64 oracle.
+
+Module globals are patched down to small geometries so the oracle finishes
+in seconds; the code paths are the ones the reference configuration runs."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+from oracle import cone
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _correlate_rows(x, taps, weight=None):
+    """FP64 conv2d(x * weight, taps, padding=(0, (len-1)/2)) over rows."""
+    x = np.asarray(x, dtype=np.float64)
+    W = x.shape[-1]
+    rows = x.reshape(-1, W).copy()
+    if weight is not None:
+        w = np.asarray(weight, dtype=np.float64).reshape(-1, W)
+        rows *= np.tile(w, (rows.shape[0] // w.shape[0], 1))
+    t = np.asarray(taps, dtype=np.float64).reshape(-1)
+    pad = (t.size - 1) // 2
+    padded = np.pad(rows, ((0, 0), (pad, pad)))
+    out = np.stack([np.correlate(r, t, mode="valid") for r in padded])
+    return out.reshape(x.shape)
+
+
+@pytest.mark.parametrize("W,ntaps,wrows", [(288, 289, 28), (1680, 1681, 5), (31, 31, 1),
+                                           (100, 7, 0), (8, 17, 3)])
+def test_row_filter_matches_fp64_correlation(cuda, W, ntaps, wrows):
+    from dtvproj import fbp
+    print(f"[case] row_filter W={W} ntaps={ntaps} wrows={wrows}", flush=True)
+    g = torch.Generator().manual_seed(W + ntaps)
+    rows = 3 * max(wrows, 1) * 2
+    x = torch.randn((rows, W), generator=g)
+    t = torch.randn(ntaps, generator=g)
+    w = torch.rand((wrows, W), generator=g) + 0.5 if wrows else None
+    out = fbp.row_filter(x.to(cuda), t, None if w is None else w.to(cuda))
+    ref = _correlate_rows(x.numpy(), t.numpy(), None if w is None else w.numpy())
+    assert rel_l2(out, torch.from_numpy(ref)) <= 1e-6
+
+
+def test_row_filter_rejects_bad_arguments(cuda):
+    from dtvproj import fbp
+    x = torch.zeros((4, 16), device=cuda)
+    with pytest.raises(RuntimeError, match="odd"):
+        fbp.row_filter(x, torch.ones(16))
+    with pytest.raises(RuntimeError, match="too large"):
+        fbp.row_filter(torch.zeros((1, 9000), device=cuda), torch.ones(9001))
+
+
+# ---- Standard modules -------------------------------------------------------
+
+SMALL_STD = dict(vol=(24, 20, 12), det=(40, 14), n_views=12)
+
+
+@pytest.fixture
+def std_small(monkeypatch):
+    import model.ConeBeamLayers.Standard.StandardGeometry as S
+    X, Y, Z = SMALL_STD["vol"]
+    U, V = SMALL_STD["det"]
+    n = SMALL_STD["n_views"]
+    monkeypatch.setattr(S, "angles", torch.from_numpy(np.linspace(0, 360, n, False)))
+    monkeypatch.setattr(S, "volumeSize", torch.tensor([X, Y, Z]))
+    monkeypatch.setattr(S, "detectorSize", torch.tensor([U, V]))
+    monkeypatch.setattr(S, "standardAngleNum", n)
+    return S
+
+
+@pytest.mark.parametrize("mode", ["matched", "reference"])
+def test_standard_fbp_module(cuda, std_small, monkeypatch, mode):
+    S = std_small
+    monkeypatch.setattr(S, "adjoint_mode", mode)
+    print(f"[case] standard fbp module mode={mode}", flush=True)
+    X, Y, Z = SMALL_STD["vol"]
+    U, V = SMALL_STD["det"]
+    n = SMALL_STD["n_views"]
+    g = torch.Generator().manual_seed(5)
+    x = torch.randn((1, 1, Z, Y, X), generator=g)
+    p = torch.randn((1, 1, n, V, U), generator=g)
+    m = S.StandardGeometryWithFBP().to(cuda)
+    out = m(x.to(cuda), p.to(cuda))
+    ang = S.angles.numpy()
+    args = (ang, SMALL_STD["vol"], SMALL_STD["det"], S.standardSID, S.standardSDD)
+    r = cone.std_forward(x, *args) - p.double()
+    r = _correlate_rows(r.numpy(), m.ramp.cpu().numpy(), m.cosWeight[0, 0, 0].cpu().numpy())
+    r = torch.from_numpy(r)
+    back = cone.std_adjoint(r, *args) if mode == "matched" else cone.std_bp_ref(r, *args)
+    assert rel_l2(out, back) <= TOL
+
+
+def test_standard_step_module(cuda, std_small):
+    S = std_small
+    print("[case] standard step module", flush=True)
+    X, Y, Z = SMALL_STD["vol"]
+    U, V = SMALL_STD["det"]
+    n = SMALL_STD["n_views"]
+    g = torch.Generator().manual_seed(6)
+    x = torch.randn((2, 1, Z, Y, X), generator=g)
+    p = torch.randn((2, 1, n, V, U), generator=g)
+    out = S.StandardGeometry().to(cuda)(x.to(cuda), p.to(cuda))
+    args = (S.angles.numpy(), SMALL_STD["vol"], SMALL_STD["det"], S.standardSID, S.standardSDD)
+    ref = x.double() - cone.std_adjoint(cone.std_forward(x, *args) - p.double(), *args)
+    assert rel_l2(out, ref) <= TOL
+
+
+# ---- Beijing modules --------------------------------------------------------
+
+SMALL_BJ = dict(vol=(20, 20, 10), sub=(6, 8), angles=3, planes=21)
+
+
+@pytest.fixture
+def bj_small(monkeypatch):
+    import model.ConeBeamLayers.Beijing.BeijingGeometry as B
+    from dtvproj import geometry as G
+    X, Y, Z = SMALL_BJ["vol"]
+    U, V = SMALL_BJ["sub"]
+    A, P = SMALL_BJ["angles"], SMALL_BJ["planes"]
+    vec = G.beijing_default_vectors(A, P, (U, V), 60.0, 100.0, (0.12, 0.9))
+    for name, val in (("parameters", torch.from_numpy(vec)),
+                      ("volumeSize", torch.IntTensor([X, Y, Z])),
+                      ("detectorSize", torch.IntTensor([U, V])),
+                      ("beijingAngleNum", A), ("beijingPlanes", P),
+                      ("beijingSubDetectorSize", [U, V]), ("beijingSDD", 100.0)):
+        monkeypatch.setattr(B, name, val)
+    return B, vec
+
+
+def _bj_public(sino, A, P, V, U):
+    """projector layout [A*P, V, U] -> the module's public interleave."""
+    return sino.reshape(sino.shape[0], A, P, V, U).permute(0, 1, 3, 2, 4) \
+        .reshape(sino.shape[0], 1, A * P, V, U)
+
+
+def _bj_projector(sino, A, P, V, U):
+    return sino.reshape(sino.shape[0], A, V, P, U).permute(0, 1, 3, 2, 4) \
+        .reshape(sino.shape[0], 1, A * P, V, U)
+
+
+@pytest.mark.parametrize("mode", ["matched", "reference"])
+def test_beijing_step_and_fbp_modules(cuda, bj_small, monkeypatch, mode):
+    B, vec = bj_small
+    monkeypatch.setattr(B, "adjoint_mode", mode)
+    print(f"[case] beijing modules mode={mode}", flush=True)
+    X, Y, Z = SMALL_BJ["vol"]
+    U, V = SMALL_BJ["sub"]
+    A, P = SMALL_BJ["angles"], SMALL_BJ["planes"]
+    si = B.sampleInterval
+    g = torch.Generator().manual_seed(7)
+    x = torch.randn((1, 1, Z, Y, X), generator=g)
+    p = torch.randn((1, 1, A * P, V, U), generator=g)
+    vs, ds = SMALL_BJ["vol"], SMALL_BJ["sub"]
+    Ax = _bj_public(cone.bj_forward(x, vec, vs, ds) * si, A, P, V, U)
+
+    def back(r):
+        rp = _bj_projector(r, A, P, V, U)
+        if mode == "matched":
+            return cone.bj_adjoint(rp, vec, vs, ds) * si
+        return cone.bj_bp_ref(rp, vec, vs, ds)
+
+    step = B.BeijingGeometry().to(cuda)(x.to(cuda), p.to(cuda))
+    assert rel_l2(step, -B.BeijingGeometry().lamb * back(Ax - p.double())) <= TOL
+    m = B.BeijingGeometryWithFBP().to(cuda)
+    out = m(x.to(cuda), p.to(cuda))
+    r = (Ax - p.double()).reshape(1, 1, A, V, U * P).numpy()
+    r = _correlate_rows(r, m.ramp.cpu().numpy(), m.cosWeight[0, 0, 0].cpu().numpy())
+    r = torch.from_numpy(r).reshape(1, 1, A * P, V, U)
+    assert rel_l2(out, x.double() - back(r)) <= TOL
+
+
+def test_simulate_loader_projects_on_gpu(cuda, bj_small, tmp_path):
+    B, vec = bj_small
+    from data import simulateLoader as L
+    from dtvproj import rawio
+    print("[case] simulate loader", flush=True)
+    X, Y, Z = SMALL_BJ["vol"]
+    scale = (1, Z, Y, X)
+    v = torch.randn(scale, generator=torch.Generator().manual_seed(8)).numpy()
+    (tmp_path / "lab").mkdir()
+    rawio.write_raw(str(tmp_path / "lab" / "a.raw"), v)
+    inp, proj, img = L.Stimulated256(str(tmp_path / "lab"), cuda, scale)[0]
+    assert proj.device.type == "cpu" and torch.all(inp == 0)
+    ref = B.ForwardProjection.apply(torch.from_numpy(v)[None].to(cuda))[0].cpu()
+    assert torch.equal(proj, ref)
