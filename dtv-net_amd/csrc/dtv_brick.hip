@@ -36,52 +36,123 @@ using namespace dtv;
 
 namespace {
 
-// Rectangle of detector pixels whose rays can pass through the box
-// [lo, hi) (voxel frame); computed by 8 lanes, one per corner, and reduced
-// by lane 0.  Written to rect[4] = {u0, u1, v0, v1} (inclusive; u1 < u0 =
-// empty).
-__device__ void brick_rect(const Geo& g, const View& v, const ViewProj& vp, const double lo[3],
-                           const double hi[3], double* ca, double* cb, double* cw, int* ok,
-                           int* rect) {
-  const int t = threadIdx.x;
-  if (t < 8) {
+// ---- per-chunk view staging --------------------------------------------------
+//
+// Views are taken in chunks of kChunk.  A prologue stages a whole chunk at
+// once, one (view, corner) pair per lane: each lane projects one corner of the
+// brick's sample box onto its view's detector, the 8 lanes of a view reduce
+// their corners with shuffles, and the first of them stores the view's
+// rectangle of candidate pixels (the bounding box of the projected corners
+// with a pixel of slack, the whole detector when a corner is not in front of
+// the source), its FP32 vectors for the pass-1 line test and, for A^T, the
+// bound on one voxel's weight sum used to pick the fixed-point scale.
+
+constexpr int kChunk = 32;
+
+struct ChunkStage {
+  float fview[kChunk][12];  // S, D0, U, V in FP32
+  int rect[kChunk][4];      // u0, u1, v0, v1 (inclusive; u1 < u0 = empty)
+  float wbound[kChunk];     // A^T: bound on sum_k w for one voxel and view
+  unsigned vmax[kChunk];    // A^T: largest |value| * scale over the rectangle (float bits)
+  View sv[2];               // FP64 view of the current view (double-buffered)
+};
+
+// Bound on sum_k w over the samples of one view that touch one voxel c of the
+// box (w = the sample's trilinear weight on c):
+//  * per ray, the weights on c are bounded by a 1-D tent along the ray's
+//    dominant axis sampled at spacing |d_ax| >= 1/sqrt(3): sum <= 3;
+//  * rays touching c pass through its support cube (side 2), so their pixels
+//    lie in the cube's projected footprint.  Along u that footprint is at
+//    most sum_k (2 / e_k) du_k times the depth ratio across the box
+//    (perspective), +10%, where du_k is the largest projected length of the
+//    box's edges along axis k and e_k the box's extent along k (triangle
+//    inequality over the cube's three edge directions); likewise along v.  A
+//    footprint of f pixels holds <= f + 1 pixel centres per axis (+0.5 slack).
+// Depths are |(corner - S).n|, proportional to the distance from the source
+// along the detector normal.
+__device__ __forceinline__ float weight_per_voxel_bound(bool all, const double du[3],
+                                                        const double dv[3], double wmin,
+                                                        double wmax, const float ext[3],
+                                                        int nur, int nvr) {
+  if (!all || !(wmin > 0.0)) return 3.0f * (float)nur * (float)nvr;
+  const double persp = wmax / wmin * 1.1;
+  double fu = 0.0, fv = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    fu += du[k] * 2.0 / ext[k];
+    fv += dv[k] * 2.0 / ext[k];
+  }
+  const float pu = (float)(fu * persp) + 1.5f, pv = (float)(fv * persp) + 1.5f;
+  return 3.0f * fminf(pu, (float)nur + 1.f) * fminf(pv, (float)nvr + 1.f);
+}
+
+// Stages views [c0, c0+n) of the box [lo, hi) (voxel frame) into st.
+__device__ void stage_chunk(const Geo& g, int c0, int n, const double lo[3], const double hi[3],
+                            bool bound, ChunkStage& st) {
+  const float ext[3] = {(float)(hi[0] - lo[0]), (float)(hi[1] - lo[1]), (float)(hi[2] - lo[2])};
+  for (int pi = threadIdx.x; pi < n * 8; pi += blockDim.x) {
+    const int vi = pi >> 3, t = pi & 7;
+    View v;
+    load_view(g, c0 + vi, v);
+    ViewProj vp;
+    view_proj(v, vp);
     const double c[3] = {(t & 1) ? hi[0] : lo[0], (t & 2) ? hi[1] : lo[1],
                          (t & 4) ? hi[2] : lo[2]};
     double a = 0.0, b = 0.0;
-    ok[t] = project_point(v, vp, c, a, b) ? 1 : 0;
-    ca[t] = a;
-    cb[t] = b;
+    int ok = project_point(v, vp, c, a, b) ? 1 : 0;
     const double w[3] = {c[0] - v.S[0], c[1] - v.S[1], c[2] - v.S[2]};
-    cw[t] = dot3(w, vp.n);
-  }
-  __syncthreads();
-  if (t == 0) {
-    bool all = true;
-    double amin = 1e300, amax = -1e300, bmin = 1e300, bmax = -1e300;
-    for (int i = 0; i < 8; ++i) {
-      all = all && ok[i];
-      amin = fmin(amin, ca[i]);
-      amax = fmax(amax, ca[i]);
-      bmin = fmin(bmin, cb[i]);
-      bmax = fmax(bmax, cb[i]);
+    const double dw = fabs(dot3(w, vp.n));
+    double amin = a, amax = a, bmin = b, bmax = b, wmin = dw, wmax = dw;
+    // the 8 corners of a view are 8 consecutive lanes of one wave; corner
+    // t ^ (1 << k) is the other end of corner t's edge along axis k
+    double du[3], dv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      du[k] = fabs(a - __shfl_xor(a, 1 << k, 64));
+      dv[k] = fabs(b - __shfl_xor(b, 1 << k, 64));
     }
-    int u0 = 0, u1 = g.nu - 1, v0 = 0, v1 = g.nv - 1;
-    if (all) {
-      // pixel index i has its ray through coordinate i exactly; keep one
-      // pixel of slack on each side against rounding
-      const double fu0 = floor(amin) - 1.0, fu1 = ceil(amax) + 1.0;
-      const double fv0 = floor(bmin) - 1.0, fv1 = ceil(bmax) + 1.0;
-      u0 = fu0 < 0.0 ? 0 : (fu0 > g.nu ? g.nu : (int)fu0);
-      u1 = fu1 < -1.0 ? -1 : (fu1 > g.nu - 1 ? g.nu - 1 : (int)fu1);
-      v0 = fv0 < 0.0 ? 0 : (fv0 > g.nv ? g.nv : (int)fv0);
-      v1 = fv1 < -1.0 ? -1 : (fv1 > g.nv - 1 ? g.nv - 1 : (int)fv1);
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        du[k] = fmax(du[k], __shfl_xor(du[k], off, 64));
+        dv[k] = fmax(dv[k], __shfl_xor(dv[k], off, 64));
+      }
+      amin = fmin(amin, __shfl_xor(amin, off, 64));
+      amax = fmax(amax, __shfl_xor(amax, off, 64));
+      bmin = fmin(bmin, __shfl_xor(bmin, off, 64));
+      bmax = fmax(bmax, __shfl_xor(bmax, off, 64));
+      wmin = fmin(wmin, __shfl_xor(wmin, off, 64));
+      wmax = fmax(wmax, __shfl_xor(wmax, off, 64));
+      ok &= __shfl_xor(ok, off, 64);
     }
-    rect[0] = u0;
-    rect[1] = u1;
-    rect[2] = v0;
-    rect[3] = v1;
+    if (t == 0) {
+      int u0 = 0, u1 = g.nu - 1, v0 = 0, v1 = g.nv - 1;
+      if (ok) {
+        // pixel index i has its ray through coordinate i exactly; keep one
+        // pixel of slack on each side against rounding
+        const double fu0 = floor(amin) - 1.0, fu1 = ceil(amax) + 1.0;
+        const double fv0 = floor(bmin) - 1.0, fv1 = ceil(bmax) + 1.0;
+        u0 = fu0 < 0.0 ? 0 : (fu0 > g.nu ? g.nu : (int)fu0);
+        u1 = fu1 < -1.0 ? -1 : (fu1 > g.nu - 1 ? g.nu - 1 : (int)fu1);
+        v0 = fv0 < 0.0 ? 0 : (fv0 > g.nv ? g.nv : (int)fv0);
+        v1 = fv1 < -1.0 ? -1 : (fv1 > g.nv - 1 ? g.nv - 1 : (int)fv1);
+      }
+      st.rect[vi][0] = u0;
+      st.rect[vi][1] = u1;
+      st.rect[vi][2] = v0;
+      st.rect[vi][3] = v1;
+      if (bound)
+        st.wbound[vi] = weight_per_voxel_bound(ok != 0, du, dv, wmin, wmax, ext, u1 - u0 + 1,
+                                               v1 - v0 + 1);
+      st.vmax[vi] = 0u;
+      for (int k = 0; k < 3; ++k) {
+        st.fview[vi][k] = (float)v.S[k];
+        st.fview[vi][3 + k] = (float)v.D0[k];
+        st.fview[vi][6 + k] = (float)v.U[k];
+        st.fview[vi][9 + k] = (float)v.V[k];
+      }
+    }
   }
-  __syncthreads();
 }
 
 // ---- ray scheduling ----------------------------------------------------------
@@ -107,32 +178,6 @@ struct RaySched {
   unsigned short list[kSchedCap];  // segment-relative candidate indices
 };
 
-// Per-workgroup LDS state shared by the brick bodies.
-struct BrickSmem {
-  View sv;
-  ViewProj sp;
-  double ca[8], cb[8], cw[8];
-  int cok[8], rect[4];
-  unsigned vmax_bits;
-  float fs[3], fd0[3], fu[3], fv[3];  // the view in FP32 for pass 1
-};
-
-__device__ __forceinline__ void stage_view(const Geo& g, int a, BrickSmem& sm) {
-  View v;
-  load_view(g, a, v);
-  sm.sv = v;
-  ViewProj p;
-  view_proj(v, p);
-  sm.sp = p;
-  for (int c = 0; c < 3; ++c) {
-    sm.fs[c] = (float)v.S[c];
-    sm.fd0[c] = (float)v.D0[c];
-    sm.fu[c] = (float)v.U[c];
-    sm.fv[c] = (float)v.V[c];
-  }
-  sm.vmax_bits = 0u;
-}
-
 // q = i / n, r = i % n for 0 <= i < 2^24 (float reciprocal rn = 1/n, one
 // correction step: the estimate is within one of the quotient)
 __device__ __forceinline__ void divmod(int i, int n, float rn, int& q, int& r) {
@@ -147,19 +192,43 @@ __device__ __forceinline__ void divmod(int i, int n, float rn, int& q, int& r) {
   }
 }
 
+// Wave-uniform copies of values read from LDS: readfirstlane puts them in
+// scalar registers, so the per-ray set-up takes them as SGPR operands and the
+// vector registers stay free for the sample loop and the A^T accumulators.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uni(double x) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ void uni_view(const View& src, View& v) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    v.S[c] = uni(src.S[c]);
+    v.D0[c] = uni(src.D0[c]);
+    v.U[c] = uni(src.U[c]);
+    v.V[c] = uni(src.V[c]);
+  }
+}
+
 // Length in samples (one per voxel of path) of the pixel (iu, iv)'s line
-// inside the box [blo, bhi], or -1 when the line misses it.  FP32; the
-// callers' boxes carry half a voxel of slack, far above the rounding here.
-__device__ __forceinline__ float chord_samples(const BrickSmem& sm, int iu, int iv,
+// inside the box [blo, bhi], or -1 when the line misses it; fv = the view's
+// FP32 vectors.  The callers' boxes carry half a voxel of slack, far above
+// the rounding here.
+__device__ __forceinline__ float chord_samples(const float* fv, int iu, int iv,
                                                const float blo[3], const float bhi[3]) {
   float tin = -3.0e38f, tout = 3.0e38f, rr = 0.f;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    float rc = sm.fd0[c] + (float)iu * sm.fu[c] + (float)iv * sm.fv[c] - sm.fs[c];
+    float rc = fv[3 + c] + (float)iu * fv[6 + c] + (float)iv * fv[9 + c] - fv[c];
     rr += rc * rc;
     rc = rc != 0.f ? rc : 1e-30f;  // parallel: huge parameters, right sign of the test
     const float ir = __builtin_amdgcn_rcpf(rc);
-    const float t0 = (blo[c] - sm.fs[c]) * ir, t1 = (bhi[c] - sm.fs[c]) * ir;
+    const float t0 = (blo[c] - fv[c]) * ir, t1 = (bhi[c] - fv[c]) * ir;
     tin = fmaxf(tin, fminf(t0, t1));
     tout = fminf(tout, fmaxf(t0, t1));
   }
@@ -208,10 +277,10 @@ struct RectMap {
   float rw;
   int n;                            // index space size
   __device__ void init(const int* rect, bool interleave) {
-    u0 = rect[0];
-    v0 = rect[2];
-    nur = rect[1] - rect[0] + 1;
-    nvr = rect[3] - rect[2] + 1;
+    u0 = uni(rect[0]);
+    v0 = uni(rect[2]);
+    nur = uni(rect[1]) - u0 + 1;
+    nvr = uni(rect[3]) - v0 + 1;
     mu = (nur + 2) / 3;
     mv = (nvr + 2) / 3;
     w = interleave ? 3 * mu : nur;
@@ -262,6 +331,20 @@ __device__ __forceinline__ bool ray_run(const Geo& g, const View& v, int iu, int
   return true;
 }
 
+// Stages view a's FP64 vectors into st.sv[slot] (one lane); ordered before
+// their readers by the first barrier of the view's sort, and the other slot
+// keeps the previously processed view for lanes still in its pass 2 (the
+// slot alternates per processed view, so its last readers have passed that
+// view's sort barriers).
+__device__ __forceinline__ void stage_fp64_view(const Geo& g, int a, int slot,
+                                                ChunkStage& st) {
+  if (threadIdx.x == 0) {
+    View v;
+    load_view(g, a, v);
+    st.sv[slot] = v;
+  }
+}
+
 // A over one brick of one volume.  vb: this volume, sbase: its sinogram
 // (accumulated into with atomics), tile: the (bx+1)*(by+1)*(bz+1) voxels
 // [o, o+b] in LDS (FP32, z stride fwd_tile_stz).
@@ -273,7 +356,7 @@ template <int CBX, int CBY, int CBZ>
 __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
                                                float* __restrict__ sbase, const Geo& g,
                                                int bx_, int by_, int bz_, int nbx, int nby,
-                                               int brick, float* tile, BrickSmem& sm,
+                                               int brick, float* tile, ChunkStage& st,
                                                RaySched& rs) {
   const int bx = CBX ? CBX : bx_, by = CBY ? CBY : by_, bz = CBZ ? CBZ : bz_;
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
@@ -301,60 +384,72 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
   const double org[3] = {ox * kFix, oy * kFix, oz * kFix};
   const float scale = (float)g.scale;
 
-  for (int a = 0; a < g.n_views; ++a) {
+  int vslot = 0;  // st.sv slot of the next processed view
+  for (int c0 = 0; c0 < g.n_views; c0 += kChunk) {
+    const int nc = min(kChunk, g.n_views - c0);
+    __syncthreads();  // the previous chunk's readers are done with st
+    stage_chunk(g, c0, nc, lo, hi, false, st);
     __syncthreads();
-    if (threadIdx.x == 0) stage_view(g, a, sm);
-    __syncthreads();
-    brick_rect(g, sm.sv, sm.sp, lo, hi, sm.ca, sm.cb, sm.cw, sm.cok, sm.rect);
-    RectMap rm;
-    rm.init(sm.rect, false);
-    if (rm.nur <= 0 || rm.nvr <= 0) continue;
-    float* srow = sbase + (size_t)a * g.nv * g.nu;
-    for (int seg = 0; seg < rm.n; seg += kSchedR * blockDim.x) {
-      unsigned slot[kSchedR];
+    for (int vi = 0; vi < nc; ++vi) {
+      const int a = c0 + vi;
+      RectMap rm;
+      rm.init(st.rect[vi], false);
+      if (rm.nur <= 0 || rm.nvr <= 0) continue;
+      stage_fp64_view(g, a, vslot, st);
+      const int cur = vslot;
+      vslot ^= 1;
+      float fv[12];
 #pragma unroll
-      for (int j = 0; j < kSchedR; ++j) {
-        // one candidate at a time: interleaving the unrolled candidates would
-        // only raise register pressure
-        asm volatile("" ::: "memory");
-        const int i = seg + threadIdx.x + j * blockDim.x;
-        slot[j] = ~0u;
-        int iu, iv;
-        if (i < rm.n && rm.pixel(i, false, iu, iv)) {
-          const float ch = chord_samples(sm, iu, iv, blo, bhi);
-          if (ch >= 0.f) slot[j] = sched_hit(rs, ch);
+      for (int k = 0; k < 12; ++k) fv[k] = uni(st.fview[vi][k]);
+      float* srow = sbase + (size_t)a * g.nv * g.nu;
+      for (int seg = 0; seg < rm.n; seg += kSchedR * blockDim.x) {
+        unsigned slot[kSchedR];
+#pragma unroll
+        for (int j = 0; j < kSchedR; ++j) {
+          // one candidate at a time: interleaving the unrolled candidates would
+          // only raise register pressure
+          asm volatile("" ::: "memory");
+          const int i = seg + threadIdx.x + j * blockDim.x;
+          slot[j] = ~0u;
+          int iu, iv;
+          if (i < rm.n && rm.pixel(i, false, iu, iv)) {
+            const float ch = chord_samples(fv, iu, iv, blo, bhi);
+            if (ch >= 0.f) slot[j] = sched_hit(rs, ch);
+          }
         }
-      }
-      sched_sort(rs, slot);
-      const int nhit = (int)rs.n;
-      for (int p = threadIdx.x; p < nhit; p += blockDim.x) {
-        int iu, iv;
-        rm.pixel(seg + rs.list[p], false, iu, iv);
-        int ka, kb;
-        int64_t X[3], D[3];
-        if (!ray_run(g, sm.sv, iu, iv, flo, fhi, org, ka, kb, X, D)) continue;
-        float acc = 0.f;
-        for (int k = ka; k < kb; ++k) {
-          const int ix = (int)(X[0] >> 32), iy = (int)(X[1] >> 32), iz = (int)(X[2] >> 32);
-          const float fx = (float)(uint32_t)X[0] * kInvFix;
-          const float fy = (float)(uint32_t)X[1] * kInvFix;
-          const float fz = (float)(uint32_t)X[2] * kInvFix;
-          const float* t0 = tile + iz * stz + iy * sty + ix;
-          const float c000 = t0[0], c100 = t0[1], c010 = t0[sty], c110 = t0[sty + 1];
-          const float c001 = t0[stz], c101 = t0[stz + 1], c011 = t0[stz + sty],
-                      c111 = t0[stz + sty + 1];
-          const float a0 = c000 + fx * (c100 - c000);
-          const float a1 = c010 + fx * (c110 - c010);
-          const float a2 = c001 + fx * (c101 - c001);
-          const float a3 = c011 + fx * (c111 - c011);
-          const float b0 = a0 + fy * (a1 - a0);
-          const float b1 = a2 + fy * (a3 - a2);
-          acc += b0 + fz * (b1 - b0);
-          X[0] += D[0];
-          X[1] += D[1];
-          X[2] += D[2];
+        sched_sort(rs, slot);
+        const int nhit = uni((int)rs.n);
+        View v;
+        uni_view(st.sv[cur], v);
+        for (int p = threadIdx.x; p < nhit; p += blockDim.x) {
+          int iu, iv;
+          rm.pixel(seg + rs.list[p], false, iu, iv);
+          int ka, kb;
+          int64_t X[3], D[3];
+          if (!ray_run(g, v, iu, iv, flo, fhi, org, ka, kb, X, D)) continue;
+          float acc = 0.f;
+          for (int k = ka; k < kb; ++k) {
+            const int ix = (int)(X[0] >> 32), iy = (int)(X[1] >> 32), iz = (int)(X[2] >> 32);
+            const float fx = (float)(uint32_t)X[0] * kInvFix;
+            const float fy = (float)(uint32_t)X[1] * kInvFix;
+            const float fz = (float)(uint32_t)X[2] * kInvFix;
+            const float* t0 = tile + iz * stz + iy * sty + ix;
+            const float c000 = t0[0], c100 = t0[1], c010 = t0[sty], c110 = t0[sty + 1];
+            const float c001 = t0[stz], c101 = t0[stz + 1], c011 = t0[stz + sty],
+                        c111 = t0[stz + sty + 1];
+            const float a0 = c000 + fx * (c100 - c000);
+            const float a1 = c010 + fx * (c110 - c010);
+            const float a2 = c001 + fx * (c101 - c001);
+            const float a3 = c011 + fx * (c111 - c011);
+            const float b0 = a0 + fy * (a1 - a0);
+            const float b1 = a2 + fy * (a3 - a2);
+            acc += b0 + fz * (b1 - b0);
+            X[0] += D[0];
+            X[1] += D[1];
+            X[2] += D[2];
+          }
+          unsafeAtomicAdd(srow + (size_t)iv * g.nu + iu, acc * scale);
         }
-        unsafeAtomicAdd(srow + (size_t)iv * g.nu + iu, acc * scale);
       }
     }
   }
@@ -365,49 +460,22 @@ __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol
                                                    float* __restrict__ sino, Geo g, int bx,
                                                    int by, int bz, int nbx, int nby) {
   extern __shared__ float tile[];
-  __shared__ BrickSmem sm;
+  __shared__ ChunkStage st;
   __shared__ RaySched rs;
   if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
   const int b = blockIdx.y;
   fwd_brick_body<CBX, CBY, CBZ>(vol + (size_t)b * g.nx * g.ny * g.nz,
                  sino + (size_t)b * g.n_views * g.nv * g.nu, g, bx, by, bz, nbx, nby,
-                 blockIdx.x, tile, sm, rs);
-}
-
-// Upper bound on sum_k w over the samples of one view that touch one voxel c
-// of the box (w = the sample's trilinear weight on c):
-//  * per ray, the weights on c are bounded by a 1-D tent along the ray's
-//    dominant axis sampled at spacing |d_ax| >= 1/sqrt(3): sum <= 3;
-//  * rays touching c pass through its 2x2x2 support cube, so their pixels
-//    lie in the cube's projected footprint, whose extent is at most 2/e_min
-//    of the box's projected extent times the depth ratio across the box
-//    (perspective), +10%; a footprint of f pixels holds <= f + 1 pixel
-//    centres per axis (+0.5 slack).
-// The corner depths ``cw`` are (corner - S).n, proportional to the distance
-// from the source along the detector normal.
-__device__ __forceinline__ float weight_per_voxel_bound(const double* ca, const double* cb,
-                                                        const double* cw, const int* ok,
-                                                        float emin, int nur, int nvr) {
-  bool all = true;
-  double amin = 1e300, amax = -1e300, bmin = 1e300, bmax = -1e300;
-  double wmin = 1e300, wmax = -1e300;
-  for (int i = 0; i < 8; ++i) {
-    all = all && ok[i];
-    amin = fmin(amin, ca[i]);
-    amax = fmax(amax, ca[i]);
-    bmin = fmin(bmin, cb[i]);
-    bmax = fmax(bmax, cb[i]);
-    wmin = fmin(wmin, fabs(cw[i]));
-    wmax = fmax(wmax, fabs(cw[i]));
-  }
-  if (!all || !(wmin > 0.0)) return 3.0f * (float)nur * (float)nvr;
-  const double persp = wmax / wmin * 1.1;
-  const float fu = (float)((amax - amin) * 2.0 / emin * persp) + 1.5f;
-  const float fv = (float)((bmax - bmin) * 2.0 / emin * persp) + 1.5f;
-  return 3.0f * fminf(fu, (float)nur + 1.f) * fminf(fv, (float)nvr + 1.f);
+                 blockIdx.x, tile, st, rs);
 }
 
 constexpr int kAdjPerThread = 32;  // owned voxels per lane (bx*by*bz <= 32*block)
+// Views accumulated under one fixed-point scale.  The quantisation step grows
+// with the group's summed bound, so the group trades flush traffic against
+// precision; 2 views keep A^T within a few 1e-6 (relative L2) of FP32
+// atomics on N(0,1) sinograms even at near-source geometries, where the
+// footprint bound is loosest.
+constexpr int kGroup = 2;
 
 // Nearest integer, halves rounded up (v_cvt_rpi_i32_f32: one instruction
 // where the compiler would emit a round and a convert); deterministic, which
@@ -419,18 +487,25 @@ __device__ __forceinline__ int round_to_int(float x) {
 }
 
 // Matched A^T over one brick of one volume.  sbase: this volume's sinogram,
-// vb: its volume (the brick's voxels are written once), iacc: per-view
-// accumulators, fixed-point int32 in LDS over the brick plus a one-voxel
-// halo, so every tap of every sample in a ray's run is an unconditional
-// atomic (the halo is never read back).  Rows are padded to odd strides
-// (bx+3, (bx+3)*(by+2)+1) so lanes differing in y or z use different banks.
-// A view whose rays carry a non-finite value falls back to FP32 LDS atomics
-// in the same buffer so infinities and NaNs propagate.
+// vb: its volume (the brick's voxels are written once), iacc: fixed-point
+// int32 accumulators in LDS over the brick plus a one-voxel halo, so every
+// tap of every sample in a ray's run is an unconditional atomic (the halo is
+// never read back).  Rows are padded to odd strides (bx+3, (bx+3)*(by+2)+1)
+// so lanes differing in y or z use different banks.
+//
+// Views are staged kChunk at a time and accumulated kGroup at a time under
+// one power-of-two scale: the group's bound on a voxel's sum, sum over its
+// views of (weight bound x largest |value| in the view's rectangle), is kept
+// below 2^31 / 1.001, so no accumulator can overflow; after the group the
+// owned integers are scaled back into per-lane FP32 registers and cleared.
+// Integer addition is order-independent, so the result is bitwise
+// reproducible.  A group whose rays carry a non-finite value accumulates in
+// FP32 LDS atomics in the same buffer so infinities and NaNs propagate.
 template <int CBX, int CBY, int CBZ>
 __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
                                                float* __restrict__ vb, const Geo& g, int bx_,
                                                int by_, int bz_, int nbx, int nby, int brick,
-                                               int* iacc, BrickSmem& sm, RaySched& rs) {
+                                               int* iacc, ChunkStage& st, RaySched& rs) {
   const int bx = CBX ? CBX : bx_, by = CBY ? CBY : by_, bz = CBZ ? CBZ : bz_;
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   const int x0 = jx * bx, y0 = jy * by, z0 = jz * bz;
@@ -450,122 +525,157 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
   const float bhi[3] = {x0 + ex + 0.5f, y0 + ey + 0.5f, z0 + ez + 0.5f};
   // tile origin = the halo corner (x0-1, y0-1, z0-1)
   const double org[3] = {(x0 - 1) * kFix, (y0 - 1) * kFix, (z0 - 1) * kFix};
-  const float emin = (float)(min(ex, min(ey, ez)) + 1);
   const float gscale = (float)g.scale;
   float* facc = reinterpret_cast<float*>(iacc);
 
-  for (int a = 0; a < g.n_views; ++a) {
+  int vslot = 0;  // st.sv slot of the next processed view
+  for (int c0 = 0; c0 < g.n_views; c0 += kChunk) {
+    const int nc = min(kChunk, g.n_views - c0);
+    __syncthreads();  // the previous chunk's readers are done with st
+    stage_chunk(g, c0, nc, lo, hi, true, st);
     __syncthreads();
-    if (threadIdx.x == 0) stage_view(g, a, sm);
-    __syncthreads();
-    brick_rect(g, sm.sv, sm.sp, lo, hi, sm.ca, sm.cb, sm.cw, sm.cok, sm.rect);
-    RectMap rm;
-    rm.init(sm.rect, true);
-    if (rm.nur <= 0 || rm.nvr <= 0) continue;
-    const float* srow = sbase + (size_t)a * g.nv * g.nu;
-    // largest |value| among this view's rays through the brick
-    float m = 0.f;
-    for (int r = threadIdx.x; r < rm.nur * rm.nvr; r += blockDim.x) {
-      int dv, du;
-      divmod(r, rm.nur, 1.f / (float)rm.nur, dv, du);
-      const float s = srow[(size_t)(rm.v0 + dv) * g.nu + rm.u0 + du];
-      m = fmaxf(m, fabsf(s * gscale));
-      if (!(fabsf(s) <= 3.0e38f)) m = __int_as_float(0x7f800000);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(&sm.vmax_bits, __float_as_uint(m));
-    __syncthreads();
-    const float vmax = __uint_as_float(sm.vmax_bits);
-    if (vmax == 0.f) continue;  // uniform: every contribution is zero
-    const bool fixed = vmax <= 3.0e38f;
-    // power-of-two scale so that one voxel's sum over this view stays below
-    // 2^30 in magnitude
-    float qscale = 1.f, qinv = 1.f;
-    if (fixed) {
-      const float bound =
-          weight_per_voxel_bound(sm.ca, sm.cb, sm.cw, sm.cok, emin, rm.nur, rm.nvr) * vmax;
-      int e;
-      frexpf(bound, &e);  // bound < 2^e
-      qscale = ldexpf(1.f, 30 - e);
-      qinv = ldexpf(1.f, e - 30);
-    }
-    const float vs = gscale * qscale;
-    for (int seg = 0; seg < rm.n; seg += kSchedR * blockDim.x) {
-      unsigned slot[kSchedR];
-#pragma unroll
-      for (int j = 0; j < kSchedR; ++j) {
-        // one candidate at a time: interleaving the unrolled candidates would
-        // only raise register pressure
-        asm volatile("" ::: "memory");
-        const int i = seg + threadIdx.x + j * blockDim.x;
-        slot[j] = ~0u;
-        int iu, iv;
-        if (i < rm.n && rm.pixel(i, true, iu, iv) && srow[(size_t)iv * g.nu + iu] != 0.f) {
-          const float ch = chord_samples(sm, iu, iv, blo, bhi);
-          if (ch >= 0.f) slot[j] = sched_hit(rs, ch);
-        }
+    // largest |value| of each view's rectangle (non-finite -> +inf)
+    for (int vi = 0; vi < nc; ++vi) {
+      const int* rc = st.rect[vi];
+      const int nur = rc[1] - rc[0] + 1, nvr = rc[3] - rc[2] + 1;
+      if (nur <= 0 || nvr <= 0) continue;
+      const float* srow = sbase + (size_t)(c0 + vi) * g.nv * g.nu;
+      const float rn = 1.f / (float)nur;
+      float m = 0.f;
+      for (int r = threadIdx.x; r < nur * nvr; r += blockDim.x) {
+        int dv, du;
+        divmod(r, nur, rn, dv, du);
+        const float s = srow[(size_t)(rc[2] + dv) * g.nu + rc[0] + du];
+        m = fmaxf(m, fabsf(s * gscale));
+        if (!(fabsf(s) <= 3.0e38f)) m = __int_as_float(0x7f800000);
       }
-      sched_sort(rs, slot);
-      const int nhit = (int)rs.n;
-      for (int p = threadIdx.x; p < nhit; p += blockDim.x) {
-        int iu, iv;
-        rm.pixel(seg + rs.list[p], true, iu, iv);
-        const float val = srow[(size_t)iv * g.nu + iu] * vs;
-        int ka, kb;
-        int64_t X[3], D[3];
-        if (!ray_run(g, sm.sv, iu, iv, flo, fhi, org, ka, kb, X, D)) continue;
-        for (int k = ka; k < kb; ++k) {
-          const int ix = (int)(X[0] >> 32), iy = (int)(X[1] >> 32), iz = (int)(X[2] >> 32);
-          const float fx = (float)(uint32_t)X[0] * kInvFix;
-          const float fy = (float)(uint32_t)X[1] * kInvFix;
-          const float fz = (float)(uint32_t)X[2] * kInvFix;
-          const float wz1 = fz * val, wz0 = val - wz1;
-          const float w10 = fy * wz0, w00 = wz0 - w10;
-          const float w11 = fy * wz1, w01 = wz1 - w11;
-          const float t001 = fx * w00, t000 = w00 - t001;
-          const float t011 = fx * w10, t010 = w10 - t011;
-          const float t101 = fx * w01, t100 = w01 - t101;
-          const float t111 = fx * w11, t110 = w11 - t111;
-          const int base = iz * stz + iy * sty + ix;
-          if (fixed) {
-            atomicAdd(iacc + base, round_to_int(t000));
-            atomicAdd(iacc + base + 1, round_to_int(t001));
-            atomicAdd(iacc + base + sty, round_to_int(t010));
-            atomicAdd(iacc + base + sty + 1, round_to_int(t011));
-            atomicAdd(iacc + base + stz, round_to_int(t100));
-            atomicAdd(iacc + base + stz + 1, round_to_int(t101));
-            atomicAdd(iacc + base + stz + sty, round_to_int(t110));
-            atomicAdd(iacc + base + stz + sty + 1, round_to_int(t111));
-          } else {
-            atomicAdd(facc + base, t000);
-            atomicAdd(facc + base + 1, t001);
-            atomicAdd(facc + base + sty, t010);
-            atomicAdd(facc + base + sty + 1, t011);
-            atomicAdd(facc + base + stz, t100);
-            atomicAdd(facc + base + stz + 1, t101);
-            atomicAdd(facc + base + stz + sty, t110);
-            atomicAdd(facc + base + stz + sty + 1, t111);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+      if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(&st.vmax[vi], __float_as_uint(m));
+    }
+    __syncthreads();
+    for (int g0 = 0; g0 < nc; g0 += kGroup) {
+      const int ng = min(kGroup, nc - g0);
+      // the group's fixed-point scale
+      float bound = 0.f;
+      bool finite = true;
+      for (int vi = g0; vi < g0 + ng; ++vi) {
+        const float vm = __uint_as_float(st.vmax[vi]);
+        finite = finite && vm <= 3.0e38f;
+        if (vm > 0.f) bound += st.wbound[vi] * vm;
+      }
+      const bool fixed = finite && bound <= 3.0e38f;
+      float qscale = 1.f, qinv = 1.f;
+      if (fixed) {
+        // bound * (1 + 1e-3) < 2^e, so sums stay below 2^31 / 1.001: the
+        // headroom (> 2e6 units) dwarfs the rounding of the contributions
+        // (half a unit each, at most ~5/3 contributions per unit of the
+        // weight bound)
+        int e;
+        frexpf(bound * 1.001f, &e);
+        qscale = ldexpf(1.f, 31 - e);
+        qinv = ldexpf(1.f, e - 31);
+      }
+      const float vs = gscale * qscale;
+      bool any = false;
+      for (int vi = g0; vi < g0 + ng; ++vi) {
+        if (st.vmax[vi] == 0u) continue;  // empty rectangle or all-zero values
+        any = true;
+        const int a = c0 + vi;
+        RectMap rm;
+        rm.init(st.rect[vi], true);
+        stage_fp64_view(g, a, vslot, st);
+        const int cur = vslot;
+        vslot ^= 1;
+        float fv[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) fv[k] = uni(st.fview[vi][k]);
+        const float* srow = sbase + (size_t)a * g.nv * g.nu;
+        for (int seg = 0; seg < rm.n; seg += kSchedR * blockDim.x) {
+          unsigned slot[kSchedR];
+#pragma unroll
+          for (int j = 0; j < kSchedR; ++j) {
+            // one candidate at a time: interleaving the unrolled candidates would
+            // only raise register pressure
+            asm volatile("" ::: "memory");
+            const int i = seg + threadIdx.x + j * blockDim.x;
+            slot[j] = ~0u;
+            int iu, iv;
+            if (i < rm.n && rm.pixel(i, true, iu, iv) && srow[(size_t)iv * g.nu + iu] != 0.f) {
+              const float ch = chord_samples(fv, iu, iv, blo, bhi);
+              if (ch >= 0.f) slot[j] = sched_hit(rs, ch);
+            }
           }
-          X[0] += D[0];
-          X[1] += D[1];
-          X[2] += D[2];
+          sched_sort(rs, slot);
+          const int nhit = uni((int)rs.n);
+          View v;
+          uni_view(st.sv[cur], v);
+          for (int p = threadIdx.x; p < nhit; p += blockDim.x) {
+            int iu, iv;
+            rm.pixel(seg + rs.list[p], true, iu, iv);
+            const float val = srow[(size_t)iv * g.nu + iu] * vs;
+            int ka, kb;
+            int64_t X[3], D[3];
+            if (!ray_run(g, v, iu, iv, flo, fhi, org, ka, kb, X, D)) continue;
+            for (int k = ka; k < kb; ++k) {
+              const int ix = (int)(X[0] >> 32), iy = (int)(X[1] >> 32), iz = (int)(X[2] >> 32);
+              const float fx = (float)(uint32_t)X[0] * kInvFix;
+              const float fy = (float)(uint32_t)X[1] * kInvFix;
+              const float fz = (float)(uint32_t)X[2] * kInvFix;
+              const float wz1 = fz * val, wz0 = val - wz1;
+              const float w10 = fy * wz0, w00 = wz0 - w10;
+              const float w11 = fy * wz1, w01 = wz1 - w11;
+              const float t001 = fx * w00, t000 = w00 - t001;
+              const float t011 = fx * w10, t010 = w10 - t011;
+              const float t101 = fx * w01, t100 = w01 - t101;
+              const float t111 = fx * w11, t110 = w11 - t111;
+              const int base = iz * stz + iy * sty + ix;
+              if (fixed) {
+                atomicAdd(iacc + base, round_to_int(t000));
+                atomicAdd(iacc + base + 1, round_to_int(t001));
+                atomicAdd(iacc + base + sty, round_to_int(t010));
+                atomicAdd(iacc + base + sty + 1, round_to_int(t011));
+                atomicAdd(iacc + base + stz, round_to_int(t100));
+                atomicAdd(iacc + base + stz + 1, round_to_int(t101));
+                atomicAdd(iacc + base + stz + sty, round_to_int(t110));
+                atomicAdd(iacc + base + stz + sty + 1, round_to_int(t111));
+              } else {
+                atomicAdd(facc + base, t000);
+                atomicAdd(facc + base + 1, t001);
+                atomicAdd(facc + base + sty, t010);
+                atomicAdd(facc + base + sty + 1, t011);
+                atomicAdd(facc + base + stz, t100);
+                atomicAdd(facc + base + stz + 1, t101);
+                atomicAdd(facc + base + stz + sty, t110);
+                atomicAdd(facc + base + stz + sty + 1, t111);
+              }
+              X[0] += D[0];
+              X[1] += D[1];
+              X[2] += D[2];
+            }
+          }
         }
       }
-    }
-    __syncthreads();
-    // flush this view's owned voxels into the lane's FP32 registers (fixed
-    // view order: the result is bitwise reproducible) and clear them; the
-    // halo keeps whatever it gathered and is never read
+      if (!any) continue;
+      __syncthreads();
+      // flush the group's owned voxels into the lane's FP32 registers (fixed
+      // group order: the result is bitwise reproducible) and clear them; the
+      // halo keeps whatever it gathered and is never read.  The lane index is
+      // made opaque so the 32 LDS addresses are recomputed here rather than
+      // hoisted out of the view loops and held (or spilled) throughout.
+      int tid = threadIdx.x;
+      asm volatile("" : "+v"(tid));
 #pragma unroll
-    for (int j = 0; j < kAdjPerThread; ++j) {
-      const int i = threadIdx.x + j * blockDim.x;
-      if (i < nown) {
-        const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
-        const int li = (lz + 1) * stz + (ly + 1) * sty + lx + 1;
-        vacc[j] += fixed ? (float)iacc[li] * qinv : facc[li];
-        iacc[li] = 0;
+      for (int j = 0; j < kAdjPerThread; ++j) {
+        const int i = tid + j * blockDim.x;
+        if (i < nown) {
+          const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
+          const int li = (lz + 1) * stz + (ly + 1) * sty + lx + 1;
+          vacc[j] += fixed ? (float)iacc[li] * qinv : facc[li];
+          iacc[li] = 0;
+        }
       }
+      __syncthreads();  // cleared before the next group's atomics
     }
   }
 #pragma unroll
@@ -579,20 +689,19 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
   }
 }
 
-// at least 3 waves per SIMD (<= 168 VGPRs): the hoisted flush indices must
-// not cost occupancy
+// at least 3 waves per SIMD (<= 168 VGPRs)
 template <int CBX, int CBY, int CBZ>
 __global__ __launch_bounds__(256, 3) void k_adj_brick(const float* __restrict__ sino,
                                                    float* __restrict__ vol, Geo g, int bx,
                                                    int by, int bz, int nbx, int nby) {
   extern __shared__ int iacc[];
-  __shared__ BrickSmem sm;
+  __shared__ ChunkStage st;
   __shared__ RaySched rs;
   if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
   const int b = blockIdx.y;
   adj_brick_body<CBX, CBY, CBZ>(sino + (size_t)b * g.n_views * g.nv * g.nu,
                  vol + (size_t)b * g.nx * g.ny * g.nz, g, bx, by, bz, nbx, nby, blockIdx.x,
-                 iacc, sm, rs);
+                 iacc, st, rs);
 }
 
 // ---- uneven instances in one launch ----------------------------------------
@@ -632,7 +741,7 @@ __global__ __launch_bounds__(256) void k_fwd_brick_multi(const dtv_instance* __r
                                                          const dtv_work_item* __restrict__ work,
                                                          int n_work, int* counter) {
   extern __shared__ float tile[];
-  __shared__ BrickSmem sm;
+  __shared__ ChunkStage st;
   __shared__ RaySched rs;
   __shared__ int slot;
   if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
@@ -642,7 +751,7 @@ __global__ __launch_bounds__(256) void k_fwd_brick_multi(const dtv_instance* __r
     const dtv_work_item it = work[w];
     const dtv_instance in = inst[it.instance];
     const Geo g = instance_geo(in);
-    fwd_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, tile, sm, rs);
+    fwd_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, tile, st, rs);
   }
 }
 
@@ -651,7 +760,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_brick_multi(const dtv_instance* 
                                                          const dtv_work_item* __restrict__ work,
                                                          int n_work, int* counter) {
   extern __shared__ int iacc[];
-  __shared__ BrickSmem sm;
+  __shared__ ChunkStage st;
   __shared__ RaySched rs;
   __shared__ int slot;
   if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
@@ -661,7 +770,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_brick_multi(const dtv_instance* 
     const dtv_work_item it = work[w];
     const dtv_instance in = inst[it.instance];
     const Geo g = instance_geo(in);
-    adj_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, iacc, sm, rs);
+    adj_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, iacc, st, rs);
   }
 }
 

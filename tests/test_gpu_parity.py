@@ -204,9 +204,9 @@ def test_brick_vs_ray_and_oracle(cuda, launch):
     ar = ops.adjoint(y.to(cuda), g, RAY)
     ab = ops.adjoint(y.to(cuda), g, launch)
     assert rel_l2(fb, fr) <= 1e-6
-    # the brick adjoint accumulates each view in 32-bit fixed point (scale
-    # bounded so no voxel can overflow); its rounding stays below 5e-6 even
-    # at this near-source geometry
+    # the brick adjoint accumulates each group of views in 32-bit fixed point
+    # (scale bounded so no voxel can overflow); its rounding stays below 5e-6
+    # even at this near-source geometry
     assert rel_l2(ab, ar) <= 5e-6
     ref_f = cone.circ_forward(x, ang, vs, 0.9, ds, 1.1, 60.0, 130.0)
     ref_a = cone.circ_adjoint(y, ang, vs, 0.9, ds, 1.1, 60.0, 130.0)
