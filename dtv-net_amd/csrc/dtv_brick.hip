@@ -141,9 +141,14 @@ __device__ void stage_chunk(const Geo& g, int c0, int n, const double lo[3], con
       st.rect[vi][1] = u1;
       st.rect[vi][2] = v0;
       st.rect[vi][3] = v1;
+      // a corner behind the source or a depth ratio above 2 across the box
+      // (the box next to the source) makes the bound loose enough to cost
+      // fixed-point precision: +inf sends the view's group to FP32 atomics
       if (bound)
-        st.wbound[vi] = weight_per_voxel_bound(ok != 0, du, dv, wmin, wmax, ext, u1 - u0 + 1,
-                                               v1 - v0 + 1);
+        st.wbound[vi] = (ok && wmin > 0.0 && wmax <= 2.0 * wmin)
+                            ? weight_per_voxel_bound(true, du, dv, wmin, wmax, ext,
+                                                     u1 - u0 + 1, v1 - v0 + 1)
+                            : __int_as_float(0x7f800000);
       st.vmax[vi] = 0u;
       for (int k = 0; k < 3; ++k) {
         st.fview[vi][k] = (float)v.S[k];
@@ -499,8 +504,10 @@ __device__ __forceinline__ int round_to_int(float x) {
 // below 2^31 / 1.001, so no accumulator can overflow; after the group the
 // owned integers are scaled back into per-lane FP32 registers and cleared.
 // Integer addition is order-independent, so the result is bitwise
-// reproducible.  A group whose rays carry a non-finite value accumulates in
-// FP32 LDS atomics in the same buffer so infinities and NaNs propagate.
+// reproducible.  A group whose rays carry a non-finite value, or that holds a
+// view seeing the box from right next to the source (loose bound),
+// accumulates in FP32 LDS atomics in the same buffer instead, so infinities
+// and NaNs propagate and precision holds.
 template <int CBX, int CBY, int CBZ>
 __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
                                                float* __restrict__ vb, const Geo& g, int bx_,
