@@ -126,20 +126,46 @@ struct Ray {
   int k0, k1;
 };
 
+// 1/x and 1/sqrt(x) to about one ulp.  On the device: the hardware estimate
+// refined by two Newton steps in FMA form (each step squares the relative
+// error, the final FMA adds at most 2^-53); on the host: the correctly
+// rounded operations.
+DTV_HD inline double recip(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+#else
+  return 1.0 / x;
+#endif
+}
+
+DTV_HD inline double rsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);  // 1 - x y^2
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+#else
+  return 1.0 / sqrt(x);
+#endif
+}
+
 // Per-ray set-up shared by every kernel, so that A and the matched A^T see
-// bitwise-identical sample positions.  Outputs the unit direction d, the
+// bitwise-identical sample positions (the FMAs are written out, so every
+// inlined copy rounds the same way).  Outputs the unit direction d, the
 // position P0 of sample k = 0 (both voxel frame, FP64) and kend: the
 // reference's samples are k in [0, kend) (beta_start + k < beta_end).
 DTV_HD inline void ray_core(const Geo& g, const View& v, int iu, int iv, double d[3],
                             double P0[3], int& kend) {
 #pragma clang fp contract(off)
   double r[3];
-  for (int c = 0; c < 3; ++c) {
-    const double pix = v.D0[c] + iu * v.U[c] + iv * v.V[c];
-    r[c] = pix - v.S[c];
-  }
-  const double len = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-  const double il = 1.0 / len;
+  for (int c = 0; c < 3; ++c)
+    r[c] = fma((double)iv, v.V[c], fma((double)iu, v.U[c], v.D0[c])) - v.S[c];
+  const double il = rsqrt(fma(r[0], r[0], fma(r[1], r[1], r[2] * r[2])));
   d[0] = r[0] * il;
   d[1] = r[1] * il;
   d[2] = r[2] * il;
@@ -156,7 +182,7 @@ DTV_HD inline void ray_core(const Geo& g, const View& v, int iu, int iv, double 
   const double sax = ax == 0 ? v.S[0] : (ax == 1 ? v.S[1] : v.S[2]);
   const double lax = ax == 0 ? g.lo[0] : (ax == 1 ? g.lo[1] : g.lo[2]);
   const double hax = ax == 0 ? g.hi[0] : (ax == 1 ? g.hi[1] : g.hi[2]);
-  const double idax = 1.0 / dax;
+  const double idax = recip(dax);
   const double a0 = (lax - sax) * idax;
   const double a1 = (hax - sax) * idax;
   const double bs = fmin(a0, a1) - g.margin;
@@ -169,7 +195,7 @@ DTV_HD inline void ray_core(const Geo& g, const View& v, int iu, int iv, double 
     if (kend > 0 && !(bs + (double)(kend - 1) < be)) --kend;
     if (bs + (double)kend < be) ++kend;
   }
-  for (int c = 0; c < 3; ++c) P0[c] = v.S[c] + bs * d[c];
+  for (int c = 0; c < 3; ++c) P0[c] = fma(bs, d[c], v.S[c]);
 }
 
 DTV_HD inline int64_t to_fix(double x) { return (int64_t)llrint(x * kFix); }
@@ -183,7 +209,8 @@ DTV_HD inline int64_t to_fix(double x) { return (int64_t)llrint(x * kFix); }
 // opposite sides of a voxel plane (a direction component below 2^-33 rounds
 // to a zero step).  Every numerator below is an integer under 2^53 and
 // |Dd| <= 2^32, so a non-integral quotient lies at least 2^-32 from every
-// integer, while n * RN(1/Dd) carries a relative error under 2^-52: for
+// integer, while n * (1/Dd) carries a relative error under 2^-52 (recip is
+// within ~2^-53, the product adds one rounding): for
 // quotients under 2^18 in magnitude ceil(t - 2^-33) and floor(t + 2^-33) are
 // the exact integer bounds.  The bounds are solved for k - kr; a run that
 // starts beyond 2^16 is solved again around its estimated start, and bounds
@@ -192,17 +219,19 @@ DTV_HD inline void exact_window(const double Pd[3], const double Dd[3], const in
                                 const int hi[3], int kend, int& ka, int& kb) {
 #pragma clang fp contract(off)
   constexpr double kBias = 1.1641532182693481e-10;  // 2^-33
+  double rD[3];
+  for (int c = 0; c < 3; ++c) rD[c] = recip(Dd[c]);  // relative error <= ~2^-53
   int kr = 0;
   for (int pass = 0; pass < 2; ++pass) {
     double a = (double)(0 - kr), b = (double)(kend - 1 - kr);
     for (int c = 0; c < 3; ++c) {
-      const double pr = Pd[c] + (double)kr * Dd[c];
+      const double pr = pass == 0 ? Pd[c] : Pd[c] + (double)kr * Dd[c];
       const double nl = (double)lo[c] * kFix - pr;               // k D >= nl
       const double nh = ((double)hi[c] + 1.0) * kFix - 1.0 - pr;  // k D <= nh
       if (Dd[c] == 0.0) {
         if (nl > 0.0 || nh < 0.0) b = a - 1.0;
       } else {
-        const double r = 1.0 / Dd[c];
+        const double r = rD[c];
         const double t0 = nl * r, t1 = nh * r;
         const double tl = Dd[c] > 0.0 ? t0 : t1, th = Dd[c] > 0.0 ? t1 : t0;
         a = fmax(a, ceil(tl - kBias));
