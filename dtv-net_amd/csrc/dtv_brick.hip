@@ -187,7 +187,7 @@ struct RaySched {
 // correction step: the estimate is within one of the quotient)
 __device__ __forceinline__ void divmod(int i, int n, float rn, int& q, int& r) {
   q = (int)((float)i * rn);
-  r = i - q * n;
+  r = i - (int)__umul24((unsigned)q, (unsigned)n);  // q, n < 2^24
   if (r < 0) {
     --q;
     r += n;
@@ -302,7 +302,8 @@ struct RectMap {
       return true;
     }
     const int su = (cu >= mu) + (cu >= 2 * mu), sv = (cv >= mv) + (cv >= 2 * mv);
-    const int du = 3 * (cu - su * mu) + su, dv = 3 * (cv - sv * mv) + sv;
+    const int du = 3 * (cu - (int)__umul24((unsigned)su, (unsigned)mu)) + su;
+    const int dv = 3 * (cv - (int)__umul24((unsigned)sv, (unsigned)mv)) + sv;
     iu = u0 + du;
     iv = v0 + dv;
     return du < nur && dv < nvr;
@@ -438,7 +439,10 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
             const float fx = (float)(uint32_t)X[0] * kInvFix;
             const float fy = (float)(uint32_t)X[1] * kInvFix;
             const float fz = (float)(uint32_t)X[2] * kInvFix;
-            const float* t0 = tile + iz * stz + iy * sty + ix;
+            // tile indices are small and non-negative (the exact window
+            // keeps them in the tile): full-rate 24-bit multiplies
+            const float* t0 =
+                tile + (__umul24((unsigned)iz, (unsigned)stz) + __umul24((unsigned)iy, (unsigned)sty) + ix);
             const float c000 = t0[0], c100 = t0[1], c010 = t0[sty], c110 = t0[sty + 1];
             const float c001 = t0[stz], c101 = t0[stz + 1], c011 = t0[stz + sty],
                         c111 = t0[stz + sty + 1];
@@ -636,7 +640,8 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
               const float t011 = fx * w10, t010 = w10 - t011;
               const float t101 = fx * w01, t100 = w01 - t101;
               const float t111 = fx * w11, t110 = w11 - t111;
-              const int base = iz * stz + iy * sty + ix;
+              const int base =
+                (int)(__umul24((unsigned)iz, (unsigned)stz) + __umul24((unsigned)iy, (unsigned)sty)) + ix;
               if (fixed) {
                 atomicAdd(iacc + base, round_to_int(t000));
                 atomicAdd(iacc + base + 1, round_to_int(t001));

@@ -62,13 +62,19 @@ class MultiProjector:
 
     ``bricks``: per-instance (forward, adjoint) brick shapes; default
     32x32x8 / 32x16x16 (the single-call defaults), or ``"tune"`` to run the
-    autotuner per instance."""
+    autotuner per instance.  ``persistent_blocks``: workgroups that pull
+    (instance, brick) items from a device counter until the list is empty;
+    ``"auto"`` (default) = 3 per compute unit, what the brick kernels' LDS
+    and registers allow to be resident; 0 = one workgroup per item."""
 
-    def __init__(self, geoms, bricks=None, block=256, persistent_blocks=0, device=None):
+    def __init__(self, geoms, bricks=None, block=256, persistent_blocks="auto", device=None):
         self.geoms = list(geoms)
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.block = block
-        self.persistent_blocks = persistent_blocks
+        if persistent_blocks == "auto":
+            cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+            persistent_blocks = 3 * cus
+        self.persistent_blocks = int(persistent_blocks)
         if bricks == "tune":
             bricks = []
             for g in self.geoms:
@@ -147,7 +153,7 @@ class BatchedFistaTV:
     iteration, all tables static, graph-capturable)."""
 
     def __init__(self, geoms, sinos, iters=20, lam=1e-3, eps=1e-3, steps=None, bricks=None,
-                 block=256, persistent_blocks=0, mode="neumann"):
+                 block=256, persistent_blocks="auto", mode="neumann"):
         self.mp = MultiProjector(geoms, bricks, block, persistent_blocks)
         dev = self.mp.device
         self.geoms = self.mp.geoms
