@@ -388,26 +388,46 @@ int run_zero(float* p, size_t n, hipStream_t st) {
   return launch_status();
 }
 
-// Resolved kernel choice.  Defaults (measured fastest on the 256^3 / 60-view
-// workload): A with 32x32x8 bricks, A^T with 32x16x16 bricks, 256 lanes.
+// Resolved kernel choice.
 struct Launch {
   int algo = DTV_ALGO_BRICK;
   int bx = 32, by = 32, bz = 8;
   int block = 256;
 };
 
-Launch default_launch(bool adjoint) {
+// Bricks of shape (bx, by, bz) covering a volume: A's bricks tile floor(p)
+// over [-1, N-1] (one more voxel per axis), A^T's the voxels.
+long long brick_count(const Geo& g, int bx, int by, int bz, bool adjoint) {
+  const int e = adjoint ? 0 : 1;
+  return (long long)((g.nx + e + bx - 1) / bx) * ((g.ny + e + by - 1) / by) *
+         ((g.nz + e + bz - 1) / bz);
+}
+
+// Default kernels, by problem size (measured on MI355X, 256 CUs):
+//  * enough bricks to fill the GPU (>= 1024 workgroups with the batch): A on
+//    32x32x8 bricks, A^T on 32x16x16 (fastest at 256^3 / 60 views and at
+//    4 x 128^3 / 90 views);
+//  * fewer: 16x16x8 bricks (2-3x faster than the large shapes at 64^3 and
+//    single 128^3 volumes, which they leave mostly idle), and for A the
+//    one-lane-per-ray kernel when even those give under 256 workgroups
+//    (64^3 / 72 views: 0.58 vs 0.79 ms; 32^3: 0.03 vs 0.22 ms).
+Launch default_launch(bool adjoint, const Geo& g, int batch) {
   Launch L;
   if (adjoint) {
     L.bx = 32;
     L.by = 16;
     L.bz = 16;
   }
+  if (brick_count(g, L.bx, L.by, L.bz, adjoint) * batch >= 1024) return L;
+  L.bx = 16;
+  L.by = 16;
+  L.bz = 8;
+  if (!adjoint && brick_count(g, 16, 16, 8, false) * batch < 256) L.algo = DTV_ALGO_RAY;
   return L;
 }
 
-int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint) {
-  out = default_launch(adjoint);
+int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint, const Geo& g, int batch) {
+  out = default_launch(adjoint, g, batch);
   if (!l) return DTV_OK;
   if (l->algorithm != DTV_ALGO_DEFAULT) {
     if (l->algorithm != DTV_ALGO_RAY && l->algorithm != DTV_ALGO_BRICK)
@@ -423,6 +443,7 @@ int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint) {
     out.bx = l->brick[0];
     out.by = l->brick[1];
     out.bz = l->brick[2];
+    if (l->algorithm == DTV_ALGO_DEFAULT) out.algo = DTV_ALGO_BRICK;
   }
   if (l->block) {
     if (l->block < 64 || l->block > 256 || l->block % 64) return DTV_ERR_INVALID_ARGUMENT;
@@ -432,7 +453,16 @@ int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint) {
 }
 
 int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st,
-                const Launch& L = default_launch(false)) {
+                const Launch& L);
+int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st,
+                const Launch& L);
+
+int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st) {
+  return run_forward(vol, sino, batch, g, st, default_launch(false, g, batch));
+}
+
+int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st,
+                const Launch& L) {
   if (L.algo == DTV_ALGO_BRICK) {
     int rc = run_zero(sino, (size_t)batch * g.n_views * g.nu * g.nv, st);
     if (rc) return rc;
@@ -445,8 +475,12 @@ int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipS
   return launch_status();
 }
 
+int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st) {
+  return run_adjoint(sino, vol, batch, g, st, default_launch(true, g, batch));
+}
+
 int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st,
-                const Launch& L = default_launch(true)) {
+                const Launch& L) {
   if (L.algo == DTV_ALGO_BRICK)
     return launch_adj_brick(sino, vol, batch, g, L.bx, L.by, L.bz, L.block, st);
   int rc = run_zero(vol, (size_t)batch * g.nx * g.ny * g.nz, st);
@@ -630,7 +664,7 @@ int dtv_cone_forward_ex(const float* volume, float* sino, int32_t batch,
   rc = fill_table_geo(g, volume_size, detector_size, views, n_views, scheme);
   if (rc) return rc;
   Launch L;
-  rc = resolve_launch(launch, L, false);
+  rc = resolve_launch(launch, L, false, g, batch);
   if (rc) return rc;
   return run_forward(volume, sino, batch, g, (hipStream_t)stream, L);
 }
@@ -646,7 +680,7 @@ int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,
   rc = fill_table_geo(g, volume_size, detector_size, views, n_views, scheme);
   if (rc) return rc;
   Launch L;
-  rc = resolve_launch(launch, L, true);
+  rc = resolve_launch(launch, L, true, g, batch);
   if (rc) return rc;
   return run_adjoint(sino, volume, batch, g, (hipStream_t)stream, L);
 }
