@@ -479,12 +479,6 @@ __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol
 }
 
 constexpr int kAdjPerThread = 32;  // owned voxels per lane (bx*by*bz <= 32*block)
-// Views accumulated under one fixed-point scale.  The quantisation step grows
-// with the group's summed bound, so the group trades flush traffic against
-// precision; 2 views keep A^T within a few 1e-6 (relative L2) of FP32
-// atomics on N(0,1) sinograms even at near-source geometries, where the
-// footprint bound is loosest.
-constexpr int kGroup = 2;
 
 // Nearest integer, halves rounded up (v_cvt_rpi_i32_f32: one instruction
 // where the compiler would emit a round and a convert); deterministic, which
@@ -502,10 +496,13 @@ __device__ __forceinline__ int round_to_int(float x) {
 // never read back).  Rows are padded to odd strides (bx+3, (bx+3)*(by+2)+1)
 // so lanes differing in y or z use different banks.
 //
-// Views are staged kChunk at a time and accumulated kGroup at a time under
-// one power-of-two scale: the group's bound on a voxel's sum, sum over its
-// views of (weight bound x largest |value| in the view's rectangle), is kept
-// below 2^31 / 1.001, so no accumulator can overflow; after the group the
+// Views are staged kChunk at a time and accumulated in groups under one
+// power-of-two scale: the group's bound on a voxel's sum, sum over its views
+// of (weight bound x largest |value| in the view's rectangle), is kept below
+// 2^31 / 1.001, so no accumulator can overflow, and a group only grows while
+// that sum stays within twice its largest view's bound, so its step is never
+// coarser than one view's at 2^30 (the flush traffic shrinks without any
+// precision cost; uniform views pair up); after the group the
 // owned integers are scaled back into per-lane FP32 registers and cleared.
 // Integer addition is order-independent, so the result is bitwise
 // reproducible.  A group whose rays carry a non-finite value, or that holds a
@@ -565,15 +562,24 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
       if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(&st.vmax[vi], __float_as_uint(m));
     }
     __syncthreads();
-    for (int g0 = 0; g0 < nc; g0 += kGroup) {
-      const int ng = min(kGroup, nc - g0);
-      // the group's fixed-point scale
-      float bound = 0.f;
+    for (int g0 = 0, ng = 0; g0 < nc; g0 += ng) {
+      // the group: consecutive views while their summed bound stays within
+      // twice the largest one's (its step at 2^31 is then no coarser than a
+      // single view's at 2^30); views with no values join for free, a
+      // non-finite view or bound ends up alone
+      float bound = 0.f, bmax = 0.f;
       bool finite = true;
-      for (int vi = g0; vi < g0 + ng; ++vi) {
+      ng = 0;
+      for (int vi = g0; vi < nc; ++vi) {
         const float vm = __uint_as_float(st.vmax[vi]);
-        finite = finite && vm <= 3.0e38f;
-        if (vm > 0.f) bound += st.wbound[vi] * vm;
+        const float b = vm > 0.f ? st.wbound[vi] * vm : 0.f;
+        const bool fin = vm <= 3.0e38f && b <= 3.0e38f;
+        const float nb = bound + b, nm = fmaxf(bmax, b);
+        if (ng > 0 && (!fin || !finite || nb > 2.f * nm)) break;
+        bound = nb;
+        bmax = nm;
+        finite = finite && fin;
+        ++ng;
       }
       const bool fixed = finite && bound <= 3.0e38f;
       float qscale = 1.f, qinv = 1.f;
