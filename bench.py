@@ -128,6 +128,55 @@ def ref_backprojector(sino, steps, log):
                 geometry="Standard scheme, 60 views, 384x384, 256^3, SID 750 / SDD 1375 px")
 
 
+def beijing_loop(steps, log):
+    """The projector pair of the reference's Beijing unroll iteration
+    (iterations/iteration.py:37-40): volume 256x256x64, 144 source angles x 21
+    sub-detectors = 3024 views of 80x128, SID 708 / SDD 1143 (options.yaml).
+    The scan's projection_matrix file does not ship, so the view vectors are the
+    synthetic 21-plane layout with the same shapes.  Times A + B_ref (the pair
+    the reference's iteration runs) and A + matched A^T, phantom input."""
+    import numpy as np
+    import torch
+    from dtvproj import geometry as G, ops, phantom
+    X, Y, Z, U, V = 256, 256, 64, 80, 128
+    mag = 1143.0 / 708.0
+    pitch = (1.2 * np.hypot(X, Y) * mag / (U * 21), 1.2 * Z * mag / V)
+    g = G.beijing(G.beijing_default_vectors(144, 21, (U, V), 708.0, 1143.0, pitch),
+                  (X, Y, Z), (U, V))
+    dev = torch.device("cuda", 0)
+    x = phantom.phantom((X, Y, Z), 1.0, 20, seed=2).to(dev)
+    y = ops.forward(x, g)
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn):
+        fn()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    ta = timed(lambda: ops.forward(x, g))
+    tb = timed(lambda: ops.bp_ref(y, g))
+    tt = timed(lambda: ops.adjoint(y, g))
+    rays = g.n_views * U * V
+    ref_ms = 168.0  # recorded: 400 iterations in about 67 s
+    log(f"beijing loop pair: A {ta:.2f} ms, B_ref {tb:.2f} ms, matched A^T {tt:.2f} ms; "
+        f"A + B_ref {ta + tb:.2f} ms vs the reference's ~{ref_ms:.0f} ms per iteration")
+    return dict(forward_ms=ta, bp_ref_ms=tb, adjoint_ms=tt, pair_ref_ms=ta + tb,
+                pair_matched_ms=ta + tt, rays=rays,
+                grays_per_s_ref_pair=rays / (ta + tb) / 1e6,
+                reference_iteration_ms=ref_ms,
+                speedup_vs_reference_iteration=ref_ms / (ta + tb),
+                geometry="Beijing, 256x256x64, 3024 views (144 x 21) of 80x128, "
+                         "synthetic view vectors",
+                note="the reference's 168 ms covers its whole iteration (A, B_ref and "
+                     "elementwise work) on an unstated NVIDIA GPU")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +186,7 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=1)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-beijing", action="store_true")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="run one device copy of this size first (rocprof counter scale)")
     args = ap.parse_args()
@@ -225,6 +275,7 @@ def main():
                                        samples=acct["samples"]))
 
     bref = ref_backprojector(s, max(1, min(args.steps, 10)), log)
+    bj = None if args.no_beijing else beijing_loop(max(1, min(args.steps, 5)), log)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -252,6 +303,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "reference_backprojector": bref,
+        "beijing_loop": bj,
         "batched_unroll": bat,
     }
     print(json.dumps(line), flush=True)

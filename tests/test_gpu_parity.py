@@ -1,6 +1,6 @@
 """GPU parity: HIP kernels (through the C ABI) vs the FP64 CPU oracle.
 
-Bar (north star): relative L2 error <= 1e-5 for A and A^T, adjoint
+Bar: relative L2 error <= 1e-5 for A and A^T, adjoint
 dot-product mismatch <= 1e-5.  Inputs come from seeded CPU generators, so the
 oracle and the GPU see identical data.
 """
