@@ -15,6 +15,12 @@
 // One HBM pass: reads y (19-point stencil, cached), g, x_prev; writes x, y'.
 // Work items are (instance, voxel) over a descriptor table, so uneven
 // instances share one launch.
+//
+// The TV descent step of ASD-POCS (model/PDHG/ASDPOCS.py:37-43) uses the
+// same stencil with another normaliser:
+//   out = f - step * sum_a D_a^T (D_a f / (|D f| + eps)),
+// the exact gradient step on sum_c [r_c - eps * log(1 + r_c / eps)],
+// r_c = |D f|_c (the reference's eps is 1).
 #include <hip/hip_runtime.h>
 
 #include "dtvproj.h"
@@ -56,39 +62,54 @@ __device__ __forceinline__ bool has_prev(const Stencil& S, int64_t i, int a) {
   return i >= S.s[a];
 }
 
-__device__ __forceinline__ void qvec(const Stencil& S, int64_t i, float eps2, float q[3]) {
+// q = D f / n at voxel i: n = sqrt(|D f|^2 + eps^2) (kAsd false) or
+// |D f| + eps (kAsd true)
+template <bool kAsd>
+__device__ __forceinline__ void qvec(const Stencil& S, int64_t i, float eps, float q[3]) {
   const float dx = diff(S, i, 0), dy = diff(S, i, 1), dz = diff(S, i, 2);
-  const float r = rsqrtf(dx * dx + dy * dy + dz * dz + eps2);
+  const float ss = dx * dx + dy * dy + dz * dz;
+  const float r = kAsd ? 1.f / (sqrtf(ss) + eps) : rsqrtf(ss + eps * eps);
   q[0] = dx * r;
   q[1] = dy * r;
   q[2] = dz * r;
 }
 
-__device__ __forceinline__ void tv_point(const dtv_tv_item& it, int mode, int64_t off,
-                                         int64_t i) {
-  const int64_t nvox = (int64_t)it.nx * it.ny * it.nz;
+__device__ __forceinline__ Stencil make_stencil(const float* f, int nx, int ny, int nz,
+                                                int mode) {
   Stencil S;
-  S.f = it.y + off;
-  S.nx = it.nx;
-  S.ny = it.ny;
-  S.nz = it.nz;
-  S.n = nvox;
+  S.f = f;
+  S.nx = nx;
+  S.ny = ny;
+  S.nz = nz;
+  S.n = (int64_t)nx * ny * nz;
   S.s[0] = 1;
-  S.s[1] = mode == 0 ? it.nx : it.ny;
-  S.s[2] = (int64_t)it.nx * it.ny;
+  S.s[1] = mode == 0 ? nx : ny;
+  S.s[2] = (int64_t)nx * ny;
   S.mode = mode;
-  const float eps2 = it.eps * it.eps;
+  return S;
+}
+
+// sum_a D_a^T q_a at voxel i
+template <bool kAsd>
+__device__ __forceinline__ float tv_grad_at(const Stencil& S, int64_t i, float eps) {
   float q[3];
-  qvec(S, i, eps2, q);
+  qvec<kAsd>(S, i, eps, q);
   float gtv = -(q[0] + q[1] + q[2]);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (has_prev(S, i, a)) {
       float qp[3];
-      qvec(S, i - S.s[a], eps2, qp);
+      qvec<kAsd>(S, i - S.s[a], eps, qp);
       gtv += qp[a];
     }
   }
+  return gtv;
+}
+
+__device__ __forceinline__ void tv_point(const dtv_tv_item& it, int mode, int64_t off,
+                                         int64_t i) {
+  const Stencil S = make_stencil(it.y + off, it.nx, it.ny, it.nz, mode);
+  const float gtv = tv_grad_at<false>(S, i, it.eps);
   // read every input at i before writing: x_prev may share storage with an
   // output (y may not: neighbours read it)
   const float yv = S.f[i];
@@ -116,6 +137,18 @@ __global__ __launch_bounds__(256) void k_tv_fista_multi(const dtv_tv_item* __res
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nvox) return;
   tv_point(it, mode, 0, i);
+}
+
+// out = f - step * TV gradient (ASD normaliser), blockIdx.y = batch item
+__global__ __launch_bounds__(256) void k_tv_descent(const float* __restrict__ f,
+                                                    float* __restrict__ out, int nx, int ny,
+                                                    int nz, float step, float eps, int mode) {
+  const int64_t nvox = (int64_t)nx * ny * nz;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nvox) return;
+  const int64_t off = (int64_t)blockIdx.y * nvox;
+  const Stencil S = make_stencil(f + off, nx, ny, nz, mode);
+  out[off + i] = S.f[i] - step * tv_grad_at<true>(S, i, eps);
 }
 
 }  // namespace
@@ -149,6 +182,22 @@ int dtv_tv_fista_step_multi(const dtv_tv_item* items, int32_t n_items, int64_t m
   if (blocks > 0x7fffffff) return DTV_ERR_TOO_LARGE;
   hipLaunchKernelGGL(k_tv_fista_multi, dim3((unsigned)blocks, n_items), dim3(256), 0,
                      (hipStream_t)stream, items, mode);
+  return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
+}
+
+int dtv_tv_descent(const float* f, float* out, int32_t batch, const int32_t volume_size[3],
+                   float step, float eps, int32_t mode, void* stream) {
+  if (!f || !out || !volume_size) return DTV_ERR_NULL_POINTER;
+  if (batch < 1 || batch > 65535 || (mode != 0 && mode != 1) || !(eps > 0.f) || f == out)
+    return DTV_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < 3; ++i)
+    if (volume_size[i] < 1) return DTV_ERR_INVALID_ARGUMENT;
+  const int64_t nvox = (int64_t)volume_size[0] * volume_size[1] * volume_size[2];
+  const int64_t blocks = (nvox + 255) / 256;
+  if (blocks > 0x7fffffff) return DTV_ERR_TOO_LARGE;
+  hipLaunchKernelGGL(k_tv_descent, dim3((unsigned)blocks, batch), dim3(256), 0,
+                     (hipStream_t)stream, f, out, volume_size[0], volume_size[1],
+                     volume_size[2], step, eps, mode);
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
 }
 

@@ -115,6 +115,78 @@ def tv_eps_grad(f, eps, mode="neumann"):
     return DT(dx / n, "x", mode) + DT(dy / n, "y", mode) + DT(dz / n, "z", mode)
 
 
+def asd_potential(f, eps=1.0, mode="reference"):
+    """sum_c [r_c - eps log(1 + r_c / eps)] with r = |D f| per voxel: the
+    functional whose gradient is the TV direction of ASD-POCS."""
+    dx, dy, dz = grad3(f, mode)
+    r = torch.sqrt(dx * dx + dy * dy + dz * dz)
+    return (r - eps * torch.log1p(r / eps)).sum()
+
+
+def asd_tv_grad(f, eps=1.0, mode="reference"):
+    """sum_a D_a^T (D_a f / (|D f| + eps)): gradTV of model/PDHG/ASDPOCS.py:38-42
+    (there eps = 1 and the matrices are Gradients.py:8-47, ``mode="reference"``)."""
+    dx, dy, dz = grad3(f, mode)
+    n = torch.sqrt(dx * dx + dy * dy + dz * dz) + eps
+    return DT(dx / n, "x", mode) + DT(dy / n, "y", mode) + DT(dz / n, "z", mode)
+
+
+def tv_descent_torch(f, step, eps=1.0, mode="reference"):
+    """Unfused torch formulation of :func:`tv_descent`."""
+    return f - step * asd_tv_grad(f, eps, mode)
+
+
+def tv_descent(f, step, eps=1.0, mode="reference", out=None):
+    """One ASD-POCS TV step f - step * asd_tv_grad(f) on the fused gfx950
+    kernel (``dtv_tv_descent``).  f [B, 1, Z, Y, X] FP32 on the GPU; ``out``
+    (optional) must not share storage with f."""
+    check_input(f, "f")
+    if f.dtype != torch.float32:
+        raise RuntimeError("f must be float32")
+    out = torch.empty_like(f) if out is None else out
+    check_input(out, "out")
+    if out.shape != f.shape or out.dtype != torch.float32:
+        raise RuntimeError("out must be float32 with the shape of f")
+    if out.data_ptr() == f.data_ptr():
+        raise RuntimeError("out must not share storage with f (stencil reads)")
+    Z, Y, X = f.shape[-3:]
+    B = f.numel() // (X * Y * Z)
+    with torch.cuda.device(f.device):
+        rc = _capi.lib().dtv_tv_descent(f.data_ptr(), out.data_ptr(), B, _capi.int3((X, Y, Z)),
+                                        ctypes.c_float(step), ctypes.c_float(eps), MODES[mode],
+                                        _stream(f.device))
+    _capi.check(rc, "dtv_tv_descent")
+    return out
+
+
+def l1ball(v, b, tol=1e-4, max_iter=100):
+    """Shrink v onto the l1 ball of radius b as model/PDHG/Utils/L1ball.py:4-17
+    does: if sum|v| < b, v itself; else the threshold lam solving
+    sum relu(|v| - lam) = b by the reference's damped Newton iteration from
+    lam = 0 (slope -count(|v| > lam) + 0.001, stop once |objective| <= tol or
+    after max_iter steps), then sign(v) relu(|v| - lam).
+
+    The iteration runs on v's device with lam a 0-d FP64 tensor (the
+    reference keeps it as a Python float) and checks for convergence on the
+    host every 8 steps, where the reference synchronises on every step."""
+    a = v.abs()
+    total = a.sum()
+    if float(total) < b:
+        return v
+    lam = torch.zeros((), dtype=torch.float64, device=v.device)
+    obj = total.double() - b
+    active = obj.abs() > tol
+    for it in range(max_iter):
+        if it % 8 == 0 and not bool(active):
+            break
+        r = torch.clamp(a - lam.to(a.dtype), min=0)
+        obj = r.sum().double() - b
+        slope = 0.001 - (r > 0).sum().double()
+        lam = torch.where(active, lam - obj / slope, lam)
+        active = active & (obj.abs() > tol)
+    return torch.sign(v) * torch.clamp(a - lam.to(a.dtype), min=0)
+
+
 def shrink(v, thr):
     """Soft threshold sign(v) max(|v| - thr, 0)."""
     return torch.sign(v) * torch.clamp(v.abs() - thr, min=0)

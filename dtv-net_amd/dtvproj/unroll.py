@@ -15,6 +15,11 @@
   tau and per-axis TV weights; the exact dual prox of lam ||D_a f||_1 (a
   clip to [-lam, lam]) replaces the reference's Newton l1-ball projection.
   Each iteration runs under torch.utils.checkpoint to cap activation memory.
+* :func:`asd_pocs` -- ASD-POCS as model/PDHG/ASDPOCS.py:28-47 runs it
+  (data step, then ``ng`` TV descent steps on the fused gfx950 kernel).
+* :func:`dtv_cp` -- the directional-TV Chambolle-Pock iteration of
+  model/PDHG/DTVCP.py:29-60 with the reference's l1-ball dual constraints.
+* :func:`power_method` -- HyperParams.py:6-13 for an operator pair.
 """
 from __future__ import annotations
 
@@ -135,3 +140,101 @@ class PDHGDTVNet(torch.nn.Module):
             else:
                 f, fbar, w, px, py, pz = self._iteration(k, f, fbar, w, px, py, pz)
         return f
+
+
+@torch.no_grad()
+def power_method(op, op_t, shape, iters: int = 10, seed: int = 0, device="cpu",
+                 dtype=torch.float32) -> float:
+    """||K|| as model/PDHG/Utils/HyperParams.py:6-13 estimates it: from a
+    seeded uniform start, ``iters`` steps x = K^T K x / ||K^T K x||, then
+    ||K x||."""
+    gen = torch.Generator().manual_seed(seed)
+    x = torch.rand(shape, generator=gen, dtype=torch.float64).to(device=device, dtype=dtype)
+    s = 0.0
+    for _ in range(iters):
+        x = op_t(op(x))
+        x = x / x.norm()
+        s = float(op(x).norm())
+    return s
+
+
+@torch.no_grad()
+def asd_pocs(image, sino, project, backproject, cascades: int = 30, beta: float = 5e-5,
+             beta_red: float = 0.9999, ng: int = 10, alpha: float = 1e-3, rmax: float = 0.95,
+             alpha_red: float = 0.95, eps: float = 1.0, mode: str = "reference",
+             fused: bool = True, adaptive: bool = False):
+    """ASD-POCS of model/PDHG/ASDPOCS.py:28-47.
+
+    Per cascade: f = f - beta B(A f - sino); dp = mean|f0 - f|; on the first
+    cascade the TV step is fixed to alpha * dp; ``ng`` steps
+    f -= step * sum_a D_a^T (D_a f / (|D f| + eps)) (:37-43, eps = 1); f0 = f;
+    beta *= beta_red; the result is relu(f).  In the reference f0 is the same
+    tensor as f, whose TV steps run in place, so its dg = mean|f - f0| is 0 and
+    the step is never reduced; ``adaptive=True`` measures dg against f before
+    the TV steps and multiplies the step by ``alpha_red`` when dg > rmax * dp
+    (the rule as published).  ``project``/``backproject`` are the pair the
+    caller runs (the reference: the Beijing module's ForwardProjection and
+    BackProjection); the TV steps use the fused kernel when ``fused`` (FP32
+    GPU volumes) and the torch formulation otherwise."""
+    f = image
+    f0 = image
+    step = None
+    for i in range(cascades):
+        f = f - beta * backproject(project(f) - sino)
+        if i == 0 or adaptive:
+            dp = float((f0 - f).abs().mean())
+        if i == 0:
+            step = alpha * dp
+        f_data = f
+        if fused:
+            bufs = (f, torch.empty_like(f))
+            for j in range(ng):
+                tv.tv_descent(bufs[j % 2], step, eps, mode, out=bufs[(j + 1) % 2])
+            f = bufs[ng % 2] if ng else f
+        else:
+            for _ in range(ng):
+                f = tv.tv_descent_torch(f, step, eps, mode)
+        if adaptive:
+            dg = float((f - f_data).abs().mean())
+            if dg > rmax * dp:
+                step *= alpha_red
+        f0 = f
+        beta *= beta_red
+    return F.relu(f)
+
+
+@torch.no_grad()
+def dtv_cp(image, sino, project, backproject, cascades: int = 30, v=(1.0, 1.0, 1.0),
+           mu: float = 1.0, sigma: float = 1e-7, tau: float = 1e-7, mode: str = "reference"):
+    """Directional-TV Chambolle-Pock of model/PDHG/DTVCP.py:29-60.
+
+    Every 10 cascades the bounds t_a = sum|D_a f| are refreshed (:34-37).
+    Per cascade: res = A f_bar - sino; w = (w + sigma res)/(1 + sigma);
+    r = B w; for each axis p_a' = p_a + v_a sigma D_a f_bar and
+    p_a = p_a' - sigma sign(p_a') l1ball(|p_a'| / sigma, v_a t_a) (:41-46);
+    c = -relu(-c - sigma mu f_bar) (:47); f_new = f - r - tau (sum_a v_a
+    D_a^T p_a + mu c) (:48-52 -- the reference does not scale r by tau);
+    f_bar = 2 f_new - f.  ``v`` = (v1, v2, v3): the reference sets
+    v_a = 1e5 / ||D_a||, mu = 1e5 / ||I||, sigma = tau = 1e-7 (:18-27)."""
+    w = torch.zeros_like(sino)
+    p = [torch.zeros_like(image) for _ in range(3)]
+    c = torch.zeros_like(image)
+    f = image
+    fbar = image
+    t = [0.0, 0.0, 0.0]
+    axes = ("x", "y", "z")
+    for k in range(cascades):
+        if k % 10 == 0:
+            t = [float(tv.D(f, a, mode).abs().sum()) for a in axes]
+        res = project(fbar) - sino
+        w = (w + sigma * res) / (1 + sigma)
+        recon = backproject(w)
+        for i, a in enumerate(axes):
+            pa = p[i] + v[i] * sigma * tv.D(fbar, a, mode)
+            p[i] = pa - sigma * torch.sign(pa) * tv.l1ball(pa.abs() / sigma, v[i] * t[i])
+        c = -F.relu(-c - sigma * mu * fbar)
+        reg = sum(v[i] * tv.DT(p[i], a, mode) for i, a in enumerate(axes)) + mu * c
+        fnew = f - recon - tau * reg
+        fbar = 2 * fnew - f
+        f = fnew
+    return f

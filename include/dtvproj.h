@@ -211,6 +211,15 @@ int dtv_tv_fista_step(const float* y, const float* g, const float* xprev, float*
 int dtv_tv_fista_step_multi(const dtv_tv_item* items, int32_t n_items, int64_t max_voxels,
                             int32_t mode, void* stream);
 
+/* One TV descent step of ASD-POCS (model/PDHG/ASDPOCS.py:37-43, whose
+ * gradTV is built from the sparse matrices of Gradients.py:8-47):
+ *   out = f - step * sum_a D_a^T (D_a f / (|D f| + eps))
+ * with |D f| = sqrt(d_x^2 + d_y^2 + d_z^2) per voxel (the reference has
+ * eps = 1).  f, out: device float32 [batch][Z][Y][X], distinct (the stencil
+ * reads neighbours of f); mode as dtv_tv_fista_step; eps > 0. */
+int dtv_tv_descent(const float* f, float* out, int32_t batch, const int32_t volume_size[3],
+                   float step, float eps, int32_t mode, void* stream);
+
 /* ---- FBP front end ------------------------------------------------------ */
 
 /* Cosine weighting fused with the per-row filtering of the FBP modules:

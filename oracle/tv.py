@@ -85,3 +85,50 @@ def tv_eps_grad(vol, eps, volume_size, mode="neumann"):
     n = np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + eps ** 2)
     g = sum(M.T @ (di / n) for M, di in zip(mats, d))
     return g.reshape(np.asarray(vol).shape)
+
+
+def asd_tv_grad(vol, eps, volume_size, mode="reference"):
+    """gradTV of model/PDHG/ASDPOCS.py:38-42 in FP64:
+    sum_a D_a^T ((D_a f) / (sqrt(sum_a (D_a f)^2) + eps)), eps = 1 there."""
+    mats = [(neumann_matrix if mode == "neumann" else reference_matrix)(volume_size, a)
+            for a in "xyz"]
+    vol = np.asarray(vol, dtype=np.float64)
+    out = []
+    for v in vol.reshape(-1, mats[0].shape[0]):  # one volume at a time
+        d = [M @ v for M in mats]
+        n = np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2) + eps
+        out.append(sum(M.T @ (di / n) for M, di in zip(mats, d)))
+    return np.stack(out).reshape(vol.shape)
+
+
+def asd_potential(vol, eps, volume_size, mode="reference"):
+    """sum_c [r_c - eps log(1 + r_c / eps)], r = |D f|_c (FP64): the
+    functional whose gradient :func:`asd_tv_grad` is."""
+    mats = [(neumann_matrix if mode == "neumann" else reference_matrix)(volume_size, a)
+            for a in "xyz"]
+    total = 0.0
+    for v in np.asarray(vol, dtype=np.float64).reshape(-1, mats[0].shape[0]):
+        r = np.sqrt(sum((M @ v) ** 2 for M in mats))
+        total += float(np.sum(r - eps * np.log1p(r / eps)))
+    return total
+
+
+def l1ball(v, b, tol=1e-4, max_iter=100):
+    """model/PDHG/Utils/L1ball.py:4-17 in FP64 (numpy): identity when
+    sum|v| < b (:5-6); otherwise Newton steps on lam from 0 with slope
+    -count(|v| - lam > 0) + 0.001 while |objective| > tol and fewer than
+    max_iter steps (:7-14); ``np.max(lam, 0)`` of a scalar is lam (:15);
+    sign(v) relu(|v| - lam) (:16)."""
+    v = np.asarray(v, dtype=np.float64)
+    a = np.abs(v)
+    if a.sum() < b:
+        return v.copy()
+    lam = 0.0
+    obj = np.maximum(a - lam, 0).sum() - b
+    it = 0
+    while abs(obj) > tol and it < max_iter:
+        obj = np.maximum(a - lam, 0).sum() - b
+        slope = -np.count_nonzero(a - lam > 0) + 0.001
+        lam -= obj / slope
+        it += 1
+    return np.sign(v) * np.maximum(a - lam, 0)

@@ -200,3 +200,37 @@ def test_real_loader_scales_clips_and_switches_geometry(tmp_path):
         np.testing.assert_array_equal(B.parameters.numpy(), m.astype(np.float32))
     finally:
         B.parameters = old
+
+
+# ---- drop-in next to a checkout of the callers -------------------------------
+
+def test_packages_merge_with_a_callers_checkout(tmp_path):
+    """The callers' tree (no __init__ files) keeps model.PDHG, model.FISTA and
+    data.* beside model.ConeBeamLayers; with this package first on sys.path
+    those resolve from the checkout while the projector modules are these."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    pkg = os.path.join(root, "dtv-net_amd")
+    (tmp_path / "model" / "PDHG").mkdir(parents=True)
+    (tmp_path / "model" / "ConeBeamLayers" / "Beijing").mkdir(parents=True)
+    (tmp_path / "data").mkdir()
+    (tmp_path / "model" / "PDHG" / "Caller.py").write_text(
+        "from model.ConeBeamLayers.Beijing.BeijingGeometry import ForwardProjection, "
+        "BackProjection\n")
+    (tmp_path / "model" / "ConeBeamLayers" / "Beijing" / "BeijingGeometry.py").write_text(
+        "raise ImportError('the checkout copy must not be imported')\n")
+    (tmp_path / "data" / "evalLoader.py").write_text("VALUE = 7\n")
+    code = (
+        "import sys; sys.path[:0] = [%r, %r]\n"
+        "import model.PDHG.Caller as c, data.evalLoader as e\n"
+        "import model.ConeBeamLayers.Beijing.BeijingGeometry as B\n"
+        "assert c.ForwardProjection is B.ForwardProjection\n"
+        "assert B.__file__.startswith(%r), B.__file__\n"
+        "assert e.VALUE == 7\n"
+        "from data.simulateLoader import Stimulated256\n"
+        "print('ok')\n" % (pkg, str(tmp_path), pkg))
+    out = subprocess.run([sys.executable, "-c", code], cwd=str(tmp_path),
+                         stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                         timeout=300)
+    assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stdout

@@ -50,6 +50,100 @@ def test_shrink():
     assert torch.equal(tv.shrink(v, 1.0), torch.tensor([-2.0, -0.0, 0.0, 0.0, 1.0]))
 
 
+@pytest.mark.parametrize("mode", ["reference", "neumann"])
+def test_oracle_asd_gradient_is_the_gradient_of_its_potential(mode):
+    """The ASD-POCS TV direction D^T (D f / (|D f| + eps)) is the exact
+    gradient of sum_c [r_c - eps log(1 + r_c / eps)]: FP64 central
+    differences of that potential pin the oracle's formula."""
+    vs = (5, 4, 3)
+    f = phantom.gaussian((1, 1, 3, 4, 5), 12, dtype=torch.float64).numpy()
+    g = otv.asd_tv_grad(f, 1.0, vs, mode)
+    rng = np.random.default_rng(5)
+    for _ in range(4):
+        v = rng.normal(size=f.shape)
+        h = 1e-5
+        fd = (otv.asd_potential(f + h * v, 1.0, vs, mode)
+              - otv.asd_potential(f - h * v, 1.0, vs, mode)) / (2 * h)
+        assert abs(fd - float((g * v).sum())) <= 1e-7 * max(1.0, abs(fd))
+    # and a step along -g lowers the potential
+    assert otv.asd_potential(f - 1e-3 * g, 1.0, vs, mode) < otv.asd_potential(f, 1.0, vs, mode)
+
+
+@pytest.mark.parametrize("mode", ["reference", "neumann"])
+def test_asd_tv_grad_matches_oracle_and_autograd(mode):
+    vs = (6, 5, 4)
+    f = phantom.gaussian((2, 1, 4, 5, 6), 13, dtype=torch.float64).requires_grad_(True)
+    g = tv.asd_tv_grad(f.detach(), 0.7, mode)
+    np.testing.assert_allclose(g.numpy(), otv.asd_tv_grad(f.detach().numpy(), 0.7, vs, mode),
+                               atol=1e-12)
+    tv.asd_potential(f, 0.7, mode).backward()
+    np.testing.assert_allclose(g.numpy(), f.grad.numpy(), atol=1e-12)
+    step = tv.tv_descent_torch(f.detach(), 0.3, 0.7, mode)
+    np.testing.assert_allclose(step.numpy(), f.detach().numpy() - 0.3 * g.numpy(), atol=1e-14)
+
+
+def _l1_projection_exact(v, b):
+    """Euclidean projection onto {w : sum|w| <= b} by sorting (Duchi et al.,
+    ICML 2008): the threshold is the lam with sum relu(|v| - lam) = b."""
+    a = np.sort(np.abs(v).ravel())[::-1]
+    cs = np.cumsum(a)
+    k = np.arange(1, a.size + 1)
+    rho = np.nonzero(a - (cs - b) / k > 0)[0][-1]
+    lam = (cs[rho] - b) / (rho + 1)
+    return np.sign(v) * np.maximum(np.abs(v) - lam, 0)
+
+
+def test_oracle_l1ball_is_the_l1_projection():
+    rng = np.random.default_rng(3)
+    for n, b in ((50, 3.0), (2000, 100.0), (2000, 1500.0)):
+        v = rng.normal(size=n)
+        w = otv.l1ball(v, b)
+        np.testing.assert_allclose(w, _l1_projection_exact(v, b), atol=1e-6)
+        assert abs(np.abs(w).sum() - b) <= 1e-4
+    # inside the ball: unchanged (L1ball.py:5-6)
+    v = rng.normal(size=30)
+    assert np.array_equal(otv.l1ball(v, np.abs(v).sum() + 1.0), v)
+
+
+def test_l1ball_matches_oracle():
+    v = phantom.gaussian((1, 1, 6, 7, 8), 14, dtype=torch.float64)
+    for b in (5.0, 60.0, 1e4):
+        want = otv.l1ball(v.numpy(), b)
+        np.testing.assert_allclose(tv.l1ball(v, b).numpy(), want, atol=1e-12)
+        got32 = tv.l1ball(v.float(), b).double().numpy()
+        assert np.linalg.norm(got32 - want) <= 1e-5 * max(np.linalg.norm(want), 1e-30)
+
+
+def test_power_method_bounds_and_approaches_the_top_singular_value():
+    vs = (5, 4, 3)
+    M = otv.reference_matrix(vs, "y").toarray()
+    smax = np.linalg.svd(M, compute_uv=False)[0]
+    from dtvproj import unroll
+    Mt = torch.from_numpy(M)
+    op = lambda x: (Mt @ x.reshape(-1)).reshape(x.shape)
+    op_t = lambda x: (Mt.T @ x.reshape(-1)).reshape(x.shape)
+    s = unroll.power_method(op, op_t, (1, 1, 3, 4, 5), iters=200, dtype=torch.float64)
+    assert s <= smax * (1 + 1e-12)
+    assert s >= 0.99 * smax
+
+
+def test_asd_pocs_without_tv_steps_is_landweber_to_the_solution():
+    """ng = 0 leaves the data steps: f <- f - beta A^T (A f - b), which for a
+    consistent, well-conditioned system with a positive solution converges
+    to that solution (the final relu then changes nothing)."""
+    from dtvproj import unroll
+    rng = np.random.default_rng(8)
+    A = torch.from_numpy(rng.normal(size=(40, 24)) + 3 * np.eye(40, 24))
+    x = torch.from_numpy(rng.uniform(0.5, 1.5, size=(1, 1, 2, 3, 4)))
+    b = (A @ x.reshape(-1)).reshape(1, 1, 40)
+    proj = lambda f: (A @ f.reshape(-1)).reshape(1, 1, 40)
+    back = lambda y: (A.T @ y.reshape(-1)).reshape(1, 1, 2, 3, 4)
+    L = float(torch.linalg.matrix_norm(A, 2)) ** 2
+    out = unroll.asd_pocs(torch.zeros_like(x), b, proj, back, cascades=1500, beta=1.0 / L,
+                          beta_red=1.0, ng=0, fused=False)
+    assert float((out - x).abs().max()) <= 1e-8
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["neumann", "reference"])
 def test_fused_fista_tv_step(cuda, mode):
@@ -67,3 +161,29 @@ def test_fused_fista_tv_step(cuda, mode):
     xp_dev = xp.to(cuda)
     x2, _ = tv.fista_tv_step(y.to(cuda), g.to(cuda), xp_dev, xnew=xp_dev, **args)
     assert rel_l2(x2, ref_x) <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["reference", "neumann"])
+@pytest.mark.parametrize("shp", [(2, 1, 13, 11, 17), (1, 1, 1, 1, 5), (1, 1, 9, 1, 1),
+                                 (3, 1, 4, 6, 5)])
+def test_fused_tv_descent_matches_oracle(cuda, mode, shp):
+    print(f"[case] tv_descent {mode} {shp}", flush=True)
+    f = phantom.gaussian(shp, 21)
+    Z, Y, X = shp[-3:]
+    want = f.double().numpy() - 0.25 * otv.asd_tv_grad(f.numpy(), 1.0, (X, Y, Z), mode)
+    got = tv.tv_descent(f.to(cuda), 0.25, 1.0, mode)
+    assert rel_l2(got, torch.from_numpy(want)) <= 1e-6
+    with pytest.raises(RuntimeError, match="share storage"):
+        fd = f.to(cuda)
+        tv.tv_descent(fd, 0.25, 1.0, mode, out=fd)
+
+
+@pytest.mark.gpu
+def test_l1ball_on_the_gpu_matches_oracle(cuda):
+    print("[case] l1ball gpu", flush=True)
+    v = phantom.gaussian((1, 1, 16, 16, 16), 22)
+    for b in (10.0, 400.0, 1e5):
+        want = torch.from_numpy(otv.l1ball(v.double().numpy(), b))
+        got = tv.l1ball(v.to(cuda), b)
+        assert rel_l2(got, want) <= 1e-5
