@@ -16,6 +16,7 @@ import argparse
 import json
 import os
 import platform
+import statistics
 import subprocess
 import sys
 import time
@@ -247,6 +248,8 @@ def main():
     elapsed = t1 - t0
     t_fwd = sum(e0.elapsed_time(e1) for e0, e1, _ in ev) / args.steps * 1e-3
     t_adj = sum(e1.elapsed_time(e2) for _, e1, e2 in ev) / args.steps * 1e-3
+    med_fwd = statistics.median(e0.elapsed_time(e1) for e0, e1, _ in ev) * 1e-3
+    med_adj = statistics.median(e1.elapsed_time(e2) for _, e1, e2 in ev) * 1e-3
     ms_per_step = elapsed / args.steps * 1e3
     value = acct["rays"] * args.steps / elapsed / 1e9
     log(f"forward {t_fwd * 1e3:.3f} ms  adjoint {t_adj * 1e3:.3f} ms  step {ms_per_step:.3f} ms")
@@ -259,16 +262,20 @@ def main():
     achieved = dom_b / dom_t / 1e9
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by a copy kernel)
-    traffic = None
+    traffic = l2_hit = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic_sparse_256.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             pmc = json.load(f)
         key = "k_adj_brick" if dom.startswith("k_adj") else "k_fwd_brick"
         traffic = pmc.get(key, {}).get("traffic_bytes")
+        l2_hit = pmc.get(key, {}).get("l2_hit_rate")
     pair_gbs = acct["pair"] / (t_fwd + t_adj) / 1e9
     roof = dict(bound="hbm", achieved=achieved, peak=roofline.HBM_PEAK_GBS, unit="GB/s",
                 frac=achieved / roofline.HBM_PEAK_GBS, traffic=traffic, kernel=dom,
+                hbm_gbs_measured=traffic / dom_t / 1e9 if traffic else None,
+                l2_hit_rate=l2_hit,
+                forward_ms_median=med_fwd * 1e3, adjoint_ms_median=med_adj * 1e3,
                 pair_achieved=pair_gbs, pair_frac=pair_gbs / roofline.HBM_PEAK_GBS,
                 forward_ms=t_fwd * 1e3, adjoint_ms=t_adj * 1e3,
                 algorithmic_bytes=dict(forward=acct["forward"], adjoint=acct["adjoint"],

@@ -391,7 +391,7 @@ int run_zero(float* p, size_t n, hipStream_t st) {
 // Resolved kernel choice.
 struct Launch {
   int algo = DTV_ALGO_BRICK;
-  int bx = 32, by = 32, bz = 8;
+  int bx = 24, by = 24, bz = 14;
   int block = 256;
 };
 

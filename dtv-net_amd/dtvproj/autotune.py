@@ -21,7 +21,8 @@ from .geometry import ConeGeometry
 
 # (bx, by, bz, lanes); the adjoint needs bx*by*bz <= 32 * lanes
 CANDIDATES = [
-    (32, 16, 16, 256), (32, 32, 8, 256), (16, 16, 32, 256), (16, 16, 16, 256),
+    (32, 16, 16, 256), (24, 24, 14, 256), (32, 32, 8, 256), (16, 16, 32, 256),
+    (16, 16, 16, 256),
     (32, 16, 8, 256), (16, 16, 16, 128), (16, 16, 8, 128), (8, 8, 8, 64),
 ]
 

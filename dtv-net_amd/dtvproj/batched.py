@@ -61,7 +61,7 @@ class MultiProjector:
     """A / matched A^T over uneven instances, one kernel launch per call.
 
     ``bricks``: per-instance (forward, adjoint) brick shapes; default
-    32x32x8 / 32x16x16 (the single-call defaults), or ``"tune"`` to run the
+    24x24x14 / 32x16x16 (the single-call defaults), or ``"tune"`` to run the
     autotuner per instance.  ``persistent_blocks``: workgroups that pull
     (instance, brick) items from a device counter until the list is empty;
     ``"auto"`` (default) = 3 per compute unit, what the brick kernels' LDS
@@ -80,7 +80,7 @@ class MultiProjector:
             for g in self.geoms:
                 t = autotune.tune(g, self.device)
                 bricks.append((t["forward"]["brick"], t["adjoint"]["brick"]))
-        self.bricks = bricks or [((32, 32, 8), (32, 16, 16))] * len(self.geoms)
+        self.bricks = bricks or [((24, 24, 14), (32, 16, 16))] * len(self.geoms)
         self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.views = [g.device_views(self.device) for g in self.geoms]
 

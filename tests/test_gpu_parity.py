@@ -186,7 +186,10 @@ def test_repeatability(cuda):
 BRICKS = [dict(algorithm=2, brick=(16, 16, 16), block=256),
           dict(algorithm=2, brick=(8, 8, 8), block=64),
           dict(algorithm=2, brick=(32, 16, 8), block=128),
-          dict(algorithm=2, brick=(7, 5, 3), block=64)]
+          dict(algorithm=2, brick=(7, 5, 3), block=64),
+          # the large-volume defaults (A, A^T)
+          dict(algorithm=2, brick=(24, 24, 14), block=256),
+          dict(algorithm=2, brick=(32, 16, 16), block=256)]
 
 
 @pytest.mark.parametrize("launch", BRICKS, ids=lambda l: "x".join(map(str, l["brick"])))
