@@ -274,6 +274,8 @@ def main():
     roof = dict(bound="hbm", achieved=achieved, peak=roofline.HBM_PEAK_GBS, unit="GB/s",
                 frac=achieved / roofline.HBM_PEAK_GBS, traffic=traffic, kernel=dom,
                 hbm_gbs_measured=traffic / dom_t / 1e9 if traffic else None,
+                # each array read or written once: volume + sinogram, FP32
+                compulsory_bytes=4 * (256 ** 3 + acct["rays"]),
                 l2_hit_rate=l2_hit,
                 forward_ms_median=med_fwd * 1e3, adjoint_ms_median=med_adj * 1e3,
                 pair_achieved=pair_gbs, pair_frac=pair_gbs / roofline.HBM_PEAK_GBS,
