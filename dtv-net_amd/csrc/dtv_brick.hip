@@ -884,6 +884,22 @@ int validate_multi(const dtv_instance* ih, int32_t n_inst, const dtv_work_item* 
   return DTV_OK;
 }
 
+// Workgroups of a multi launch: one per item, or for the persistent queue
+// persistent_blocks capped at what the device holds at once for this kernel
+// (workgroups beyond that would only start once the queue is empty).
+unsigned multi_grid(const void* kern, int block, size_t lds, bool persistent,
+                    int32_t persistent_blocks, int32_t n_work) {
+  if (!persistent) return (unsigned)n_work;
+  int n = std::min(persistent_blocks, n_work);
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds) == hipSuccess &&
+      cus > 0 && per_cu > 0)
+    n = std::min(n, cus * per_cu);
+  return (unsigned)(n > 0 ? n : 1);
+}
+
 int launch_multi(bool adjoint, const dtv_instance* ih, const dtv_instance* id, int32_t n_inst,
                  const dtv_work_item* wh, const dtv_work_item* wd, int32_t n_work, int32_t block,
                  int32_t* counter, int32_t persistent_blocks, hipStream_t st) {
@@ -906,7 +922,6 @@ int launch_multi(bool adjoint, const dtv_instance* ih, const dtv_instance* id, i
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, st, counter);
     if (hipGetLastError() != hipSuccess) return DTV_ERR_HIP;
   }
-  const dim3 grid(persistent ? (unsigned)std::min(persistent_blocks, n_work) : (unsigned)n_work);
   int* ctr = persistent ? counter : nullptr;
   // one compiled shape when every instance uses the same listed brick shape
   bool same = true;
@@ -919,6 +934,8 @@ int launch_multi(bool adjoint, const dtv_instance* ih, const dtv_instance* id, i
     if (same && bx == a && by == b && bz == c) kern = &k_adj_brick_multi<a, b, c>;
     DTV_BRICK_SHAPES(DTV_PICK)
 #undef DTV_PICK
+    const dim3 grid(multi_grid(reinterpret_cast<const void*>(kern), block, lds, persistent,
+                               persistent_blocks, n_work));
     hipLaunchKernelGGL(kern, grid, dim3(block), lds, st, id, wd, n_work, ctr);
   } else {
     auto kern = &k_fwd_brick_multi<0, 0, 0>;
@@ -926,6 +943,8 @@ int launch_multi(bool adjoint, const dtv_instance* ih, const dtv_instance* id, i
     if (same && bx == a && by == b && bz == c) kern = &k_fwd_brick_multi<a, b, c>;
     DTV_BRICK_SHAPES(DTV_PICK)
 #undef DTV_PICK
+    const dim3 grid(multi_grid(reinterpret_cast<const void*>(kern), block, lds, persistent,
+                               persistent_blocks, n_work));
     hipLaunchKernelGGL(kern, grid, dim3(block), lds, st, id, wd, n_work, ctr);
   }
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;

@@ -64,16 +64,20 @@ class MultiProjector:
     24x24x14 / 32x16x16 (the single-call defaults), or ``"tune"`` to run the
     autotuner per instance.  ``persistent_blocks``: workgroups that pull
     (instance, brick) items from a device counter until the list is empty;
-    ``"auto"`` (default) = 3 per compute unit, what the brick kernels' LDS
-    and registers allow to be resident; 0 = one workgroup per item."""
+    ``"auto"`` (default) = as many as are resident at once for each kernel
+    (the library caps the grid by the kernel's LDS and registers: four per
+    compute unit for A's 24x24x14 bricks, three for A^T's 32x16x16);
+    0 = one workgroup per item."""
 
     def __init__(self, geoms, bricks=None, block=256, persistent_blocks="auto", device=None):
         self.geoms = list(geoms)
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.block = block
         if persistent_blocks == "auto":
+            # the library caps a persistent grid at the workgroups the device
+            # holds at once for the kernel it launches (LDS, registers)
             cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-            persistent_blocks = 3 * cus
+            persistent_blocks = 8 * cus
         self.persistent_blocks = int(persistent_blocks)
         if bricks == "tune":
             bricks = []

@@ -169,8 +169,9 @@ typedef struct dtv_work_item {
  * tables inst_dev / work_dev (validated and sized on the host; the kernel
  * reads the device copies, so the call can be captured in a graph).  With
  * counter != NULL (a device int) and persistent_blocks > 0, that many
- * workgroups pull items from a queue (dynamic balance); otherwise one
- * workgroup per item, in list order.  No reference counterpart: the
+ * workgroups (at most as many as the device holds at once for the kernel)
+ * pull items from a queue (dynamic balance); otherwise one workgroup per
+ * item, in list order.  No reference counterpart: the
  * reference launches per view and per batch item (forwardKernel.cu:115-141). */
 int dtv_cone_forward_multi(const dtv_instance* inst_host, const dtv_instance* inst_dev,
                            int32_t n_inst, const dtv_work_item* work_host,
