@@ -65,9 +65,8 @@ class MultiProjector:
     autotuner per instance.  ``persistent_blocks``: workgroups that pull
     (instance, brick) items from a device counter until the list is empty;
     ``"auto"`` (default) = as many as are resident at once for each kernel
-    (the library caps the grid by the kernel's LDS and registers: four per
-    compute unit for A's 24x24x14 bricks, three for A^T's 32x16x16);
-    0 = one workgroup per item."""
+    (the library caps the grid by the kernel's LDS and registers: three per
+    compute unit for the default bricks); 0 = one workgroup per item."""
 
     def __init__(self, geoms, bricks=None, block=256, persistent_blocks="auto", device=None):
         self.geoms = list(geoms)
