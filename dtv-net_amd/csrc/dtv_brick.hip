@@ -490,6 +490,52 @@ __device__ __forceinline__ int round_to_int(float x) {
   return r;
 }
 
+// Adds the trilinear taps of samples [ka, kb) of one ray (value val, tile-
+// relative 32.32 position X, step D) into the LDS tile: int32 fixed point
+// (kFixed, taps pre-scaled) or FP32.
+template <bool kFixed>
+__device__ __forceinline__ void adj_march(int ka, int kb, int64_t X[3], const int64_t D[3],
+                                          float val, int stz, int sty, int* iacc) {
+  float* facc = reinterpret_cast<float*>(iacc);
+  for (int k = ka; k < kb; ++k) {
+    const int ix = (int)(X[0] >> 32), iy = (int)(X[1] >> 32), iz = (int)(X[2] >> 32);
+    const float fx = (float)(uint32_t)X[0] * kInvFix;
+    const float fy = (float)(uint32_t)X[1] * kInvFix;
+    const float fz = (float)(uint32_t)X[2] * kInvFix;
+    const float wz1 = fz * val, wz0 = val - wz1;
+    const float w10 = fy * wz0, w00 = wz0 - w10;
+    const float w11 = fy * wz1, w01 = wz1 - w11;
+    const float t001 = fx * w00, t000 = w00 - t001;
+    const float t011 = fx * w10, t010 = w10 - t011;
+    const float t101 = fx * w01, t100 = w01 - t101;
+    const float t111 = fx * w11, t110 = w11 - t111;
+    const int base =
+        (int)(__umul24((unsigned)iz, (unsigned)stz) + __umul24((unsigned)iy, (unsigned)sty)) + ix;
+    if (kFixed) {
+      atomicAdd(iacc + base, round_to_int(t000));
+      atomicAdd(iacc + base + 1, round_to_int(t001));
+      atomicAdd(iacc + base + sty, round_to_int(t010));
+      atomicAdd(iacc + base + sty + 1, round_to_int(t011));
+      atomicAdd(iacc + base + stz, round_to_int(t100));
+      atomicAdd(iacc + base + stz + 1, round_to_int(t101));
+      atomicAdd(iacc + base + stz + sty, round_to_int(t110));
+      atomicAdd(iacc + base + stz + sty + 1, round_to_int(t111));
+    } else {
+      atomicAdd(facc + base, t000);
+      atomicAdd(facc + base + 1, t001);
+      atomicAdd(facc + base + sty, t010);
+      atomicAdd(facc + base + sty + 1, t011);
+      atomicAdd(facc + base + stz, t100);
+      atomicAdd(facc + base + stz + 1, t101);
+      atomicAdd(facc + base + stz + sty, t110);
+      atomicAdd(facc + base + stz + sty + 1, t111);
+    }
+    X[0] += D[0];
+    X[1] += D[1];
+    X[2] += D[2];
+  }
+}
+
 // Matched A^T over one brick of one volume.  sbase: this volume's sinogram,
 // vb: its volume (the brick's voxels are written once), iacc: fixed-point
 // int32 accumulators in LDS over the brick plus a one-voxel halo, so every
@@ -635,43 +681,12 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
             int ka, kb;
             int64_t X[3], D[3];
             if (!ray_run(g, v, iu, iv, flo, fhi, org, ka, kb, X, D)) continue;
-            for (int k = ka; k < kb; ++k) {
-              const int ix = (int)(X[0] >> 32), iy = (int)(X[1] >> 32), iz = (int)(X[2] >> 32);
-              const float fx = (float)(uint32_t)X[0] * kInvFix;
-              const float fy = (float)(uint32_t)X[1] * kInvFix;
-              const float fz = (float)(uint32_t)X[2] * kInvFix;
-              const float wz1 = fz * val, wz0 = val - wz1;
-              const float w10 = fy * wz0, w00 = wz0 - w10;
-              const float w11 = fy * wz1, w01 = wz1 - w11;
-              const float t001 = fx * w00, t000 = w00 - t001;
-              const float t011 = fx * w10, t010 = w10 - t011;
-              const float t101 = fx * w01, t100 = w01 - t101;
-              const float t111 = fx * w11, t110 = w11 - t111;
-              const int base =
-                (int)(__umul24((unsigned)iz, (unsigned)stz) + __umul24((unsigned)iy, (unsigned)sty)) + ix;
-              if (fixed) {
-                atomicAdd(iacc + base, round_to_int(t000));
-                atomicAdd(iacc + base + 1, round_to_int(t001));
-                atomicAdd(iacc + base + sty, round_to_int(t010));
-                atomicAdd(iacc + base + sty + 1, round_to_int(t011));
-                atomicAdd(iacc + base + stz, round_to_int(t100));
-                atomicAdd(iacc + base + stz + 1, round_to_int(t101));
-                atomicAdd(iacc + base + stz + sty, round_to_int(t110));
-                atomicAdd(iacc + base + stz + sty + 1, round_to_int(t111));
-              } else {
-                atomicAdd(facc + base, t000);
-                atomicAdd(facc + base + 1, t001);
-                atomicAdd(facc + base + sty, t010);
-                atomicAdd(facc + base + sty + 1, t011);
-                atomicAdd(facc + base + stz, t100);
-                atomicAdd(facc + base + stz + 1, t101);
-                atomicAdd(facc + base + stz + sty, t110);
-                atomicAdd(facc + base + stz + sty + 1, t111);
-              }
-              X[0] += D[0];
-              X[1] += D[1];
-              X[2] += D[2];
-            }
+            // the group's accumulation mode is uniform: one loop per mode, so
+            // the sample loop carries no branch on it
+            if (fixed)
+              adj_march<true>(ka, kb, X, D, val, stz, sty, iacc);
+            else
+              adj_march<false>(ka, kb, X, D, val, stz, sty, iacc);
           }
         }
       }
