@@ -4,8 +4,9 @@ tests/golden/make_golden.py from the FP64 oracle).
 CPU: the oracle still reproduces every stored output from the stored inputs
 (regression freeze), and the stored pairs satisfy the identities they must:
 <A x, y> = <x, A^T y> for every scheme, and the reference backprojectors are
-*not* the adjoint (the survey's central quirk).  GPU (``-m gpu``): the HIP
-path meets the stored outputs to the parity bar, for both kernel families."""
+*not* the adjoint (the reference wires its FDK backprojector as A's
+gradient).  GPU (``-m gpu``): the HIP path meets the stored outputs to the
+parity bar, for both kernel families."""
 import os
 
 import numpy as np
