@@ -536,6 +536,61 @@ __device__ __forceinline__ void adj_march(int ka, int kb, int64_t X[3], const in
   }
 }
 
+// Shapes whose owned voxels i = tid + 256 j (j < kAdjPerThread) sit, for
+// every lane, at compile-time offsets from one lane base: 256 lanes cover
+// whole rows (256 % bx == 0), a step of j whole row groups of a plane
+// (by % (256 / bx) == 0), and the 32 slots exactly the brick.
+__host__ __device__ constexpr bool fast_flush_shape(int bx, int by, int bz) {
+  return bx > 0 && by > 0 && bz > 0 && 256 % bx == 0 && by % (256 / bx) == 0 &&
+         bx * by * bz == kAdjPerThread * 256;
+}
+
+// Adds the group's owned fixed-point (or FP32) sums into the lane's FP32
+// registers and clears them (fixed order per voxel: bitwise reproducible).
+// The lane index is made opaque so the addresses are computed here rather
+// than hoisted out of the view loops and held (or spilled) throughout.
+template <int CBX, int CBY, int CBZ>
+__device__ __forceinline__ void flush_owned(int* iacc, float (&vacc)[kAdjPerThread], bool fixed,
+                                            float qinv, int bx, int by, int nown, int stz,
+                                            int sty) {
+  float* facc = reinterpret_cast<float*>(iacc);
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  constexpr bool kFast = fast_flush_shape(CBX, CBY, CBZ);
+  if (kFast && blockDim.x == 256) {
+    // voxel j of lane tid: row q + R (j % P) of plane j / P, q = tid / bx
+    constexpr int R = kFast ? 256 / CBX : 1, P = kFast ? CBY / R : 1;
+    constexpr int SY = CBX + 3, SZ = (CBX + 3) * (CBY + 2) + 1;
+    const int base = SZ + (tid / (kFast ? CBX : 1) + 1) * SY + tid % (kFast ? CBX : 1) + 1;
+    if (fixed) {
+#pragma unroll
+      for (int j = 0; j < kAdjPerThread; ++j) {
+        const int li = base + (j % P) * R * SY + (j / P) * SZ;
+        vacc[j] += (float)iacc[li] * qinv;
+        iacc[li] = 0;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kAdjPerThread; ++j) {
+        const int li = base + (j % P) * R * SY + (j / P) * SZ;
+        vacc[j] += facc[li];
+        iacc[li] = 0;
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < kAdjPerThread; ++j) {
+    const int i = tid + j * blockDim.x;
+    if (i < nown) {
+      const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
+      const int li = (lz + 1) * stz + (ly + 1) * sty + lx + 1;
+      vacc[j] += fixed ? (float)iacc[li] * qinv : facc[li];
+      iacc[li] = 0;
+    }
+  }
+}
+
 // Matched A^T over one brick of one volume.  sbase: this volume's sinogram,
 // vb: its volume (the brick's voxels are written once), iacc: fixed-point
 // int32 accumulators in LDS over the brick plus a one-voxel halo, so every
@@ -589,24 +644,40 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
     __syncthreads();  // the previous chunk's readers are done with st
     stage_chunk(g, c0, nc, lo, hi, true, st);
     __syncthreads();
-    // largest |value| of each view's rectangle (non-finite -> +inf)
-    for (int vi = 0; vi < nc; ++vi) {
-      const int* rc = st.rect[vi];
-      const int nur = rc[1] - rc[0] + 1, nvr = rc[3] - rc[2] + 1;
-      if (nur <= 0 || nvr <= 0) continue;
-      const float* srow = sbase + (size_t)(c0 + vi) * g.nv * g.nu;
-      const float rn = 1.f / (float)nur;
-      float m = 0.f;
-      for (int r = threadIdx.x; r < nur * nvr; r += blockDim.x) {
-        int dv, du;
-        divmod(r, nur, rn, dv, du);
-        const float s = srow[(size_t)(rc[2] + dv) * g.nu + rc[0] + du];
-        m = fmaxf(m, fabsf(s * gscale));
-        if (!(fabsf(s) <= 3.0e38f)) m = __int_as_float(0x7f800000);
-      }
+    // largest |value| of each view's rectangle (non-finite -> +inf): one wave
+    // per view, four loads in flight per lane
+    {
+      const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+      for (int vi = threadIdx.x >> 6; vi < nc; vi += nw) {
+        const int* rc = st.rect[vi];
+        const int nur = rc[1] - rc[0] + 1, nvr = rc[3] - rc[2] + 1;
+        if (nur <= 0 || nvr <= 0) continue;
+        const float* srow = sbase + (size_t)(c0 + vi) * g.nv * g.nu;
+        const float rn = 1.f / (float)nur;
+        const int n = nur * nvr;
+        float m = 0.f;
+        for (int r0 = lane; r0 < n; r0 += 256) {
+          float s[4];
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-      if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(&st.vmax[vi], __float_as_uint(m));
+          for (int q = 0; q < 4; ++q) {
+            const int r = r0 + 64 * q;
+            s[q] = 0.f;
+            if (r < n) {
+              int dv, du;
+              divmod(r, nur, rn, dv, du);
+              s[q] = srow[(size_t)(rc[2] + dv) * g.nu + rc[0] + du];
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            m = fmaxf(m, fabsf(s[q] * gscale));
+            if (!(fabsf(s[q]) <= 3.0e38f)) m = __int_as_float(0x7f800000);
+          }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+        if (lane == 0) st.vmax[vi] = __float_as_uint(m);
+      }
     }
     __syncthreads();
     for (int g0 = 0, ng = 0; g0 < nc; g0 += ng) {
@@ -697,18 +768,7 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
       // halo keeps whatever it gathered and is never read.  The lane index is
       // made opaque so the 32 LDS addresses are recomputed here rather than
       // hoisted out of the view loops and held (or spilled) throughout.
-      int tid = threadIdx.x;
-      asm volatile("" : "+v"(tid));
-#pragma unroll
-      for (int j = 0; j < kAdjPerThread; ++j) {
-        const int i = tid + j * blockDim.x;
-        if (i < nown) {
-          const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
-          const int li = (lz + 1) * stz + (ly + 1) * sty + lx + 1;
-          vacc[j] += fixed ? (float)iacc[li] * qinv : facc[li];
-          iacc[li] = 0;
-        }
-      }
+      flush_owned<CBX, CBY, CBZ>(iacc, vacc, fixed, qinv, bx, by, nown, stz, sty);
       __syncthreads();  // cleared before the next group's atomics
     }
   }
