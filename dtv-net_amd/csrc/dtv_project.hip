@@ -404,13 +404,14 @@ long long brick_count(const Geo& g, int bx, int by, int bz, bool adjoint) {
 }
 
 // Default kernels, by problem size (measured on MI355X, 256 CUs):
-//  * enough bricks to fill the GPU (>= 1024 workgroups with the batch): A on
-//    32x32x8 bricks, A^T on 32x16x16 (fastest at 256^3 / 60 views and at
-//    4 x 128^3 / 90 views);
-//  * fewer: 16x16x8 bricks (2-3x faster than the large shapes at 64^3 and
-//    single 128^3 volumes, which they leave mostly idle), and for A the
-//    one-lane-per-ray kernel when even those give under 256 workgroups
-//    (64^3 / 72 views: 0.58 vs 0.79 ms; 32^3: 0.03 vs 0.22 ms).
+//  * at least one workgroup per compute unit (>= 256 with the batch): A on
+//    24x24x14 bricks, A^T on 32x16x16 (fastest at 256^3 / 60 views, 4 x 128^3
+//    / 90 views and the 256x256x64 / 3024-view Beijing scan; 160^3 / 60
+//    views: A 1.03 vs 1.47 ms, A^T 1.24 vs 1.44 ms on 16x16x8);
+//  * fewer: 16x16x8 bricks (96^3 / 72 views: A 0.64 vs 0.98 ms, A^T 0.85 vs
+//    1.39 ms; still so at 720 views), and for A the one-lane-per-ray kernel
+//    when even those give under 256 workgroups (64^3 / 72 views: 0.58 vs
+//    0.79 ms; 32^3: 0.03 vs 0.22 ms).
 Launch default_launch(bool adjoint, const Geo& g, int batch) {
   Launch L;
   if (adjoint) {
@@ -418,7 +419,7 @@ Launch default_launch(bool adjoint, const Geo& g, int batch) {
     L.by = 16;
     L.bz = 16;
   }
-  if (brick_count(g, L.bx, L.by, L.bz, adjoint) * batch >= 1024) return L;
+  if (brick_count(g, L.bx, L.by, L.bz, adjoint) * batch >= 256) return L;
   L.bx = 16;
   L.by = 16;
   L.bz = 8;
