@@ -262,7 +262,7 @@ def main():
     achieved = dom_b / dom_t / 1e9
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by a copy kernel)
-    traffic = l2_hit = None
+    traffic = l2_hit = valu = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic_sparse_256.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
@@ -270,6 +270,7 @@ def main():
         key = "k_adj_brick" if dom.startswith("k_adj") else "k_fwd_brick"
         traffic = pmc.get(key, {}).get("traffic_bytes")
         l2_hit = pmc.get(key, {}).get("l2_hit_rate")
+        valu = pmc.get(key, {}).get("sq_insts_valu")
     pair_gbs = acct["pair"] / (t_fwd + t_adj) / 1e9
     roof = dict(bound="hbm", achieved=achieved, peak=roofline.HBM_PEAK_GBS, unit="GB/s",
                 frac=achieved / roofline.HBM_PEAK_GBS, traffic=traffic, kernel=dom,
@@ -277,6 +278,9 @@ def main():
                 # each array read or written once: volume + sinogram, FP32
                 compulsory_bytes=4 * (256 ** 3 + acct["rays"]),
                 l2_hit_rate=l2_hit,
+                # what does bound it: vector-instruction issue (PMC SQ_INSTS_VALU
+                # per launch / this launch's time / the chip's issue rate)
+                valu_issue_frac=valu / dom_t / roofline.VALU_PEAK_WAVE_INSTS if valu else None,
                 forward_ms_median=med_fwd * 1e3, adjoint_ms_median=med_adj * 1e3,
                 pair_achieved=pair_gbs, pair_frac=pair_gbs / roofline.HBM_PEAK_GBS,
                 forward_ms=t_fwd * 1e3, adjoint_ms=t_adj * 1e3,

@@ -17,6 +17,9 @@ import numpy as np
 from .geometry import AXIS_XY, ConeGeometry
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, GB/s
+# MI355X vector issue: one wave64 VALU instruction per compute unit per cycle
+# (four SIMD16 units, four cycles each), 256 CUs at 2.4 GHz
+VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
 
 
 def _open_count(lo, hi):
