@@ -105,6 +105,23 @@ def test_capi_rejects_bad_arguments_without_gpu():
                                     _capi.int3((4, 4, 4)), _capi.int2((4, 4)),
                                     ctypes.c_void_p(16), 20, -1, None) == 5
     assert "multiple of 21" in lib.dtv_strerror(5).decode()
+    # TV steps and the row filter: null pointers, aliasing, bad sizes and modes
+    p = ctypes.c_void_p(16)
+    q = ctypes.c_void_p(32)
+    v3 = _capi.int3((4, 4, 4))
+    f32 = ctypes.c_float
+    assert lib.dtv_tv_descent(None, q, 1, v3, f32(0.1), f32(1.0), 1, None) == 2
+    assert lib.dtv_tv_descent(p, p, 1, v3, f32(0.1), f32(1.0), 1, None) == 1   # in place
+    assert lib.dtv_tv_descent(p, q, 1, v3, f32(0.1), f32(0.0), 1, None) == 1   # eps <= 0
+    assert lib.dtv_tv_descent(p, q, 1, v3, f32(0.1), f32(1.0), 2, None) == 1   # mode
+    assert lib.dtv_tv_descent(p, q, 0, v3, f32(0.1), f32(1.0), 0, None) == 1   # batch
+    assert lib.dtv_tv_descent(p, q, 1, _capi.int3((4, 0, 4)), f32(0.1), f32(1.0), 0,
+                              None) == 1
+    assert lib.dtv_row_filter(p, q, 4, 8, None, 0, p, 4, None) == 1            # even taps
+    assert lib.dtv_row_filter(p, p, 4, 8, None, 0, q, 5, None) == 1            # in place
+    assert lib.dtv_row_filter(None, q, 4, 8, None, 0, p, 5, None) == 2
+    assert lib.dtv_row_filter(p, q, 4, 20000, None, 0, ctypes.c_void_p(48), 5001,
+                              None) == 3                                       # LDS budget
 
 
 def test_ops_require_device_tensors():
