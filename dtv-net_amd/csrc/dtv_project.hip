@@ -269,14 +269,20 @@ __global__ __launch_bounds__(kBpTileX) void k_bp_beijing(
       for (int j = 0; j < na; ++j) {
         const BjView& w = vw[j];
         const double csx = cx - w.S[0], csy = cy - w.S[1];
-        double csz = (z0 - g.nz / 2.0) - w.S[2];
+        const double csz = (z0 - g.nz / 2.0) - w.S[2];
         const float* img = sb + (size_t)(a0 + j) * g.nu * g.nv;
+        // the three triple products are affine in c: step them along z
+        // (one add each per voxel; 16 steps add ~16 ulp to exact FP64 values)
+        double den = -(csx * w.n[0] + csy * w.n[1] + csz * w.n[2]);
+        double nu1 = csx * w.m1[0] + csy * w.m1[1] + csz * w.m1[2];
+        double nu2 = csx * w.m2[0] + csy * w.m2[1] + csz * w.m2[2];
 #pragma unroll
         for (int k = 0; k < kBpZ; ++k) {
-          const double den = -(csx * w.n[0] + csy * w.n[1] + csz * w.n[2]);
-          const double f = 1.0 / den;
-          const double sx = f * (csx * w.m1[0] + csy * w.m1[1] + csz * w.m1[2]) + g.nu / 2.0;
-          const double sy = f * (csx * w.m2[0] + csy * w.m2[1] + csz * w.m2[2]) + g.nv / 2.0;
+          // reciprocal from the hardware estimate and two Newton steps
+          // (about one ulp); a zero denominator keeps IEEE 1/0
+          const double f = den == 0.0 ? copysign(__builtin_inf(), den) : recip(den);
+          const double sx = f * nu1 + g.nu / 2.0;
+          const double sy = f * nu2 + g.nv / 2.0;
           if (!(sx < -1.0 || sx > xlim || sy < -1.0 || sy > ylim)) {
             const double x = sx - 0.5, y = sy - 0.5;
             const double fxd = floor(x), fyd = floor(y);
@@ -284,7 +290,9 @@ __global__ __launch_bounds__(kBpTileX) void k_bp_beijing(
                       bil_checked(img, g.nu, g.nv, (int)fxd, (float)(x - fxd), (int)fyd,
                                   (float)(y - fyd));
           }
-          csz += 1.0;
+          den -= w.n[2];
+          nu1 += w.m1[2];
+          nu2 += w.m2[2];
         }
       }
     }
