@@ -636,7 +636,6 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
   // tile origin = the halo corner (x0-1, y0-1, z0-1)
   const double org[3] = {(x0 - 1) * kFix, (y0 - 1) * kFix, (z0 - 1) * kFix};
   const float gscale = (float)g.scale;
-  float* facc = reinterpret_cast<float*>(iacc);
 
   int vslot = 0;  // st.sv slot of the next processed view
   for (int c0 = 0; c0 < g.n_views; c0 += kChunk) {
