@@ -14,12 +14,13 @@
 //                sinogram with one hardware FP32 atomic per (ray, brick).
 //                The volume is read from HBM once per call.
 //   k_adj_brick  matched A^T: per view, each sample whose stencil touches
-//                the brick adds its owned taps into fixed-point int32 LDS
+//                the brick adds its taps into fixed-point int32 LDS
 //                accumulators (integer LDS atomics are several times faster
-//                than FP32 ones on gfx950 and order-independent); after the
-//                view the integers are scaled back into per-lane FP32
-//                registers.  The brick is written to HBM once at the end:
-//                no global atomics, no zero-fill, bitwise reproducible.
+//                than FP32 ones on gfx950 and order-independent); after each
+//                group of views sharing one scale the owned integers are
+//                scaled back into per-lane FP32 registers.  The brick is
+//                written to HBM once at the end: no global atomics, no
+//                zero-fill, bitwise reproducible.
 #include <hip/hip_runtime.h>
 
 #include "dtv_geom.h"
