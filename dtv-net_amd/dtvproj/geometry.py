@@ -270,3 +270,47 @@ def beijing_default_vectors(n_angles=144, planes=21, sub_detector=(80, 128),
             rows.append([src[0], src[1], -src[2], det[0], det[1], -det[2],
                          v[0], v[1], v[2], u[0], u[1], u[2]])
     return np.asarray(rows, dtype=np.float32)
+
+
+def angle_groups(n_blocks: int, block: int, take: int, step: int = 1, start: int = 0):
+    """Source-angle indices of the reference's sparse-view selections: from
+    each of ``n_blocks`` consecutive blocks of ``block`` angles, ``take``
+    angles ``step`` apart starting ``start`` into the block.  The reference
+    scripts use 24 blocks of 45 of the scan's 1080 angles: the first 3 of
+    each (iterations/eval_generateSparseView.py:46-48, 72 angles) or 6 taken
+    8 apart (iterations/fista_eval.py:14-16, ``[j*45:(j+1)*45:45//6+1]``,
+    144 angles)."""
+    if take < 1 or step < 1 or start < 0 or start + (take - 1) * step >= block:
+        raise ValueError("the selection must fit inside one block")
+    b = np.arange(n_blocks, dtype=np.int64)[:, None] * block
+    k = start + np.arange(take, dtype=np.int64)[None, :] * step
+    return (b + k).reshape(-1)
+
+
+def subsample_beijing(projection, vectors, angles, planes: int = 21):
+    """Keep the given source angles of a Beijing scan.
+
+    ``projection``: [..., n_angles * planes, V, U] in the projector's view
+    order (view = angle * planes + plane, as the raw files store
+    [angles, planes, V, U]); ``vectors``: [n_angles * planes, 12].  Returns
+    both restricted to ``angles`` in the same layout, which is what the
+    reference scripts build by reshaping to [angles, planes, ...] and
+    slicing (iterations/fista_eval.py:12-20)."""
+    angles = np.asarray(angles, dtype=np.int64)
+    views = (angles[:, None] * planes + np.arange(planes, dtype=np.int64)[None, :]).reshape(-1)
+    n_views = vectors.shape[0]
+    if n_views % planes or (views.size and (views.min() < 0 or views.max() >= n_views)):
+        raise ValueError("angle index out of range or view count not a multiple of planes")
+    if projection.shape[-3] != n_views:
+        raise ValueError(f"projection has {projection.shape[-3]} views, vectors {n_views}")
+    import torch
+    if isinstance(projection, torch.Tensor):
+        idx = torch.as_tensor(views, device=projection.device)
+        proj = projection.index_select(projection.dim() - 3, idx).contiguous()
+    else:
+        proj = np.ascontiguousarray(np.take(np.asarray(projection), views, axis=-3))
+    if isinstance(vectors, torch.Tensor):
+        vec = vectors.index_select(0, torch.as_tensor(views, device=vectors.device)).contiguous()
+    else:
+        vec = np.ascontiguousarray(np.asarray(vectors)[views])
+    return proj, vec

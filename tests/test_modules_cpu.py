@@ -234,3 +234,41 @@ def test_packages_merge_with_a_callers_checkout(tmp_path):
                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                          timeout=300)
     assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stdout
+
+
+# ---- sparse-view selection of Beijing scans ----------------------------------
+
+def test_angle_groups_match_the_reference_slicings():
+    from dtvproj import geometry as G
+    a = np.arange(1080)
+    # fista_eval.py: projection[j*45:(j+1)*45:45//6+1] for 24 blocks
+    want = np.concatenate([a[j * 45:(j + 1) * 45:45 // 6 + 1] for j in range(24)])
+    assert np.array_equal(G.angle_groups(24, 45, 6, 8), want)
+    # eval_generateSparseView.py: projection[j*45:j*45+3]
+    want = np.concatenate([a[j * 45:j * 45 + 3] for j in range(24)])
+    assert np.array_equal(G.angle_groups(24, 45, 3), want)
+    with pytest.raises(ValueError):
+        G.angle_groups(24, 45, 7, 8)
+
+
+def test_subsample_beijing_layout_and_geometry():
+    """The selected views equal the reference's reshape-to-[angles, planes]
+    and slice, for arrays and tensors, and the geometry built from the
+    selected vectors is the full geometry's view subset."""
+    from dtvproj import geometry as G
+    n_ang, planes, U, V = 12, 21, 6, 5
+    rng = np.random.default_rng(4)
+    proj = rng.normal(size=(1, 1, n_ang * planes, V, U)).astype(np.float32)
+    vec = G.beijing_default_vectors(n_ang, planes, (U, V), 40.0, 70.0, (0.6, 0.7))
+    ang = G.angle_groups(3, 4, 2, 2)                      # angles 0, 2, 4, 6, 8, 10
+    sp, sv = G.subsample_beijing(proj, vec, ang, planes)
+    want_p = proj.reshape(n_ang, planes, V, U)[ang].reshape(1, 1, -1, V, U)
+    want_v = vec.reshape(n_ang, planes, 12)[ang].reshape(-1, 12)
+    assert np.array_equal(sp, want_p) and np.array_equal(sv, want_v)
+    tp, tv_ = G.subsample_beijing(torch.from_numpy(proj), torch.from_numpy(vec), ang, planes)
+    assert torch.equal(tp, torch.from_numpy(want_p)) and torch.equal(tv_, torch.from_numpy(want_v))
+    full = G.beijing(vec, (8, 8, 6), (U, V))
+    views = (ang[:, None] * planes + np.arange(planes)[None, :]).reshape(-1)
+    assert np.array_equal(G.beijing(sv, (8, 8, 6), (U, V)).views, full.subset(views).views)
+    with pytest.raises(ValueError):
+        G.subsample_beijing(proj, vec, [n_ang], planes)
