@@ -91,7 +91,9 @@ def batched_unroll(log):
         return (time.perf_counter() - t) / n
 
     t_plain = timed(lambda: batched.plain_fista_tv(geoms, b, iters, lam, eps, steps), 1)
-    bf = batched.BatchedFistaTV(geoms, b, iters, lam, eps, steps)
+    # one A and one A^T brick shape for the whole launch, picked by the
+    # multi-launch autotuner (before the timed region)
+    bf = batched.BatchedFistaTV(geoms, b, iters, lam, eps, steps, bricks="tune_shared")
     bf.capture()
     t_batched = timed(bf.replay, 3)
     plain = batched.plain_fista_tv(geoms, b, iters, lam, eps, steps)
@@ -103,7 +105,8 @@ def batched_unroll(log):
     return dict(metric="batched vs plain 20-iteration FISTA-TV unroll, 16 uneven instances",
                 plain_steps_per_s=1.0 / t_plain, batched_steps_per_s=1.0 / t_batched,
                 ratio=t_plain / t_batched, max_rel_diff=err, instances=len(geoms),
-                iterations=iters)
+                iterations=iters, tuned_bricks=dict(forward=list(bf.mp.tuned["forward"]),
+                                                    adjoint=list(bf.mp.tuned["adjoint"])))
 
 
 def ref_backprojector(sino, steps, log):

@@ -103,3 +103,37 @@ def tune(geom: ConeGeometry, device=None, reps: int = 3, candidates=CANDIDATES,
             with open(path, "w") as f:
                 json.dump(disk, f, indent=1)
     return res
+
+
+def tune_multi(geoms, device=None, reps: int = 3, candidates=CANDIDATES,
+               verbose: bool = False) -> dict:
+    """One brick shape for A and one for A^T shared by every instance of a
+    multi-instance launch (a single compiled shape keeps the launch on its
+    constant-shape kernel), each the fastest over the candidates that fit
+    every instance, timed on the multi launch itself with uniform random
+    inputs (dense, like unroll residuals)."""
+    from . import batched
+    geoms = list(geoms)
+    device = device or torch.device("cuda", torch.cuda.current_device())
+    gen = torch.Generator().manual_seed(0)
+    xs = [torch.rand(g.volume_shape(1), generator=gen).to(device) for g in geoms]
+    ys = [torch.rand(g.sino_shape(1), generator=gen).to(device) for g in geoms]
+    outs_s = [torch.empty_like(y) for y in ys]
+    outs_v = [torch.empty_like(x) for x in xs]
+    times = {"forward": {}, "adjoint": {}}
+    for c in candidates:
+        if c[3] != 256 or c[0] * c[1] * c[2] > 32 * c[3]:
+            continue
+        shape = tuple(c[:3])
+        mp = batched.MultiProjector(geoms, [(shape, shape)] * len(geoms), device=device)
+        bf = mp.bind_forward(xs, outs_s)
+        ba = mp.bind_adjoint(ys, outs_v)
+        times["forward"][str(shape)] = _time(lambda: mp.run(bf, False), reps)
+        times["adjoint"][str(shape)] = _time(lambda: mp.run(ba, True), reps)
+        if verbose:
+            print(f"tune_multi {shape}: fwd {times['forward'][str(shape)]:.3f} ms  "
+                  f"adj {times['adjoint'][str(shape)]:.3f} ms", flush=True)
+    parse = lambda k: tuple(json.loads(k.replace("(", "[").replace(")", "]")))
+    best_f = parse(min(times["forward"], key=times["forward"].get))
+    best_a = parse(min(times["adjoint"], key=times["adjoint"].get))
+    return dict(forward=best_f, adjoint=best_a, times=times)

@@ -61,8 +61,10 @@ class MultiProjector:
     """A / matched A^T over uneven instances, one kernel launch per call.
 
     ``bricks``: per-instance (forward, adjoint) brick shapes; default
-    24x24x14 / 32x16x16 (the single-call defaults), or ``"tune"`` to run the
-    autotuner per instance.  ``persistent_blocks``: workgroups that pull
+    24x24x14 / 32x16x16 (the single-call defaults), ``"tune"`` to run the
+    autotuner per instance, or ``"tune_shared"`` for the one A shape and one
+    A^T shape that time fastest on this multi launch (one compiled shape
+    keeps the launch on its constant-shape kernel).  ``persistent_blocks``: workgroups that pull
     (instance, brick) items from a device counter until the list is empty;
     ``"auto"`` (default) = as many as are resident at once for each kernel
     (the library caps the grid by the kernel's LDS and registers: three per
@@ -78,11 +80,15 @@ class MultiProjector:
             cus = torch.cuda.get_device_properties(self.device).multi_processor_count
             persistent_blocks = 8 * cus
         self.persistent_blocks = int(persistent_blocks)
+        self.tuned = None
         if bricks == "tune":
             bricks = []
             for g in self.geoms:
                 t = autotune.tune(g, self.device)
                 bricks.append((t["forward"]["brick"], t["adjoint"]["brick"]))
+        elif bricks == "tune_shared":
+            self.tuned = autotune.tune_multi(self.geoms, self.device)
+            bricks = [(self.tuned["forward"], self.tuned["adjoint"])] * len(self.geoms)
         self.bricks = bricks or [((24, 24, 14), (32, 16, 16))] * len(self.geoms)
         self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.views = [g.device_views(self.device) for g in self.geoms]

@@ -78,3 +78,22 @@ def test_autotune_choices_are_valid_and_equivalent(cuda):
     assert rel_l2(ops.forward(x, g, t["forward"]), ops.forward(x, g)) <= 1e-6
     assert rel_l2(ops.adjoint(y, g, t["adjoint"]), ops.adjoint(y, g)) <= 5e-6
     assert autotune.tune(g, cuda) is t   # cached
+
+
+def test_shared_shape_tuning_for_multi_launches(cuda):
+    """The multi-launch tuner returns one A shape and one A^T shape among its
+    candidates, and a projector built with them matches the default one."""
+    geoms = _small_uneven()
+    _case("tune_multi")
+    t = autotune.tune_multi(geoms, cuda, reps=1)
+    shapes = {tuple(c[:3]) for c in autotune.CANDIDATES if c[3] == 256}
+    assert t["forward"] in shapes and t["adjoint"] in shapes
+    mp = batched.MultiProjector(geoms, bricks="tune_shared")
+    assert mp.tuned is not None and mp.bricks[0] == (mp.tuned["forward"], mp.tuned["adjoint"])
+    ref = batched.MultiProjector(geoms)
+    vols = [phantom.gaussian(g.volume_shape(1), 90 + i).to(cuda) for i, g in enumerate(geoms)]
+    for a, b in zip(mp.forward(vols), ref.forward(vols)):
+        assert rel_l2(a, b) <= 1e-6
+    ys = [phantom.gaussian(g.sino_shape(1), 95 + i).to(cuda) for i, g in enumerate(geoms)]
+    for a, b in zip(mp.adjoint(ys), ref.adjoint(ys)):
+        assert rel_l2(a, b) <= 5e-6
