@@ -105,6 +105,13 @@ def test_capi_rejects_bad_arguments_without_gpu():
                                     _capi.int3((4, 4, 4)), _capi.int2((4, 4)),
                                     ctypes.c_void_p(16), 20, -1, None) == 5
     assert "multiple of 21" in lib.dtv_strerror(5).decode()
+    # beyond 32-bit per-volume indexing (2048 x 2048 x 1024 voxels)
+    sc.axis_mode = 3
+    views = np.zeros((4, 12))
+    assert lib.dtv_cone_forward(ctypes.c_void_p(16), ctypes.c_void_p(16), 1,
+                                _capi.int3((2048, 2048, 1024)), _capi.int2((64, 64)),
+                                views.ctypes.data_as(_capi.c_double_p), 4, ctypes.byref(sc),
+                                None) == 3
     # TV steps and the row filter: null pointers, aliasing, bad sizes and modes
     p = ctypes.c_void_p(16)
     q = ctypes.c_void_p(32)
