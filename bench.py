@@ -39,10 +39,11 @@ def _cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(views_sample, log):
+def cpu_baseline(views_sample, log, reps=3):
     """FP32 CPU restatement (the oracle in FP32) on a bounded sample of the
-    same workload: ``views_sample`` of the 60 views for A and for A^T.
-    Views are independent, so rays/s on the sample is the workload's rate."""
+    same workload: ``views_sample`` of the 60 views for A and for A^T, the
+    median of ``reps`` runs.  Views are independent, so rays/s on the sample
+    is the workload's rate."""
     import torch
     from oracle import cone
     from dtvproj import geometry as G, phantom
@@ -51,18 +52,24 @@ def cpu_baseline(views_sample, log):
     ang = G.uniform_angles(60)[views_sample]
     vol = phantom.phantom((256, 256, 256), 0.5, 20, seed=2)
     args = ((256, 256, 256), 0.5, (384, 384), 0.8, 600.0, 1100.0)
-    t0 = time.perf_counter()
-    sino = cone.circ_forward(vol, ang, *args, dtype=torch.float32)
-    t1 = time.perf_counter()
-    cone.circ_adjoint(sino, ang, *args, dtype=torch.float32)
-    t2 = time.perf_counter()
+    runs = []
+    for _ in range(max(1, reps)):
+        t0 = time.perf_counter()
+        sino = cone.circ_forward(vol, ang, *args, dtype=torch.float32)
+        t1 = time.perf_counter()
+        cone.circ_adjoint(sino, ang, *args, dtype=torch.float32)
+        t2 = time.perf_counter()
+        runs.append((t2 - t0, t1 - t0, t2 - t1))
+    runs.sort()
+    pair, fwd, adj = runs[len(runs) // 2]
     rays = len(views_sample) * 384 * 384
-    log(f"cpu baseline: A {t1 - t0:.2f}s  A^T {t2 - t1:.2f}s on {len(views_sample)} view(s), "
-        f"{threads} threads")
-    return dict(value=rays / (t2 - t0) / 1e9, unit="Grays/s", cores=threads, kind="port",
-                sample=f"{len(views_sample)} of 60 views (A then matched A^T), FP32 oracle "
+    log(f"cpu baseline: A {fwd:.2f}s  A^T {adj:.2f}s on {len(views_sample)} view(s), "
+        f"{threads} threads, median of {len(runs)}")
+    return dict(value=rays / pair / 1e9, unit="Grays/s", cores=threads, kind="port",
+                sample=f"{len(views_sample)} of 60 views (A then matched A^T; views "
+                       f"{list(views_sample)}), median of {len(runs)} runs, FP32 oracle "
                        f"restatement, torch CPU, {_cpu_model()}",
-                seconds_forward=t1 - t0, seconds_adjoint=t2 - t1)
+                seconds_forward=fwd, seconds_adjoint=adj)
 
 
 def batched_unroll(log):
@@ -187,7 +194,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-views", type=int, default=1)
+    ap.add_argument("--cpu-views", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-beijing", action="store_true")
@@ -296,7 +304,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         views = list(range(0, 60, max(1, 60 // max(1, args.cpu_views))))[:args.cpu_views]
-        cpu = cpu_baseline(views, log)
+        cpu = cpu_baseline(views, log, args.cpu_reps)
 
     bat = None if args.no_batched else batched_unroll(log)
 
