@@ -187,3 +187,33 @@ def test_l1ball_on_the_gpu_matches_oracle(cuda):
         want = torch.from_numpy(otv.l1ball(v.double().numpy(), b))
         got = tv.l1ball(v.to(cuda), b)
         assert rel_l2(got, want) <= 1e-5
+
+
+def test_asd_pocs_step_schedule(monkeypatch):
+    """The TV step is alpha * mean|first data step| and, as in the reference
+    (f0 aliases f there), never shrinks; with adaptive=True it shrinks by
+    alpha_red after a cascade whose TV change exceeds rmax times its data
+    change."""
+    from dtvproj import unroll
+    rng = np.random.default_rng(9)
+    A = torch.from_numpy(rng.normal(size=(30, 24)))
+    x = torch.from_numpy(rng.uniform(0.5, 1.5, size=(1, 1, 2, 3, 4)))
+    b = (A @ x.reshape(-1)).reshape(1, 1, 30)
+    proj = lambda f: (A @ f.reshape(-1)).reshape(1, 1, 30)
+    back = lambda y: (A.T @ y.reshape(-1)).reshape(1, 1, 2, 3, 4)
+    seen = []
+    real = tv.tv_descent_torch
+    monkeypatch.setattr(tv, "tv_descent_torch",
+                        lambda f, step, eps=1.0, mode="reference": (seen.append(step),
+                                                                    real(f, step, eps, mode))[1])
+    kw = dict(cascades=3, beta=1e-3, ng=2, alpha=50.0, fused=False)
+    f0 = torch.zeros_like(x)
+    unroll.asd_pocs(f0, b, proj, back, **kw)
+    assert len(seen) == 6 and len(set(seen)) == 1 and seen[0] > 0
+    dp0 = float((1e-3 * back(proj(f0) - b)).abs().mean())
+    assert abs(seen[0] - 50.0 * dp0) <= 1e-12 * seen[0]
+    seen.clear()
+    unroll.asd_pocs(f0, b, proj, back, adaptive=True, alpha_red=0.5, **kw)
+    # a large TV step moves f far more than the data step: every cascade halves it
+    assert seen[0:2] == [seen[0]] * 2 and abs(seen[2] - 0.5 * seen[0]) <= 1e-15 * seen[0]
+    assert abs(seen[4] - 0.25 * seen[0]) <= 1e-15 * seen[0]
