@@ -217,3 +217,20 @@ def test_asd_pocs_step_schedule(monkeypatch):
     # a large TV step moves f far more than the data step: every cascade halves it
     assert seen[0:2] == [seen[0]] * 2 and abs(seen[2] - 0.5 * seen[0]) <= 1e-15 * seen[0]
     assert abs(seen[4] - 0.25 * seen[0]) <= 1e-15 * seen[0]
+
+
+def test_dtv_cp_without_regularisation_solves_the_data_term():
+    """With v = 0 and mu = 0 the DTV Chambolle-Pock cascade is a primal-dual
+    iteration on 1/2 ||A f - b||^2 alone (dual w, over-relaxed primal); for a
+    consistent, well-conditioned system it converges to the solution."""
+    from dtvproj import unroll
+    rng = np.random.default_rng(8)
+    A = torch.from_numpy(rng.normal(size=(40, 24)) + 3 * np.eye(40, 24))
+    x = torch.from_numpy(rng.uniform(0.5, 1.5, size=(1, 1, 2, 3, 4)))
+    b = (A @ x.reshape(-1)).reshape(1, 1, 40)
+    proj = lambda f: (A @ f.reshape(-1)).reshape(1, 1, 40)
+    back = lambda y: (A.T @ y.reshape(-1)).reshape(1, 1, 2, 3, 4)
+    L = float(torch.linalg.matrix_norm(A, 2)) ** 2
+    out = unroll.dtv_cp(torch.zeros_like(x), b, proj, back, cascades=2000, v=(0.0, 0.0, 0.0),
+                        mu=0.0, sigma=0.5 / L, tau=0.5 / L)
+    assert float((out - x).abs().max()) <= 1e-8
