@@ -23,7 +23,6 @@ import numpy as np
 import torch
 
 from . import _capi, autotune, ops, tv, unroll
-from .geometry import ConeGeometry
 
 
 class Instance(ctypes.Structure):

@@ -32,7 +32,6 @@ Three constructors restate the reference geometries:
 """
 from __future__ import annotations
 
-import math
 from dataclasses import dataclass, field
 from typing import Sequence
 

@@ -6,8 +6,6 @@ paths see bit-identical inputs; move the result with ``.to(device)``.
 """
 from __future__ import annotations
 
-import math
-
 import torch
 
 
