@@ -4,7 +4,6 @@ Bar: relative L2 error <= 1e-5 for A and A^T, adjoint
 dot-product mismatch <= 1e-5.  Inputs come from seeded CPU generators, so the
 oracle and the GPU see identical data.
 """
-import numpy as np
 import pytest
 import torch
 

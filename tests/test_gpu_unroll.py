@@ -1,13 +1,13 @@
-"""GPU: autograd pair vs FP64 finite differences, FISTA-TV and the learned
-PDHG/DTV unroll vs the same algorithms driven by the FP64 oracle."""
-import numpy as np
+"""GPU: autograd pair vs FP64 finite differences; FISTA-TV, the learned
+PDHG/DTV unroll, ASD-POCS and the DTV Chambolle-Pock solver vs the same
+algorithms driven by the FP64 oracle."""
 import pytest
 import torch
 
 from conftest import rel_l2
 
 from oracle import cone
-from dtvproj import autograd, configs, geometry as G, ops, phantom, tv, unroll
+from dtvproj import autograd, configs, ops, phantom, tv, unroll
 
 pytestmark = pytest.mark.gpu
 
