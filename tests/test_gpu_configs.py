@@ -9,6 +9,9 @@ properties (the FP64 oracle would take hours at these sizes).
 * batch 4 of 128^3, 90 views, 192^2 (Poisson I0 = 1e5): one training step of
   the checkpointed 10-iteration PDHG/DTV network -- batch independence of the
   forward, finite gradients, and a loss that an SGD step lowers.
+* the reference's own geometries at full size (Standard options: 256x256x64,
+  48 views of 288x28; a 3024-view Beijing scan): A on a few views against
+  the oracle, the adjoint identity, the reference backprojector's linearity.
 """
 import pytest
 import torch
@@ -129,3 +132,49 @@ def test_large_512_training_step_five_checkpointed_iterations(cuda, large_case):
     # checkpointing keeps one iteration's activations: well under one set
     # of all five iterations' intermediates
     assert peak < 40.0
+
+
+# ---- the reference's own two geometries at full size ---------------------------
+
+def _ref_configs():
+    import numpy as np
+    from dtvproj import geometry as G
+    std = G.standard(G.uniform_angles(48), (256, 256, 64), (288, 28), 1000.0, 2000.0)
+    X, Y, Z, U, V = 256, 256, 64, 80, 128
+    mag = 1143.0 / 708.0
+    pitch = (1.2 * np.hypot(X, Y) * mag / (U * 21), 1.2 * Z * mag / V)
+    vec = G.beijing_default_vectors(144, 21, (U, V), 708.0, 1143.0, pitch)
+    bj = G.beijing(vec, (X, Y, Z), (U, V))
+    return std, bj, vec
+
+
+def test_reference_geometries_full_size(cuda):
+    """The Standard options geometry (256x256x64, 48 views of 288x28) and a
+    3024-view Beijing scan of the same volume: A on a few views against the
+    FP64 oracle (views are independent), the adjoint dot-product identity
+    over all views, and the reference backprojector's linearity."""
+    import numpy as np
+    from oracle import cone
+    from dtvproj import geometry as G
+    std, bj, vec = _ref_configs()
+    x = phantom.phantom((256, 256, 64), 1.0, 20, seed=11)
+    xd = x.to(cuda)
+    for name, g in (("standard", std), ("beijing", bj)):
+        print(f"[case] reference geometry {name} views={g.n_views}", flush=True)
+        y = ops.forward(xd, g)
+        pick = [0, g.n_views // 3, g.n_views - 1]
+        if name == "standard":
+            ref = cone.std_forward(x.double(), G.uniform_angles(48)[pick], (256, 256, 64),
+                                   (288, 28), 1000.0, 2000.0)
+        else:
+            ref = cone.bj_forward(x.double(), vec[pick], (256, 256, 64), (80, 128))
+        assert rel_l2(y[:, :, pick], ref) <= 1e-5
+        # a noisy measurement of x as the A^T input (as in the 512 case): with
+        # a pure N(0,1) sinogram and a non-negative x the dot product is a small
+        # difference of large sums, below the FP32 rounding of either side
+        s = y + phantom.gaussian(g.sino_shape(1), 12).to(cuda)
+        z = ops.adjoint(s, g)
+        lhs, rhs = _dot(y, s), _dot(xd, z)
+        assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+        b1, b2 = ops.bp_ref(s, g), ops.bp_ref(y, g)
+        assert rel_l2(ops.bp_ref(s + 2 * y, g), b1 + 2 * b2) <= 1e-5
