@@ -451,9 +451,12 @@ def intersect_lines_2d(p1, p2, p3, p4):
     return torch.where(par, nan, x / dnom), torch.where(par, nan, y / dnom)
 
 
-def std_bp_ref(sino, angles, volume_size, detector_size, sid, sdd, dtype=F64):
+def std_bp_ref(sino, angles, volume_size, detector_size, sid, sdd, dtype=F64, voxels=None):
     """Standard backwardKernel.cu:22-53 (voxel-driven, bilinear, weight
-    (SDD/L)^2 * pi / A), summed over views."""
+    (SDD/L)^2 * pi / A), summed over views.
+
+    ``voxels`` (optional int array [K, 3] of (ix, iy, iz)): evaluate only
+    those voxels and return [B, K] (the same arithmetic per voxel)."""
     X, Y, Z = (int(v) for v in volume_size)
     U, V = (int(v) for v in detector_size)
     sino = sino.to(dtype)
@@ -461,13 +464,19 @@ def std_bp_ref(sino, angles, volume_size, detector_size, sid, sdd, dtype=F64):
     A = len(angles)
     s = sid / sdd                                                       # :29
     ang = torch.as_tensor(np.asarray(angles, dtype=np.float64), dtype=dtype)
-    ix = torch.arange(X, dtype=dtype).view(1, X)
-    iy = torch.arange(Y, dtype=dtype).view(Y, 1)
+    if voxels is None:
+        ix = torch.arange(X, dtype=dtype).view(1, X)
+        iy = torch.arange(Y, dtype=dtype).view(Y, 1)
+        kk = torch.arange(Z, dtype=dtype).view(Z, 1, 1)
+        shape = (Z, Y, X)
+    else:
+        vox = torch.as_tensor(np.asarray(voxels), dtype=dtype).view(-1, 3)
+        ix, iy, kk = vox[:, 0].view(1, -1), vox[:, 1].view(1, -1), vox[:, 2].view(1, 1, -1)
+        shape = (1, vox.shape[0])
     cx = ((-X / 2.0) + ix) * s                                          # :30
     cy = ((-Y / 2.0) + iy) * s
     cx, cy = torch.broadcast_tensors(cx, cy)
-    kk = torch.arange(Z, dtype=dtype).view(Z, 1, 1)
-    out = torch.zeros(B, 1, Z, Y, X, dtype=dtype)
+    out = torch.zeros(B, 1, *shape, dtype=dtype)
     for a in range(A):
         theta = ang[a] / 180 * PI_REF - PI_REF / 2                       # :32
         exx, exy = torch.cos(theta), torch.sin(theta)
@@ -483,12 +492,15 @@ def std_bp_ref(sino, angles, volume_size, detector_size, sid, sdd, dtype=F64):
         z0 = (-Z / 2.0) * dz - (-V / 2.0)                               # :42
         coff = dz / s                                                   # :44
         zk = z0.unsqueeze(0) + kk * dz.unsqueeze(0)                     # :51 (exact)
-        xk = x.unsqueeze(0).expand(Z, Y, X)
-        wk = (coff * coff * PI_REF / A).unsqueeze(0)
+        if voxels is None:
+            xk = x.unsqueeze(0).expand(Z, Y, X)
+            wk = (coff * coff * PI_REF / A).unsqueeze(0)
+        else:
+            zk, xk, wk = zk[0], x, coff * coff * PI_REF / A
         for b in range(B):
-            val = _bilinear(sino[b, 0, a], xk.reshape(-1), zk.reshape(-1)).reshape(Z, Y, X)
+            val = _bilinear(sino[b, 0, a], xk.reshape(-1), zk.reshape(-1)).reshape(shape)
             out[b, 0] += val * wk                                       # :49-50
-    return out
+    return out if voxels is None else out[:, 0, 0]
 
 
 def _det3(a, b, c):
@@ -498,9 +510,12 @@ def _det3(a, b, c):
             + a[..., 2] * (b[..., 0] * c[..., 1] - b[..., 1] * c[..., 0]))
 
 
-def bj_bp_ref(sino, project_vector, volume_size, detector_size, dtype=F64):
+def bj_bp_ref(sino, project_vector, volume_size, detector_size, dtype=F64, voxels=None):
     """Beijing backwardKernel.cu:18-45 (voxel-driven, triple-product
-    projection, skip rule on *volume* dimensions, weight fr * 2pi/(n/21))."""
+    projection, skip rule on *volume* dimensions, weight fr * 2pi/(n/21)).
+
+    ``voxels`` (optional int array [K, 3] of (ix, iy, iz)): evaluate only
+    those voxels and return [B, K] (the same arithmetic per voxel)."""
     X, Y, Z = (int(v) for v in volume_size)
     U, V = (int(v) for v in detector_size)
     P = torch.as_tensor(np.asarray(project_vector, dtype=np.float32)).to(dtype)
@@ -508,11 +523,17 @@ def bj_bp_ref(sino, project_vector, volume_size, detector_size, dtype=F64):
     angles_num = n // 21                                                # host :96
     sino = sino.to(dtype)
     B = sino.shape[0]
-    iz = torch.arange(Z, dtype=dtype).view(Z, 1, 1)
-    iy = torch.arange(Y, dtype=dtype).view(1, Y, 1)
-    ix = torch.arange(X, dtype=dtype).view(1, 1, X)
-    c = torch.stack(torch.broadcast_tensors(-X / 2.0 + ix, -Y / 2.0 + iy, -Z / 2.0 + iz), -1)
-    out = torch.zeros(B, 1, Z, Y, X, dtype=dtype)
+    if voxels is None:
+        iz = torch.arange(Z, dtype=dtype).view(Z, 1, 1)
+        iy = torch.arange(Y, dtype=dtype).view(1, Y, 1)
+        ix = torch.arange(X, dtype=dtype).view(1, 1, X)
+        c = torch.stack(torch.broadcast_tensors(-X / 2.0 + ix, -Y / 2.0 + iy, -Z / 2.0 + iz), -1)
+        shape = (Z, Y, X)
+    else:
+        vox = torch.as_tensor(np.asarray(voxels), dtype=dtype).view(-1, 3)
+        c = torch.stack([-X / 2.0 + vox[:, 0], -Y / 2.0 + vox[:, 1], -Z / 2.0 + vox[:, 2]], -1)
+        shape = (vox.shape[0],)
+    out = torch.zeros(B, 1, *shape, dtype=dtype)
     for j in range(n):
         S = torch.stack([P[j, 0], P[j, 1], -P[j, 2]])                  # :29
         D = torch.stack([P[j, 3], P[j, 4], -P[j, 5]])                  # :30
@@ -530,5 +551,6 @@ def bj_bp_ref(sino, project_vector, volume_size, detector_size, dtype=F64):
         skip = (sx < -1) | (sx > X + 1) | (sy < -1) | (sy > Y + 1)     # :38
         for b in range(B):
             val = _bilinear(sino[b, 0, j], (sx - 0.5).reshape(-1), (sy - 0.5).reshape(-1))
-            out[b, 0] += torch.where(skip, torch.zeros_like(fr), fr * val.reshape(Z, Y, X))
-    return out * (2 * PI_REF / angles_num)                              # :43
+            out[b, 0] += torch.where(skip, torch.zeros_like(fr), fr * val.reshape(shape))
+    out = out * (2 * PI_REF / angles_num)                               # :43
+    return out if voxels is None else out[:, 0]

@@ -152,7 +152,8 @@ def test_reference_geometries_full_size(cuda):
     """The Standard options geometry (256x256x64, 48 views of 288x28) and a
     3024-view Beijing scan of the same volume: A on a few views against the
     FP64 oracle (views are independent), the adjoint dot-product identity
-    over all views, and the reference backprojector's linearity."""
+    over all views, and the reference backprojector's linearity and its
+    values at 300 voxels against the oracle."""
     import numpy as np
     from oracle import cone
     from dtvproj import geometry as G
@@ -178,3 +179,16 @@ def test_reference_geometries_full_size(cuda):
         assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
         b1, b2 = ops.bp_ref(s, g), ops.bp_ref(y, g)
         assert rel_l2(ops.bp_ref(s + 2 * y, g), b1 + 2 * b2) <= 1e-5
+        # the reference backprojector of the projection against the oracle at
+        # 300 seeded voxels (all views)
+        rng = np.random.default_rng(13)
+        vox = np.stack([rng.integers(0, 256, 300), rng.integers(0, 256, 300),
+                        rng.integers(0, 64, 300)], 1)
+        yc = y.cpu()
+        if name == "standard":
+            want = cone.std_bp_ref(yc, G.uniform_angles(48), (256, 256, 64), (288, 28), 1000.0,
+                                   2000.0, voxels=vox)
+        else:
+            want = cone.bj_bp_ref(yc, vec, (256, 256, 64), (80, 128), voxels=vox)
+        got = b2[0, 0].cpu()[vox[:, 2], vox[:, 1], vox[:, 0]]
+        assert rel_l2(got, want[0]) <= 1e-5
