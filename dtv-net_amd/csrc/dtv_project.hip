@@ -232,9 +232,14 @@ struct BjView {
 __global__ __launch_bounds__(kBpTileX) void k_bp_beijing(
     const float* __restrict__ sino, float* __restrict__ vol, Geo g, int b) {
   __shared__ BjView vw[kBpViewChunk];
+  __shared__ int live[kBpViewChunk];
   const int ix = blockIdx.x * kBpTileX + threadIdx.x;
   const int iy = blockIdx.y;
   const int z0 = blockIdx.z * kBpZ;
+  // this workgroup's voxels: x in [xa, xb], row iy, z in [z0, zb] (centres)
+  const double xa = blockIdx.x * kBpTileX - g.nx / 2.0;
+  const double xb = min((int)(blockIdx.x + 1) * kBpTileX, g.nx) - 1 - g.nx / 2.0;
+  const double za = z0 - g.nz / 2.0, zb = min(z0 + kBpZ, g.nz) - 1 - g.nz / 2.0;
   float acc[kBpZ];
 #pragma unroll
   for (int k = 0; k < kBpZ; ++k) acc[k] = 0.f;
@@ -263,10 +268,36 @@ __global__ __launch_bounds__(kBpTileX) void k_bp_beijing(
       w.m2[2] = sd[0] * uu[1] - sd[1] * uu[0];
       w.fr = sd[0] * w.n[0] + sd[1] * w.n[1] + sd[2] * w.n[2];
       for (int c = 0; c < 3; ++c) w.S[c] = S[c];
+      // Views whose detector this workgroup's voxels cannot reach add exact
+      // zeros: when den has one strict sign at every corner of the voxel
+      // rectangle (so on all of it: den is affine) the projective map sends
+      // the rectangle inside the hull of the corner images, and a voxel adds
+      // nothing once its taps (floor(s - 0.5), +1) all leave the detector,
+      // i.e. sx < -0.5 or sx >= U + 0.5 (likewise sy); one more pixel of
+      // slack covers rounding.
+      int pos = 0, neg = 0;
+      double sxl = 1e300, sxh = -1e300, syl = 1e300, syh = -1e300;
+      for (int q = 0; q < 4; ++q) {
+        const double c0 = ((q & 1) ? xb : xa) - S[0], c1 = (iy - g.ny / 2.0) - S[1];
+        const double c2 = ((q & 2) ? zb : za) - S[2];
+        const double den = -(c0 * w.n[0] + c1 * w.n[1] + c2 * w.n[2]);
+        pos += den > 0.0;
+        neg += den < 0.0;
+        const double sx = (c0 * w.m1[0] + c1 * w.m1[1] + c2 * w.m1[2]) / den + g.nu / 2.0;
+        const double sy = (c0 * w.m2[0] + c1 * w.m2[1] + c2 * w.m2[2]) / den + g.nv / 2.0;
+        sxl = fmin(sxl, sx);
+        sxh = fmax(sxh, sx);
+        syl = fmin(syl, sy);
+        syh = fmax(syh, sy);
+      }
+      const bool keep = pos != 4 && neg != 4;
+      const bool out = sxh < -1.5 || sxl > g.nu + 1.5 || syh < -1.5 || syl > g.nv + 1.5;
+      live[threadIdx.x] = keep || !out || !(sxl <= sxh && syl <= syh);
     }
     __syncthreads();
     if (ix < g.nx) {
       for (int j = 0; j < na; ++j) {
+        if (!live[j]) continue;
         const BjView& w = vw[j];
         const double csx = cx - w.S[0], csy = cy - w.S[1];
         const double csz = (z0 - g.nz / 2.0) - w.S[2];
