@@ -122,7 +122,8 @@ def test_fista_64_phantom_and_random(cuda):
 
 def test_sparse_256_views_subset(cuda):
     """Full-size 256^3 volume and 384^2 detector, 3 of the 60 views (views are
-    independent, so the subset is a faithful slice of the workload)."""
+    independent, so the subset is a faithful slice of the workload): A and
+    the matched A^T against the oracle."""
     g = configs.sparse_256()
     idx = [0, 17, 41]
     gs = g.subset(idx)
@@ -132,6 +133,11 @@ def test_sparse_256_views_subset(cuda):
     ang = G.uniform_angles(60)[idx]
     ref = cone.circ_forward(vol, ang, (256, 256, 256), 0.5, (384, 384), 0.8, 600.0, 1100.0)
     assert rel_l2(out, ref) <= TOL
+    # and the matched adjoint over the same views (full-size brick kernels)
+    y = phantom.gaussian(gs.sino_shape(1), 21)
+    adj = ops.adjoint(y.to(cuda), gs)
+    ref = cone.circ_adjoint(y, ang, (256, 256, 256), 0.5, (384, 384), 0.8, 600.0, 1100.0)
+    assert rel_l2(adj, ref) <= TOL
 
 
 def test_sparse_256_adjoint_dot(cuda):
