@@ -21,6 +21,13 @@
 //   out = f - step * sum_a D_a^T (D_a f / (|D f| + eps)),
 // the exact gradient step on sum_c [r_c - eps * log(1 + r_c / eps)],
 // r_c = |D f|_c (the reference's eps is 1).
+//
+// The primal-dual (Chambolle-Pock) step of the directional-TV unroll runs
+// as two launches after A and A^T: the dual box projection of
+// p_a + sigma D_a fbar onto [-lam_a, lam_a] (the l1-ball dual step of
+// model/PDHG/DTVCP.py:41-46 with a box in place of the ball), then the
+// primal step f' = max(f - tau (A^T w + sum_a D_a^T p_a), 0) with the
+// extrapolation fbar = 2 f' - f.  28 B per voxel each.
 #include <hip/hip_runtime.h>
 
 #include "dtvproj.h"
@@ -151,6 +158,62 @@ __global__ __launch_bounds__(256) void k_tv_descent(const float* __restrict__ f,
   out[off + i] = S.f[i] - step * tv_grad_at<true>(S, i, eps);
 }
 
+// one PDHG step of the directional-TV unroll, after A and A^T.  Dual half:
+//   p_a <- clamp(p_a + sigma * d_a(fbar), -lam_a, lam_a)
+// (p: three planes [3][batch][Z][Y][X], updated in place; each lane reads
+// only its own p entries).
+__global__ __launch_bounds__(256) void k_pdhg_dual(const float* __restrict__ fbar,
+                                                   float* __restrict__ p, int nx, int ny,
+                                                   int nz, int64_t plane, float sigma, float l0,
+                                                   float l1, float l2, int mode) {
+  const int64_t nvox = (int64_t)nx * ny * nz;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nvox) return;
+  const int64_t off = (int64_t)blockIdx.y * nvox;
+  const Stencil S = make_stencil(fbar + off, nx, ny, nz, mode);
+  const float lam[3] = {l0, l1, l2};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float* pa = p + a * plane + off;
+    const float v = pa[i] + sigma * diff(S, i, a);
+    pa[i] = fminf(fmaxf(v, -lam[a]), lam[a]);
+  }
+}
+
+// Primal half: g = atw + sum_a D_a^T p_a;  f_new = max(f - tau g, 0);
+// fbar <- 2 f_new - f;  f <- f_new.  D_a^T p at i is p_a(i - s_a) where row
+// i - s_a of D_a has its +1 at i, minus p_a(i) where row i is not empty (in
+// the reference boundary mode every row has its -1).
+__global__ __launch_bounds__(256) void k_pdhg_primal(float* __restrict__ f,
+                                                     float* __restrict__ fbar,
+                                                     const float* __restrict__ atw,
+                                                     const float* __restrict__ p, int nx,
+                                                     int ny, int nz, int64_t plane, float tau,
+                                                     int mode) {
+  const int64_t nvox = (int64_t)nx * ny * nz;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nvox) return;
+  const int64_t off = (int64_t)blockIdx.y * nvox;
+  const Stencil S = make_stencil(nullptr, nx, ny, nz, mode);
+  float g = atw[off + i];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float* pa = p + a * plane + off;
+    float dt = has_prev(S, i, a) ? pa[i - S.s[a]] : 0.f;
+    bool own;
+    if (mode != 0) own = true;
+    else if (a == 0) own = (i % nx) != nx - 1;
+    else if (a == 1) own = ((i / nx) % ny) != ny - 1;
+    else own = (i / ((int64_t)nx * ny)) != nz - 1;
+    if (own) dt -= pa[i];
+    g += dt;
+  }
+  const float fo = f[off + i];
+  const float fn = fmaxf(fo - tau * g, 0.f);
+  f[off + i] = fn;
+  fbar[off + i] = 2.f * fn - fo;
+}
+
 }  // namespace
 
 extern "C" {
@@ -198,6 +261,31 @@ int dtv_tv_descent(const float* f, float* out, int32_t batch, const int32_t volu
   hipLaunchKernelGGL(k_tv_descent, dim3((unsigned)blocks, batch), dim3(256), 0,
                      (hipStream_t)stream, f, out, volume_size[0], volume_size[1],
                      volume_size[2], step, eps, mode);
+  return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
+}
+
+int dtv_pdhg_dtv_step(float* f, float* fbar, const float* atw, float* p, int32_t batch,
+                      const int32_t volume_size[3], float sigma, float tau, const float lam[3],
+                      int32_t mode, void* stream) {
+  if (!f || !fbar || !atw || !p || !volume_size || !lam) return DTV_ERR_NULL_POINTER;
+  if (batch < 1 || batch > 65535 || (mode != 0 && mode != 1))
+    return DTV_ERR_INVALID_ARGUMENT;
+  if ((const void*)f == fbar || (const void*)f == atw || (const void*)fbar == atw ||
+      (const void*)p == f || (const void*)p == fbar || (const void*)p == atw)
+    return DTV_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < 3; ++i)
+    if (volume_size[i] < 1 || !(lam[i] >= 0.f)) return DTV_ERR_INVALID_ARGUMENT;
+  const int64_t nvox = (int64_t)volume_size[0] * volume_size[1] * volume_size[2];
+  const int64_t blocks = (nvox + 255) / 256;
+  if (blocks > 0x7fffffff) return DTV_ERR_TOO_LARGE;
+  const int64_t plane = nvox * batch;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_pdhg_dual, dim3((unsigned)blocks, batch), dim3(256), 0, st, fbar, p,
+                     volume_size[0], volume_size[1], volume_size[2], plane, sigma, lam[0],
+                     lam[1], lam[2], mode);
+  if (hipGetLastError() != hipSuccess) return DTV_ERR_HIP;
+  hipLaunchKernelGGL(k_pdhg_primal, dim3((unsigned)blocks, batch), dim3(256), 0, st, f, fbar,
+                     atw, p, volume_size[0], volume_size[1], volume_size[2], plane, tau, mode);
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
 }
 

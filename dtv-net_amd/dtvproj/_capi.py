@@ -86,6 +86,8 @@ _SIGS = {
     "dtv_tv_fista_step_multi": [c_void_p, c_int32, ctypes.c_int64, c_int32, c_void_p],
     "dtv_tv_descent": [c_void_p, c_void_p, c_int32, c_int32_p, ctypes.c_float, ctypes.c_float,
                        c_int32, c_void_p],
+    "dtv_pdhg_dtv_step": [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32_p,
+                          ctypes.c_float, ctypes.c_float, c_float_p, c_int32, c_void_p],
     "dtv_row_filter": [c_void_p, c_void_p, ctypes.c_int64, c_int32, c_void_p, c_int32, c_void_p,
                        c_int32, c_void_p],
     "dtv_views_standard": [c_double_p, c_int32, c_int32_p, c_int32_p, c_double, c_double,

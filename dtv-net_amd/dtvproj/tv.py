@@ -159,6 +159,45 @@ def tv_descent(f, step, eps=1.0, mode="reference", out=None):
     return out
 
 
+def pdhg_dtv_step_torch(f, fbar, atw, p, sigma, tau, lam, mode="neumann"):
+    """One primal-dual step of the directional-TV unroll after A and A^T, as
+    torch ops (differentiable): p_a = clamp(p_a + sigma D_a fbar, -lam_a,
+    lam_a); f' = relu(f - tau (atw + sum_a D_a^T p_a)); returns
+    (f', 2 f' - f, [p_x, p_y, p_z])."""
+    axes = ("x", "y", "z")
+    p = [torch.clamp(p[i] + sigma * D(fbar, a, mode), -lam[i], lam[i])
+         for i, a in enumerate(axes)]
+    grad = atw + DT(p[0], "x", mode) + DT(p[1], "y", mode) + DT(p[2], "z", mode)
+    fn = torch.nn.functional.relu(f - tau * grad)
+    return fn, 2 * fn - f, p
+
+
+def pdhg_dtv_step(f, fbar, atw, p, sigma, tau, lam, mode="neumann"):
+    """:func:`pdhg_dtv_step_torch` in place on the fused gfx950 kernels
+    (``dtv_pdhg_dtv_step``): f, fbar [B, 1, Z, Y, X] and p [3, B, 1, Z, Y, X]
+    FP32 on the GPU are overwritten with f', 2 f' - f and the new p; atw is
+    read.  The four tensors must be distinct and contiguous."""
+    ts = {"f": f, "fbar": fbar, "atw": atw, "p": p}
+    for name, t in ts.items():
+        check_input(t, name)
+        if t.dtype != torch.float32:
+            raise RuntimeError(f"{name} must be float32")
+    if fbar.shape != f.shape or atw.shape != f.shape or p.shape != (3,) + tuple(f.shape):
+        raise RuntimeError("fbar and atw must have the shape of f, p the shape [3, *f.shape]")
+    if len({t.untyped_storage().data_ptr() for t in ts.values()}) != 4:
+        raise RuntimeError("f, fbar, atw and p must not share storage")
+    Z, Y, X = f.shape[-3:]
+    B = f.numel() // (X * Y * Z)
+    lam3 = (ctypes.c_float * 3)(*[float(v) for v in lam])
+    with torch.cuda.device(f.device):
+        rc = _capi.lib().dtv_pdhg_dtv_step(f.data_ptr(), fbar.data_ptr(), atw.data_ptr(),
+                                           p.data_ptr(), B, _capi.int3((X, Y, Z)),
+                                           ctypes.c_float(sigma), ctypes.c_float(tau), lam3,
+                                           MODES[mode], _stream(f.device))
+    _capi.check(rc, "dtv_pdhg_dtv_step")
+    return f, fbar, p
+
+
 def l1ball(v, b, tol=1e-4, max_iter=100):
     """Shrink v onto the l1 ball of radius b as model/PDHG/Utils/L1ball.py:4-17
     does: if sum|v| < b, v itself; else the threshold lam solving

@@ -88,13 +88,19 @@ def fista_tv(b, geom: ConeGeometry, iters: int, lam: float, eps: float, step: fl
 
 class PDHGDTVNet(torch.nn.Module):
     """Learned ``iters``-iteration PDHG for directional TV with per-iteration
-    sigma_k, tau_k and per-axis weights lam_{a,k} (softplus-positive)."""
+    sigma_k, tau_k and per-axis weights lam_{a,k} (softplus-positive).
+
+    Without autograd (``torch.no_grad``, inference) on GPU tensors the
+    volume updates of each iteration run as the two fused launches of
+    :func:`tv.pdhg_dtv_step` in preallocated buffers (``fused=False`` keeps
+    the torch ops)."""
 
     def __init__(self, geom: ConeGeometry, iters: int = 10, sigma: float = 0.5,
                  tau: float = 0.5, lam=(1e-2, 1e-2, 1e-2), norm: float = 1.0,
                  mode: str = "neumann", checkpointing: bool = True, project=None,
-                 backproject=None):
+                 backproject=None, fused: bool = True):
         super().__init__()
+        self.fused = fused
         self.geom = geom
         # differentiable A and A^T; defaults are the gfx950 matched pair
         self.A = project or (lambda x: _ag.project(x, geom))
@@ -115,15 +121,31 @@ class PDHGDTVNet(torch.nn.Module):
         lam = F.softplus(self.lam[k])
         res = self.A(fbar) - self._b
         w = (w + s * res) / (1 + s)
-        px = torch.clamp(px + s * tv.D(fbar, "x", self.mode), -lam[0], lam[0])
-        py = torch.clamp(py + s * tv.D(fbar, "y", self.mode), -lam[1], lam[1])
-        pz = torch.clamp(pz + s * tv.D(fbar, "z", self.mode), -lam[2], lam[2])
-        grad = self.AT(w) + tv.DT(px, "x", self.mode) + \
-            tv.DT(py, "y", self.mode) + tv.DT(pz, "z", self.mode)
-        fn = F.relu(f - t * grad)
-        return fn, 2 * fn - f, w, px, py, pz
+        fn, fbar, (px, py, pz) = tv.pdhg_dtv_step_torch(f, fbar, self.AT(w), (px, py, pz), s,
+                                                        t, lam, self.mode)
+        return fn, fbar, w, px, py, pz
+
+    @torch.no_grad()
+    def _forward_fused(self, b, f0):
+        f = torch.zeros(self.geom.volume_shape(b.shape[0]), device=b.device, dtype=b.dtype) \
+            if f0 is None else f0.detach().clone().contiguous()
+        fbar = f.clone()
+        w = torch.zeros_like(b)
+        p = torch.zeros((3,) + tuple(f.shape), device=f.device, dtype=f.dtype)
+        # the step sizes as the torch iteration forms them (float32), one sync
+        s = (F.softplus(self.sigma) / self.knorm).tolist()
+        s1 = (1 + F.softplus(self.sigma) / self.knorm).tolist()
+        t = (F.softplus(self.tau) / self.knorm).tolist()
+        lam = F.softplus(self.lam).tolist()
+        for k in range(self.iters):
+            r = self.A(fbar)
+            w.add_(r.sub_(b), alpha=s[k]).div_(s1[k])
+            tv.pdhg_dtv_step(f, fbar, self.AT(w), p, s[k], t[k], lam[k], self.mode)
+        return f
 
     def forward(self, b, f0=None):
+        if self.fused and b.is_cuda and not torch.is_grad_enabled():
+            return self._forward_fused(b, f0)
         self._b = b
         B = b.shape[0]
         f = torch.zeros(self.geom.volume_shape(B), device=b.device, dtype=b.dtype) \

@@ -221,6 +221,21 @@ int dtv_tv_fista_step_multi(const dtv_tv_item* items, int32_t n_items, int64_t m
 int dtv_tv_descent(const float* f, float* out, int32_t batch, const int32_t volume_size[3],
                    float step, float eps, int32_t mode, void* stream);
 
+/* One primal-dual step of the learned directional-TV PDHG unroll, after the
+ * caller has run A on fbar, updated w and run A^T on w into atw (the update
+ * of model/PDHG/DTVCP.py:41-52, with the dual l1-ball projection of
+ * L1ball.py:4-17 replaced by the box [-lam_a, lam_a]; D_a as in
+ * dtv_tv_fista_step):
+ *   p_a  <- clamp(p_a + sigma D_a fbar, -lam_a, lam_a)        (a = x, y, z)
+ *   f'    = max(f - tau (atw + sum_a D_a^T p_a), 0)
+ *   fbar <- 2 f' - f;  f <- f'
+ * f, fbar, atw: device float32 [batch][Z][Y][X]; p: [3][batch][Z][Y][X]
+ * (x, y, z planes); f, fbar and p are updated in place, all four distinct;
+ * lam: host float[3] >= 0.  Two launches on ``stream``. */
+int dtv_pdhg_dtv_step(float* f, float* fbar, const float* atw, float* p, int32_t batch,
+                      const int32_t volume_size[3], float sigma, float tau, const float lam[3],
+                      int32_t mode, void* stream);
+
 /* ---- FBP front end ------------------------------------------------------ */
 
 /* Cosine weighting fused with the per-row filtering of the FBP modules:

@@ -124,6 +124,18 @@ def test_capi_rejects_bad_arguments_without_gpu():
     assert lib.dtv_tv_descent(p, q, 0, v3, f32(0.1), f32(1.0), 0, None) == 1   # batch
     assert lib.dtv_tv_descent(p, q, 1, _capi.int3((4, 0, 4)), f32(0.1), f32(1.0), 0,
                               None) == 1
+    r = ctypes.c_void_p(48)
+    s = ctypes.c_void_p(64)
+    lam = (f32 * 3)(0.1, 0.1, 0.1)
+    neg = (f32 * 3)(0.1, -0.1, 0.1)
+    assert lib.dtv_pdhg_dtv_step(p, q, r, None, 1, v3, f32(0.5), f32(0.5), lam, 0, None) == 2
+    assert lib.dtv_pdhg_dtv_step(p, q, r, s, 1, v3, f32(0.5), f32(0.5), None, 0, None) == 2
+    assert lib.dtv_pdhg_dtv_step(p, p, r, s, 1, v3, f32(0.5), f32(0.5), lam, 0, None) == 1
+    assert lib.dtv_pdhg_dtv_step(p, q, q, s, 1, v3, f32(0.5), f32(0.5), lam, 0, None) == 1
+    assert lib.dtv_pdhg_dtv_step(p, q, r, p, 1, v3, f32(0.5), f32(0.5), lam, 0, None) == 1
+    assert lib.dtv_pdhg_dtv_step(p, q, r, s, 1, v3, f32(0.5), f32(0.5), neg, 0, None) == 1
+    assert lib.dtv_pdhg_dtv_step(p, q, r, s, 1, v3, f32(0.5), f32(0.5), lam, 3, None) == 1
+    assert lib.dtv_pdhg_dtv_step(p, q, r, s, 0, v3, f32(0.5), f32(0.5), lam, 0, None) == 1
     assert lib.dtv_row_filter(p, q, 4, 8, None, 0, p, 4, None) == 1            # even taps
     assert lib.dtv_row_filter(p, p, 4, 8, None, 0, q, 5, None) == 1            # in place
     assert lib.dtv_row_filter(None, q, 4, 8, None, 0, p, 5, None) == 2

@@ -108,6 +108,17 @@ def test_large_512_dtv_reconstruction(cuda, large_case):
             print(f"  {iters} iterations: {time.perf_counter() - t0:.2f} s, "
                   f"rel error {errs[iters]:.4f}", flush=True)
             assert torch.isfinite(f).all()
+            if iters == 5:
+                # the fused volume updates vs the torch ops of the same network
+                net.fused = False
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                plain = net(y)
+                torch.cuda.synchronize()
+                print(f"  5 iterations, torch volume updates: {time.perf_counter() - t0:.2f} s",
+                      flush=True)
+                assert _rel(f, plain) <= 1e-5
+                del plain
     assert errs[30] < errs[5] < 1.0
 
 

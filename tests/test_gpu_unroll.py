@@ -155,6 +155,26 @@ def test_pdhg_dtv_net_matches_fp64_oracle_network():
     assert abs(fd - float(net64.sigma.grad[1])) <= 1e-6 + 1e-4 * abs(fd)
 
 
+def test_pdhg_dtv_net_fused_inference_matches_torch_iteration():
+    """No-grad inference on the fused PDHG launches vs the torch ops of the
+    same network (both on the gfx950 projector pair)."""
+    cuda = torch.device("cuda", 0)
+    g = configs.fd_32()
+    _case("pdhg_dtv fused inference fd_32")
+    ph = phantom.phantom((32, 32, 32), 1.0, 6, seed=4).to(cuda)
+    with torch.no_grad():
+        b = ops.forward(torch.cat([ph, 0.5 * ph]), g)
+    for mode in ("neumann", "reference"):
+        net = unroll.PDHGDTVNet(g, iters=6, norm=unroll.power_norm(g, device=cuda),
+                                lam=(2e-2, 1e-2, 3e-2), mode=mode).to(cuda)
+        with torch.no_grad():
+            fused = net(b)
+            net.fused = False
+            plain = net(b)
+        assert rel_l2(fused, plain) <= 1e-5
+        assert float(plain.norm()) > 0
+
+
 def test_checkpointing_same_gradients_less_memory():
     cuda = torch.device("cuda", 0)
     g = configs.fd_32()

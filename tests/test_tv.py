@@ -24,6 +24,29 @@ def test_difference_operators_match_sparse_matrices(vs, mode, axis):
                                atol=1e-13)
 
 
+@pytest.mark.parametrize("mode", ["reference", "neumann"])
+def test_pdhg_dtv_step_torch_matches_sparse_matrices(mode):
+    """The torch PDHG step (reference for the fused kernels) against the
+    same update written with the reference's sparse matrices."""
+    X, Y, Z = 6, 5, 4
+    shp = (2, 1, Z, Y, X)
+    f, fbar, atw = (phantom.gaussian(shp, s, dtype=torch.float64) for s in (41, 42, 43))
+    p = [phantom.gaussian(shp, 44 + a, dtype=torch.float64) for a in range(3)]
+    sigma, tau, lam = 0.6, 0.3, (0.4, 0.7, 0.25)
+    fn, fb, pn = tv.pdhg_dtv_step_torch(f, fbar, atw, p, sigma, tau, lam, mode)
+    build = otv.reference_matrix if mode == "reference" else otv.neumann_matrix
+    Ms = [build((X, Y, Z), a) for a in "xyz"]
+    want_p = [np.clip(p[a].numpy() + sigma * otv.apply(Ms[a], fbar.numpy()), -lam[a], lam[a])
+              for a in range(3)]
+    grad = atw.numpy() + sum(otv.apply(Ms[a].T.tocsr(), want_p[a]) for a in range(3))
+    want_f = np.maximum(f.numpy() - tau * grad, 0.0)
+    for a in range(3):
+        np.testing.assert_allclose(pn[a].numpy(), want_p[a], atol=1e-13)
+    np.testing.assert_allclose(fn.numpy(), want_f, atol=1e-13)
+    np.testing.assert_allclose(fb.numpy(), 2 * want_f - f.numpy(), atol=1e-13)
+    assert (want_f == 0).any() and (np.abs(want_p[0]) == lam[0]).any()
+
+
 def test_reference_dy_uses_y_stride_quirk():
     """Gradients.py:21-23 strides D_y by volumeSize[1]; for X != Y that is
     not the y-neighbour."""
@@ -177,6 +200,35 @@ def test_fused_tv_descent_matches_oracle(cuda, mode, shp):
     with pytest.raises(RuntimeError, match="share storage"):
         fd = f.to(cuda)
         tv.tv_descent(fd, 0.25, 1.0, mode, out=fd)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["neumann", "reference"])
+@pytest.mark.parametrize("shp", [(2, 1, 13, 11, 17), (1, 1, 1, 1, 5), (1, 1, 9, 1, 1),
+                                 (3, 1, 4, 6, 5)])
+def test_fused_pdhg_dtv_step(cuda, mode, shp):
+    """The two fused launches vs the torch step in FP64 (whose D and D^T are
+    checked against the sparse matrices above); the clamp and the relu both
+    act on these inputs."""
+    print(f"[case] pdhg_dtv_step {mode} {shp}", flush=True)
+    f = phantom.gaussian(shp, 31) - 0.3
+    fbar = phantom.gaussian(shp, 32)
+    atw = phantom.gaussian(shp, 33)
+    p = torch.stack([phantom.gaussian(shp, 34 + a) - 0.4 for a in range(3)])
+    sigma, tau, lam = 0.7, 0.4, (0.3, 0.5, 0.2)
+    rf, rfb, rp = tv.pdhg_dtv_step_torch(f.double(), fbar.double(), atw.double(), p.double(),
+                                         sigma, tau, lam, mode)
+    if f.numel() >= 100:
+        assert float((rf == 0).double().mean()) > 0.05
+        assert float((rp[0].abs() == lam[0]).double().mean()) > 0.05
+    fd, fbd, pd = f.to(cuda), fbar.to(cuda), p.to(cuda)
+    tv.pdhg_dtv_step(fd, fbd, atw.to(cuda), pd, sigma, tau, lam, mode)
+    assert rel_l2(fd, rf) <= 1e-6
+    assert rel_l2(fbd, rfb) <= 1e-6
+    for a in range(3):
+        assert rel_l2(pd[a], rp[a]) <= 1e-6
+    with pytest.raises(RuntimeError, match="share storage"):
+        tv.pdhg_dtv_step(fd, fd, atw.to(cuda), pd, sigma, tau, lam, mode)
 
 
 @pytest.mark.gpu
