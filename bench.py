@@ -188,6 +188,44 @@ def beijing_loop(steps, log):
                      "elementwise work) on an unstated NVIDIA GPU")
 
 
+def dense_inputs(geom, sino, acct, reps, log):
+    """A on an i.i.d. N(0,1) volume (seed 20) and the matched A^T on an
+    i.i.d. N(0,1) sinogram (seed 21), medians over ``reps`` event-timed
+    launches.  The matched A^T drops rays whose value is zero, so on the
+    phantom's projection (``sino``, where rays that miss every ellipsoid
+    are zero) it does less than the algorithmic count; every ray of the
+    N(0,1) sinogram is nonzero."""
+    import torch
+    from dtvproj import ops, phantom
+    dev = sino.device
+    x = phantom.gaussian(geom.volume_shape(1), 20).to(dev)
+    y = phantom.gaussian(geom.sino_shape(1), 21).to(dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def med(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return statistics.median(ts)
+
+    fwd = med(lambda: ops.forward(x, geom))
+    adj = med(lambda: ops.adjoint(y, geom))
+    nz = float((sino != 0).double().mean())
+    log(f"dense inputs: forward {fwd:.3f} ms  adjoint {adj:.3f} ms; phantom sinogram "
+        f"nonzero rays {nz:.3f}")
+    return dict(forward_ms_median=fwd, adjoint_ms_median=adj,
+                pair_grays=acct["rays"] / ((fwd + adj) * 1e-3) / 1e9,
+                adjoint_achieved=acct["adjoint"] / (adj * 1e-3) / 1e9,
+                phantom_sino_nonzero_ray_frac=nz, reps=reps,
+                inputs="N(0,1) volume seed 20, N(0,1) sinogram seed 21")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -298,6 +336,8 @@ def main():
                 algorithmic_bytes=dict(forward=acct["forward"], adjoint=acct["adjoint"],
                                        samples=acct["samples"]))
 
+    dense = dense_inputs(geom, s, acct, max(3, min(args.steps, 20)), log)
+
     bref = ref_backprojector(s, max(1, min(args.steps, 10)), log)
     bj = None if args.no_beijing else beijing_loop(max(1, min(args.steps, 5)), log)
 
@@ -325,6 +365,7 @@ def main():
                                "384x384 @0.8mm, SOD 600, SDD 1100",
                    "batch": 1, "views": 60, "volume": [256, 256, 256], "detector": [384, 384]},
         "roofline": roof,
+        "dense_inputs": dense,
         "cpu_baseline": cpu,
         "reference_backprojector": bref,
         "beijing_loop": bj,
