@@ -491,25 +491,30 @@ __device__ __forceinline__ int round_to_int(float x) {
   return r;
 }
 
+// 1 + f as an FP32 value for the 32.32 fraction f = lo / 2^32, truncated to
+// its top 23 bits: the exponent bits of 1.0 above lo >> 9, one v_alignbit.
+__device__ __forceinline__ float one_plus_frac(int64_t x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_alignbit(0x7Fu, (uint32_t)x, 9u));
+}
+
 // Adds the trilinear taps of samples [ka, kb) of one ray (value val, tile-
 // relative 32.32 position X, step D) into the LDS tile: int32 fixed point
-// (kFixed, taps pre-scaled) or FP32.
+// (kFixed, taps pre-scaled) or FP32.  Each split w -> (w - f w, f w) takes
+// f w = fma(1 + f, w, -w), rounded once.
 template <bool kFixed>
 __device__ __forceinline__ void adj_march(int ka, int kb, int64_t X[3], const int64_t D[3],
                                           float val, int stz, int sty, int* iacc) {
   float* facc = reinterpret_cast<float*>(iacc);
   for (int k = ka; k < kb; ++k) {
     const int ix = (int)(X[0] >> 32), iy = (int)(X[1] >> 32), iz = (int)(X[2] >> 32);
-    const float fx = (float)(uint32_t)X[0] * kInvFix;
-    const float fy = (float)(uint32_t)X[1] * kInvFix;
-    const float fz = (float)(uint32_t)X[2] * kInvFix;
-    const float wz1 = fz * val, wz0 = val - wz1;
-    const float w10 = fy * wz0, w00 = wz0 - w10;
-    const float w11 = fy * wz1, w01 = wz1 - w11;
-    const float t001 = fx * w00, t000 = w00 - t001;
-    const float t011 = fx * w10, t010 = w10 - t011;
-    const float t101 = fx * w01, t100 = w01 - t101;
-    const float t111 = fx * w11, t110 = w11 - t111;
+    const float gx = one_plus_frac(X[0]), gy = one_plus_frac(X[1]), gz = one_plus_frac(X[2]);
+    const float wz1 = fmaf(gz, val, -val), wz0 = val - wz1;
+    const float w10 = fmaf(gy, wz0, -wz0), w00 = wz0 - w10;
+    const float w11 = fmaf(gy, wz1, -wz1), w01 = wz1 - w11;
+    const float t001 = fmaf(gx, w00, -w00), t000 = w00 - t001;
+    const float t011 = fmaf(gx, w10, -w10), t010 = w10 - t011;
+    const float t101 = fmaf(gx, w01, -w01), t100 = w01 - t101;
+    const float t111 = fmaf(gx, w11, -w11), t110 = w11 - t111;
     const int base =
         (int)(__umul24((unsigned)iz, (unsigned)stz) + __umul24((unsigned)iy, (unsigned)sty)) + ix;
     if (kFixed) {
