@@ -237,6 +237,8 @@ def main():
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-beijing", action="store_true")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the dense N(0,1)-input leg (profiles of the headline pair alone)")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="run one device copy of this size first (rocprof counter scale)")
     args = ap.parse_args()
@@ -336,7 +338,8 @@ def main():
                 algorithmic_bytes=dict(forward=acct["forward"], adjoint=acct["adjoint"],
                                        samples=acct["samples"]))
 
-    dense = dense_inputs(geom, s, acct, max(3, min(args.steps, 20)), log)
+    dense = None if args.no_dense else dense_inputs(geom, s, acct, max(3, min(args.steps, 20)),
+                                                    log)
 
     bref = ref_backprojector(s, max(1, min(args.steps, 10)), log)
     bj = None if args.no_beijing else beijing_loop(max(1, min(args.steps, 5)), log)
