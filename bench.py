@@ -226,6 +226,37 @@ def dense_inputs(geom, sino, acct, reps, log):
                 inputs="N(0,1) volume seed 20, N(0,1) sinogram seed 21")
 
 
+def pdhg_512(iters, log):
+    """Inference of the learned directional-TV PDHG unroll at 512x512x384 /
+    120 views / 768x576 (40-ellipsoid phantom, Poisson I0 = 5e4 sinogram,
+    weights 2e-3 / 2e-3 / 4e-3 for x / y / z): one warm-up pass, then one
+    timed pass of ``iters`` iterations."""
+    import time
+    import torch
+    from dtvproj import configs, ops, phantom, unroll
+    dev = torch.device("cuda", 0)
+    g = configs.large_512()
+    truth = (phantom.phantom(g.volume_size, 0.4, 40, seed=5) * 0.02).to(dev)
+    with torch.no_grad():
+        y = phantom.poisson_noise(ops.forward(truth, g), 5e4, seed=8)
+        norm = unroll.power_norm(g, iters=6, device=dev)
+        net = unroll.PDHGDTVNet(g, iters=iters, lam=(2e-3, 2e-3, 4e-3), norm=norm).to(dev)
+        net(y)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f = net(y)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+    err = float((f - truth).norm() / truth.norm())
+    log(f"pdhg/dtv 512: {iters} iterations in {t:.3f} s ({1e3 * t / iters:.1f} ms per "
+        f"iteration), rel error {err:.4f}")
+    return dict(iterations=iters, seconds=t, ms_per_iteration=1e3 * t / iters, rel_error=err,
+                geometry="512x512x384 @0.4mm, 120 views, 768x576 @0.6mm, SOD 700, SDD 1200")
+
+
+LEGS = ("beijing", "batched", "pdhg512")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +270,9 @@ def main():
     ap.add_argument("--no-beijing", action="store_true")
     ap.add_argument("--no-dense", action="store_true",
                     help="skip the dense N(0,1)-input leg (profiles of the headline pair alone)")
+    ap.add_argument("--leg", choices=LEGS, default=None,
+                    help="run only this secondary workload (--steps = its repetitions or "
+                         "iterations) and print its result as JSON, for profiling it alone")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
                     help="run one device copy of this size first (rocprof counter scale)")
     args = ap.parse_args()
@@ -260,6 +294,12 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     dev = torch.device("cuda", 0)
+    if args.leg is not None:
+        res = {"beijing": lambda: beijing_loop(max(1, args.steps), log),
+               "batched": lambda: batched_unroll(log),
+               "pdhg512": lambda: pdhg_512(max(1, args.steps), log)}[args.leg]()
+        print(json.dumps({"leg": args.leg, "result": res}), flush=True)
+        return
     _capi.lib()
     geom = configs.sparse_256()
     acct = roofline.pair_bytes(geom)
