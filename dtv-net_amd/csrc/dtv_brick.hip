@@ -312,6 +312,15 @@ struct RectMap {
   }
 };
 
+// (int64_t)v for an integral double |v| < 2^51: v + 1.5 * 2^52 lies in
+// [2^52, 2^53), where doubles are spaced by 1, so the sum is exact and its
+// encoding is that of 1.5 * 2^52 plus v (one FP64 add and one integer
+// subtract instead of the split conversion the compiler emits).
+__device__ __forceinline__ int64_t integral_to_i64(double v) {
+  const double t = v + 6755399441055744.0;
+  return __double_as_longlong(t) - 0x4338000000000000ll;
+}
+
 // Exact run [ka, kb) of the ray's samples whose floor index lies in
 // [flo, fhi], and the fixed-point position of sample ka relative to the
 // tile origin org (in units of 2^-32 voxel) with the fixed-point step.
@@ -332,9 +341,10 @@ __device__ __forceinline__ bool ray_run(const Geo& g, const View& v, int iu, int
   if (kb <= ka) return false;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    // integers below 2^53 throughout: exact
-    X[c] = (int64_t)(Pd[c] + (double)ka * Dd[c] - org[c]);
-    D[c] = (int64_t)Dd[c];
+    // integers below 2^53 throughout, so exact; the results are tile-relative
+    // (0 <= X < 2^38 inside the exact window) and |D| <= 2^32
+    X[c] = integral_to_i64(Pd[c] + (double)ka * Dd[c] - org[c]);
+    D[c] = integral_to_i64(Dd[c]);
   }
   return true;
 }
