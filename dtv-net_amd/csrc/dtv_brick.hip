@@ -48,8 +48,21 @@ namespace {
 // with a pixel of slack, the whole detector when a corner is not in front of
 // the source), its FP32 vectors for the pass-1 line test and, for A^T, the
 // bound on one voxel's weight sum used to pick the fixed-point scale.
+//
+// Before the chunks, one lane per view tests whether the box can reach the
+// view's detector at all (the same corner projections, with two pixels more
+// slack than the staging keeps); chunks are then formed from the views that
+// can, in increasing view order.  On scans with many narrow sub-detectors most
+// views never see a given brick, and they no longer cost a chunk slot.
 
 constexpr int kChunk = 32;
+constexpr int kCullWords = 128;  // views culled per brick: up to 32 * kCullWords
+
+// Culling pays only when it can remove chunks: scans of more than four chunks
+// (and at most 32 * kCullWords views) are culled, shorter ones are not.
+__device__ __forceinline__ bool culls_views(const Geo& g) {
+  return g.n_views > 4 * kChunk && g.n_views <= 32 * kCullWords;
+}
 
 struct ChunkStage {
   float fview[kChunk][12];  // S, D0, U, V in FP32
@@ -57,6 +70,10 @@ struct ChunkStage {
   float wbound[kChunk];     // A^T: bound on sum_k w for one voxel and view
   unsigned vmax[kChunk];    // A^T: largest |value| * scale over the rectangle (float bits)
   View sv[2];               // FP64 view of the current view (double-buffered)
+  int vidx[kChunk];         // view index of each chunk slot
+  unsigned live[kCullWords];  // views whose projected box can reach the detector
+  int livepre[kCullWords];    // live views in the words before each word
+  int n_live;                 // chunked views (all of them without culling)
 };
 
 // Bound on sum_k w over the samples of one view that touch one voxel c of the
@@ -87,14 +104,93 @@ __device__ __forceinline__ float weight_per_voxel_bound(bool all, const double d
   return 3.0f * fminf(pu, (float)nur + 1.f) * fminf(pv, (float)nvr + 1.f);
 }
 
-// Stages views [c0, c0+n) of the box [lo, hi) (voxel frame) into st.
+// Marks in st.live the views that the box [lo, hi) can reach (see above) and
+// sets st.livepre / st.n_live (only called when culls_views(g)).  A view is
+// dropped only when all 8 corners project in front of the
+// source and their bounding box, widened by 3 pixels, misses the detector, so
+// its staged rectangle (1 pixel of slack) would have been empty.  Ends with a
+// barrier.
+__device__ void cull_views(const Geo& g, const double lo[3], const double hi[3],
+                           ChunkStage& st) {
+  for (int w = threadIdx.x; w < kCullWords; w += blockDim.x) st.live[w] = 0u;
+  __syncthreads();
+  {
+    for (int a = threadIdx.x; a < g.n_views; a += blockDim.x) {
+      View v;
+      load_view(g, a, v);
+      ViewProj vp;
+      view_proj(v, vp);
+      bool keep = false;
+      double amin = 0.0, amax = 0.0, bmin = 0.0, bmax = 0.0;
+      for (int t = 0; t < 8; ++t) {
+        const double c[3] = {(t & 1) ? hi[0] : lo[0], (t & 2) ? hi[1] : lo[1],
+                             (t & 4) ? hi[2] : lo[2]};
+        double pa = 0.0, pb = 0.0;
+        if (!project_point(v, vp, c, pa, pb)) keep = true;
+        amin = t ? fmin(amin, pa) : pa;
+        amax = t ? fmax(amax, pa) : pa;
+        bmin = t ? fmin(bmin, pb) : pb;
+        bmax = t ? fmax(bmax, pb) : pb;
+      }
+      // comparisons with NaN are false: such a view is kept
+      const bool miss = ceil(amax) + 3.0 < 0.0 || floor(amin) - 3.0 > (double)(g.nu - 1) ||
+                        ceil(bmax) + 3.0 < 0.0 || floor(bmin) - 3.0 > (double)(g.nv - 1);
+      if (keep || !miss) atomicOr(&st.live[a >> 5], 1u << (a & 31));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    // exclusive prefix of the words' popcounts, two words per lane
+    const int l = threadIdx.x;
+    const int c0 = __popc(st.live[2 * l]), c1 = __popc(st.live[2 * l + 1]);
+    int incl = c0 + c1;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(incl, off, 64);
+      if (l >= off) incl += y;
+    }
+    st.livepre[2 * l] = incl - c0 - c1;
+    st.livepre[2 * l + 1] = incl - c1;
+    if (l == 63) st.n_live = incl;
+  }
+  __syncthreads();
+}
+
+// Index of the j-th chunked view: the j-th set bit of st.live, or j itself
+// without culling.
+__device__ __forceinline__ int chunk_view(const Geo& g, const ChunkStage& st, int j) {
+  if (!culls_views(g)) return j;
+  int w = 0;  // last word whose prefix is <= j
+#pragma unroll
+  for (int step = kCullWords / 2; step > 0; step >>= 1)
+    if (st.livepre[w + step] <= j) w += step;
+  unsigned m = st.live[w];
+  int r = j - st.livepre[w], pos = 0;
+#pragma unroll
+  for (int sh = 16; sh > 0; sh >>= 1) {
+    const unsigned low = m & ((1u << sh) - 1u);
+    const int c = __popc(low);
+    if (r >= c) {
+      r -= c;
+      m >>= sh;
+      pos += sh;
+    } else {
+      m = low;
+    }
+  }
+  return 32 * w + pos;
+}
+
+// Stages chunked views [c0, c0+n) of the box [lo, hi) (voxel frame) into st
+// (their view indices into st.vidx).
 __device__ void stage_chunk(const Geo& g, int c0, int n, const double lo[3], const double hi[3],
                             bool bound, ChunkStage& st) {
   const float ext[3] = {(float)(hi[0] - lo[0]), (float)(hi[1] - lo[1]), (float)(hi[2] - lo[2])};
   for (int pi = threadIdx.x; pi < n * 8; pi += blockDim.x) {
     const int vi = pi >> 3, t = pi & 7;
+    const int va = chunk_view(g, st, c0 + vi);
     View v;
-    load_view(g, c0 + vi, v);
+    load_view(g, va, v);
     ViewProj vp;
     view_proj(v, vp);
     const double c[3] = {(t & 1) ? hi[0] : lo[0], (t & 2) ? hi[1] : lo[1],
@@ -143,6 +239,7 @@ __device__ void stage_chunk(const Geo& g, int c0, int n, const double lo[3], con
       st.rect[vi][1] = u1;
       st.rect[vi][2] = v0;
       st.rect[vi][3] = v1;
+      st.vidx[vi] = va;
       // a corner behind the source or a depth ratio above 2 across the box
       // (the box next to the source) makes the bound loose enough to cost
       // fixed-point precision: +inf sends the view's group to FP32 atomics
@@ -402,14 +499,19 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
   const double org[3] = {ox * kFix, oy * kFix, oz * kFix};
   const float scale = (float)g.scale;
 
+  int nlive = g.n_views;
+  if (culls_views(g)) {
+    cull_views(g, lo, hi, st);
+    nlive = st.n_live;
+  }
   int vslot = 0;  // st.sv slot of the next processed view
-  for (int c0 = 0; c0 < g.n_views; c0 += kChunk) {
-    const int nc = min(kChunk, g.n_views - c0);
+  for (int c0 = 0; c0 < nlive; c0 += kChunk) {
+    const int nc = min(kChunk, nlive - c0);
     __syncthreads();  // the previous chunk's readers are done with st
     stage_chunk(g, c0, nc, lo, hi, false, st);
     __syncthreads();
     for (int vi = 0; vi < nc; ++vi) {
-      const int a = c0 + vi;
+      const int a = st.vidx[vi];
       RectMap rm;
       rm.init(st.rect[vi], false);
       if (rm.nur <= 0 || rm.nvr <= 0) continue;
@@ -653,9 +755,14 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
   const double org[3] = {(x0 - 1) * kFix, (y0 - 1) * kFix, (z0 - 1) * kFix};
   const float gscale = (float)g.scale;
 
+  int nlive = g.n_views;
+  if (culls_views(g)) {
+    cull_views(g, lo, hi, st);
+    nlive = st.n_live;
+  }
   int vslot = 0;  // st.sv slot of the next processed view
-  for (int c0 = 0; c0 < g.n_views; c0 += kChunk) {
-    const int nc = min(kChunk, g.n_views - c0);
+  for (int c0 = 0; c0 < nlive; c0 += kChunk) {
+    const int nc = min(kChunk, nlive - c0);
     __syncthreads();  // the previous chunk's readers are done with st
     stage_chunk(g, c0, nc, lo, hi, true, st);
     __syncthreads();
@@ -667,7 +774,7 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
         const int* rc = st.rect[vi];
         const int nur = rc[1] - rc[0] + 1, nvr = rc[3] - rc[2] + 1;
         if (nur <= 0 || nvr <= 0) continue;
-        const float* srow = sbase + (size_t)(c0 + vi) * g.nv * g.nu;
+        const float* srow = sbase + (size_t)st.vidx[vi] * g.nv * g.nu;
         const float rn = 1.f / (float)nur;
         const int n = nur * nvr;
         float m = 0.f;
@@ -731,7 +838,7 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
       for (int vi = g0; vi < g0 + ng; ++vi) {
         if (st.vmax[vi] == 0u) continue;  // empty rectangle or all-zero values
         any = true;
-        const int a = c0 + vi;
+        const int a = st.vidx[vi];
         RectMap rm;
         rm.init(st.rect[vi], true);
         stage_fp64_view(g, a, vslot, st);
