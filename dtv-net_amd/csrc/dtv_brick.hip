@@ -323,6 +323,12 @@ __device__ __forceinline__ void uni_view(const View& src, View& v) {
 // inside the box [blo, bhi], or -1 when the line misses it; fv = the view's
 // FP32 vectors.  The callers' boxes carry half a voxel of slack, far above
 // the rounding here.
+// Slack of the pass-1 box around a brick's sample box, in voxels.  The FP32
+// line test is off by well under 1e-3 voxel for any geometry whose source
+// lies within 1e4 voxels (rounded FP32 view vectors, one-ulp reciprocal); rays
+// that only graze the box are dropped by the exact window of pass 2 anyway.
+constexpr float kLineSlack = 0.0625f;
+
 __device__ __forceinline__ float chord_samples(const float* fv, int iu, int iv,
                                                const float blo[3], const float bhi[3]) {
   float tin = -3.0e38f, tout = 3.0e38f, rr = 0.f;
@@ -494,8 +500,8 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
   const int flo[3] = {ox, oy, oz}, fhi[3] = {ox + ex - 1, oy + ey - 1, oz + ez - 1};
   const double lo[3] = {(double)ox, (double)oy, (double)oz};
   const double hi[3] = {(double)(ox + ex), (double)(oy + ey), (double)(oz + ez)};
-  const float blo[3] = {ox - 0.5f, oy - 0.5f, oz - 0.5f};
-  const float bhi[3] = {ox + ex + 0.5f, oy + ey + 0.5f, oz + ez + 0.5f};
+  const float blo[3] = {ox - kLineSlack, oy - kLineSlack, oz - kLineSlack};
+  const float bhi[3] = {ox + ex + kLineSlack, oy + ey + kLineSlack, oz + ez + kLineSlack};
   const double org[3] = {ox * kFix, oy * kFix, oz * kFix};
   const float scale = (float)g.scale;
 
@@ -749,8 +755,8 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
   const int flo[3] = {x0 - 1, y0 - 1, z0 - 1}, fhi[3] = {x0 + ex - 1, y0 + ey - 1, z0 + ez - 1};
   const double lo[3] = {(double)(x0 - 1), (double)(y0 - 1), (double)(z0 - 1)};
   const double hi[3] = {(double)(x0 + ex), (double)(y0 + ey), (double)(z0 + ez)};
-  const float blo[3] = {x0 - 1.5f, y0 - 1.5f, z0 - 1.5f};
-  const float bhi[3] = {x0 + ex + 0.5f, y0 + ey + 0.5f, z0 + ez + 0.5f};
+  const float blo[3] = {x0 - 1 - kLineSlack, y0 - 1 - kLineSlack, z0 - 1 - kLineSlack};
+  const float bhi[3] = {x0 + ex + kLineSlack, y0 + ey + kLineSlack, z0 + ez + kLineSlack};
   // tile origin = the halo corner (x0-1, y0-1, z0-1)
   const double org[3] = {(x0 - 1) * kFix, (y0 - 1) * kFix, (z0 - 1) * kFix};
   const float gscale = (float)g.scale;
