@@ -44,15 +44,15 @@ namespace {
 // once, one (view, corner) pair per lane: each lane projects one corner of the
 // brick's sample box onto its view's detector, the 8 lanes of a view reduce
 // their corners with shuffles, and the first of them stores the view's
-// rectangle of candidate pixels (the bounding box of the projected corners
-// with a pixel of slack, the whole detector when a corner is not in front of
-// the source), its FP32 vectors for the pass-1 line test and, for A^T, the
+// rectangle of candidate pixels (the bounding box of the projected corners,
+// rounded outwards; the whole detector when a corner is not in front of the
+// source), its FP32 vectors for the pass-1 line test and, for A^T, the
 // bound on one voxel's weight sum used to pick the fixed-point scale.
 //
 // Before the chunks, one lane per view tests whether the box can reach the
-// view's detector at all (the same corner projections, with two pixels more
-// slack than the staging keeps); chunks are then formed from the views that
-// can, in increasing view order.  On scans with many narrow sub-detectors most
+// view's detector at all (the same corner projections, with three pixels more
+// slack than the staging's outward rounding); chunks are then formed from the
+// views that can, in increasing view order.  On scans with many narrow sub-detectors most
 // views never see a given brick, and they no longer cost a chunk slot.
 
 constexpr int kChunk = 32;
@@ -108,7 +108,7 @@ __device__ __forceinline__ float weight_per_voxel_bound(bool all, const double d
 // sets st.livepre / st.n_live (only called when culls_views(g)).  A view is
 // dropped only when all 8 corners project in front of the
 // source and their bounding box, widened by 3 pixels, misses the detector, so
-// its staged rectangle (1 pixel of slack) would have been empty.  Ends with a
+// its staged rectangle would have been empty.  Ends with a
 // barrier.
 __device__ void cull_views(const Geo& g, const double lo[3], const double hi[3],
                            ChunkStage& st) {
@@ -226,10 +226,12 @@ __device__ void stage_chunk(const Geo& g, int c0, int n, const double lo[3], con
     if (t == 0) {
       int u0 = 0, u1 = g.nu - 1, v0 = 0, v1 = g.nv - 1;
       if (ok) {
-        // pixel index i has its ray through coordinate i exactly; keep one
-        // pixel of slack on each side against rounding
-        const double fu0 = floor(amin) - 1.0, fu1 = ceil(amax) + 1.0;
-        const double fv0 = floor(bmin) - 1.0, fv1 = ceil(bmax) + 1.0;
+        // pixel index i has its ray through coordinate i exactly, so the rays
+        // that can meet the box are those with i in [amin, amax]; floor and
+        // ceil already give a pixel of slack against the FP64 projection's
+        // rounding (~1e-10 pixel)
+        const double fu0 = floor(amin), fu1 = ceil(amax);
+        const double fv0 = floor(bmin), fv1 = ceil(bmax);
         u0 = fu0 < 0.0 ? 0 : (fu0 > g.nu ? g.nu : (int)fu0);
         u1 = fu1 < -1.0 ? -1 : (fu1 > g.nu - 1 ? g.nu - 1 : (int)fu1);
         v0 = fv0 < 0.0 ? 0 : (fv0 > g.nv ? g.nv : (int)fv0);
