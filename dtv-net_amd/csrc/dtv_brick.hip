@@ -265,7 +265,7 @@ __device__ void stage_chunk(const Geo& g, int c0, int n, const double lo[3], con
 //
 // Per (brick, view) the rectangle of candidate rays is taken in segments of
 // kSchedR rays per lane.  Pass 1 tests each candidate's line against the
-// brick's sample box in FP32 with half a voxel of slack (no ray that has a
+// brick's sample box in FP32 with kLineSlack of slack (no ray that has a
 // sample in the box can fail it) and estimates its chord in samples; the
 // hits are counting-sorted by chord, longest first, into an LDS list.  Pass
 // 2 marches the list: each wave gets rays of similar length, and rays that
@@ -321,16 +321,16 @@ __device__ __forceinline__ void uni_view(const View& src, View& v) {
   }
 }
 
-// Length in samples (one per voxel of path) of the pixel (iu, iv)'s line
-// inside the box [blo, bhi], or -1 when the line misses it; fv = the view's
-// FP32 vectors.  The callers' boxes carry half a voxel of slack, far above
-// the rounding here.
 // Slack of the pass-1 box around a brick's sample box, in voxels.  The FP32
 // line test is off by well under 1e-3 voxel for any geometry whose source
 // lies within 1e4 voxels (rounded FP32 view vectors, one-ulp reciprocal); rays
 // that only graze the box are dropped by the exact window of pass 2 anyway.
 constexpr float kLineSlack = 0.0625f;
 
+// Length in samples (one per voxel of path) of the pixel (iu, iv)'s line
+// inside the box [blo, bhi], or -1 when the line misses it; fv = the view's
+// FP32 vectors.  The callers' boxes carry kLineSlack, far above the rounding
+// here.
 __device__ __forceinline__ float chord_samples(const float* fv, int iu, int iv,
                                                const float blo[3], const float bhi[3]) {
   float tin = -3.0e38f, tout = 3.0e38f, rr = 0.f;
