@@ -43,6 +43,31 @@ def test_multi_matches_single_instance_calls(cuda, persistent):
         assert torch.equal(outs[i], ref)   # integer accumulation: bitwise equal
 
 
+@pytest.mark.parametrize("persistent", [0, 37])
+def test_multi_with_a_culled_many_view_instance(cuda, persistent):
+    """A 147-view Beijing instance (more than four chunks: each brick culls the
+    views it cannot reach) beside a small circular one, in one launch and in
+    persistent workgroups that take several (instance, brick) items in turn:
+    equal to the single-instance calls."""
+    P = G.beijing_default_vectors(n_angles=7, planes=21, sub_detector=(8, 24), sid=70.0,
+                                  sdd=114.0)
+    geoms = [G.beijing(P, (40, 36, 16), (8, 24)),
+             G.circular(G.uniform_angles(11), (40, 40, 24), 1.0, (60, 30), 1.6, 200.0, 400.0)]
+    bricks = [((16, 16, 8), (16, 16, 8)), ((24, 24, 14), (32, 16, 16))]
+    _case("multi_culled", persistent=persistent, views=[g.n_views for g in geoms])
+    mp = batched.MultiProjector(geoms, bricks=bricks, block=256, persistent_blocks=persistent)
+    vols = [phantom.gaussian(g.volume_shape(1), 70 + i).to(cuda) for i, g in enumerate(geoms)]
+    sinos = mp.forward(vols)
+    for i, g in enumerate(geoms):
+        ref = ops.forward(vols[i], g, dict(algorithm=2, brick=bricks[i][0], block=256))
+        assert rel_l2(sinos[i], ref) <= 1e-6
+    ys = [phantom.gaussian(g.sino_shape(1), 80 + i).to(cuda) for i, g in enumerate(geoms)]
+    outs = mp.adjoint(ys)
+    for i, g in enumerate(geoms):
+        ref = ops.adjoint(ys[i], g, dict(algorithm=2, brick=bricks[i][1], block=256))
+        assert torch.equal(outs[i], ref)
+
+
 def test_batched_fista_tv_eager_graph_and_plain(cuda):
     geoms = _small_uneven()
     _case("batched_fista_tv")
