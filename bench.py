@@ -4,8 +4,8 @@ Workload (default): 256^3 FP32 volume at 0.5 mm, 60 views over 360 deg,
 384x384 detector at 0.8 mm, SOD 600 mm, SDD 1100 mm.  One step = forward
 projection of a seeded 20-ellipsoid phantom followed by the matched adjoint
 of the resulting sinogram, inputs resident in HBM.  Metric = detector rays
-per second of the pair (GRays/s) with the dominant kernel's fraction of the
-HBM roofline (algorithmic bytes / kernel time / 8 TB/s).
+per second of the pair (GRays/s), with the dominant kernel's roofline against
+the resource that binds it (vector-instruction issue or HBM bandwidth).
 
 Prints ONE JSON line.  ``--gpus N`` with N > 1 is refused: this build runs
 on one GPU.
@@ -254,6 +254,66 @@ def pdhg_512(iters, log):
                 geometry="512x512x384 @0.4mm, 120 views, 768x576 @0.6mm, SOD 700, SDD 1200")
 
 
+PMC_FILE = os.path.join(ROOT, "profiles", "pmc_sparse_256.json")
+
+
+def roofline_record(dom, dom_t, dom_b, acct, t_fwd, t_adj, med_fwd, med_adj, log):
+    """Roofline of the dominant kernel against the resource that binds it.
+
+    Per-launch counts come from the committed rocprofv3 PMC summary of this
+    workload (``profiles/pmc_sparse_256.json``): HBM bytes (FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for a copy kernel of
+    known size) and VALU wave-instructions (SQ_INSTS_VALU).  They are divided
+    by this run's event-timed launch duration.  Each resource's fraction is
+    against its physical peak: 8 TB/s HBM, and one wave64 VALU instruction per
+    compute unit per cycle (256 CUs at 2.4 GHz).  ``bound`` names the resource
+    with the larger fraction.  ``tap_gbs`` is the algorithmic figure (32 B per
+    in-support sample: the 8 trilinear taps), which the LDS-tiled kernels
+    serve from LDS, so it is not a bound and exceeds the HBM peak."""
+    from dtvproj import roofline
+    key = "k_adj_brick" if dom.startswith("k_adj") else "k_fwd_brick"
+    rec = {}
+    if os.path.exists(PMC_FILE):
+        with open(PMC_FILE) as f:
+            rec = json.load(f).get(key, {})
+    traffic = rec.get("traffic_bytes")
+    valu = rec.get("sq_insts_valu")
+    conflict = rec.get("sq_lds_bank_conflict")
+    cu_cycles = roofline.VALU_PEAK_WAVE_INSTS * dom_t
+    hbm = None
+    if traffic:
+        g = traffic / dom_t / 1e9
+        hbm = dict(achieved=g, peak=roofline.HBM_PEAK_GBS, unit="GB/s",
+                   frac=g / roofline.HBM_PEAK_GBS)
+    issue = None
+    if valu:
+        a = valu / dom_t / 1e9
+        issue = dict(achieved=a, peak=roofline.VALU_PEAK_WAVE_INSTS / 1e9,
+                     unit="G VALU wave64 instructions/s", frac=valu / cu_cycles)
+    cands = [(n, r) for n, r in (("valu", issue), ("hbm", hbm)) if r is not None]
+    if not cands:
+        log("no PMC summary for the dominant kernel: roofline fractions unavailable")
+        bound, best = "hbm", dict(achieved=None, peak=roofline.HBM_PEAK_GBS, unit="GB/s",
+                                  frac=None)
+    else:
+        bound, best = max(cands, key=lambda c: c[1]["frac"])
+    return dict(bound=bound, achieved=best["achieved"], peak=best["peak"], unit=best["unit"],
+                frac=best["frac"], traffic=traffic, kernel=dom,
+                kernel_template=rec.get("kernel"),
+                hbm=hbm, valu_issue=issue,
+                lds_bank_conflict_frac=conflict / cu_cycles if conflict else None,
+                l2_hit_rate=rec.get("l2_hit_rate"),
+                tap_gbs=dom_b / dom_t / 1e9,
+                # each array read or written once: volume + sinogram, FP32
+                compulsory_bytes=4 * (256 ** 3 + acct["rays"]),
+                forward_ms=t_fwd * 1e3, adjoint_ms=t_adj * 1e3,
+                forward_ms_median=med_fwd * 1e3, adjoint_ms_median=med_adj * 1e3,
+                pair_tap_gbs=acct["pair"] / (t_fwd + t_adj) / 1e9,
+                algorithmic_bytes=dict(forward=acct["forward"], adjoint=acct["adjoint"],
+                                       samples=acct["samples"], per_sample=32),
+                pmc_source="profiles/pmc_sparse_256.json")
+
+
 LEGS = ("beijing", "batched", "pdhg512")
 
 
@@ -350,33 +410,7 @@ def main():
         dom, dom_t, dom_b = "k_adj (matched A^T)", t_adj, acct["adjoint"]
     else:
         dom, dom_t, dom_b = "k_fwd (A)", t_fwd, acct["forward"]
-    achieved = dom_b / dom_t / 1e9
-    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by a copy kernel)
-    traffic = l2_hit = valu = None
-    tpath = os.path.join(ROOT, "profiles", "pmc_traffic_sparse_256.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            pmc = json.load(f)
-        key = "k_adj_brick" if dom.startswith("k_adj") else "k_fwd_brick"
-        traffic = pmc.get(key, {}).get("traffic_bytes")
-        l2_hit = pmc.get(key, {}).get("l2_hit_rate")
-        valu = pmc.get(key, {}).get("sq_insts_valu")
-    pair_gbs = acct["pair"] / (t_fwd + t_adj) / 1e9
-    roof = dict(bound="hbm", achieved=achieved, peak=roofline.HBM_PEAK_GBS, unit="GB/s",
-                frac=achieved / roofline.HBM_PEAK_GBS, traffic=traffic, kernel=dom,
-                hbm_gbs_measured=traffic / dom_t / 1e9 if traffic else None,
-                # each array read or written once: volume + sinogram, FP32
-                compulsory_bytes=4 * (256 ** 3 + acct["rays"]),
-                l2_hit_rate=l2_hit,
-                # what does bound it: vector-instruction issue (PMC SQ_INSTS_VALU
-                # per launch / this launch's time / the chip's issue rate)
-                valu_issue_frac=valu / dom_t / roofline.VALU_PEAK_WAVE_INSTS if valu else None,
-                forward_ms_median=med_fwd * 1e3, adjoint_ms_median=med_adj * 1e3,
-                pair_achieved=pair_gbs, pair_frac=pair_gbs / roofline.HBM_PEAK_GBS,
-                forward_ms=t_fwd * 1e3, adjoint_ms=t_adj * 1e3,
-                algorithmic_bytes=dict(forward=acct["forward"], adjoint=acct["adjoint"],
-                                       samples=acct["samples"]))
+    roof = roofline_record(dom, dom_t, dom_b, acct, t_fwd, t_adj, med_fwd, med_adj, log)
 
     dense = None if args.no_dense else dense_inputs(geom, s, acct, max(3, min(args.steps, 20)),
                                                     log)

@@ -76,34 +76,6 @@ struct ChunkStage {
   int n_live;                 // chunked views (all of them without culling)
 };
 
-// Bound on sum_k w over the samples of one view that touch one voxel c of the
-// box (w = the sample's trilinear weight on c):
-//  * per ray, the weights on c are bounded by a 1-D tent along the ray's
-//    dominant axis sampled at spacing |d_ax| >= 1/sqrt(3): sum <= 3;
-//  * rays touching c pass through its support cube (side 2), so their pixels
-//    lie in the cube's projected footprint.  Along u that footprint is at
-//    most sum_k (2 / e_k) du_k times the depth ratio across the box
-//    (perspective), +10%, where du_k is the largest projected length of the
-//    box's edges along axis k and e_k the box's extent along k (triangle
-//    inequality over the cube's three edge directions); likewise along v.  A
-//    footprint of f pixels holds <= f + 1 pixel centres per axis (+0.5 slack).
-// Depths are |(corner - S).n|, proportional to the distance from the source
-// along the detector normal.
-__device__ __forceinline__ float weight_per_voxel_bound(bool all, const double du[3],
-                                                        const double dv[3], double wmin,
-                                                        double wmax, const float ext[3],
-                                                        int nur, int nvr) {
-  if (!all || !(wmin > 0.0)) return 3.0f * (float)nur * (float)nvr;
-  const double persp = wmax / wmin * 1.1;
-  double fu = 0.0, fv = 0.0;
-  for (int k = 0; k < 3; ++k) {
-    fu += du[k] * 2.0 / ext[k];
-    fv += dv[k] * 2.0 / ext[k];
-  }
-  const float pu = (float)(fu * persp) + 1.5f, pv = (float)(fv * persp) + 1.5f;
-  return 3.0f * fminf(pu, (float)nur + 1.f) * fminf(pv, (float)nvr + 1.f);
-}
-
 // Marks in st.live the views that the box [lo, hi) can reach (see above) and
 // sets st.livepre / st.n_live (only called when culls_views(g)).  A view is
 // dropped only when all 8 corners project in front of the
@@ -185,7 +157,6 @@ __device__ __forceinline__ int chunk_view(const Geo& g, const ChunkStage& st, in
 // (their view indices into st.vidx).
 __device__ void stage_chunk(const Geo& g, int c0, int n, const double lo[3], const double hi[3],
                             bool bound, ChunkStage& st) {
-  const float ext[3] = {(float)(hi[0] - lo[0]), (float)(hi[1] - lo[1]), (float)(hi[2] - lo[2])};
   for (int pi = threadIdx.x; pi < n * 8; pi += blockDim.x) {
     const int vi = pi >> 3, t = pi & 7;
     const int va = chunk_view(g, st, c0 + vi);
@@ -196,60 +167,27 @@ __device__ void stage_chunk(const Geo& g, int c0, int n, const double lo[3], con
     const double c[3] = {(t & 1) ? hi[0] : lo[0], (t & 2) ? hi[1] : lo[1],
                          (t & 4) ? hi[2] : lo[2]};
     double a = 0.0, b = 0.0;
-    int ok = project_point(v, vp, c, a, b) ? 1 : 0;
+    const int ok = project_point(v, vp, c, a, b) ? 1 : 0;
     const double w[3] = {c[0] - v.S[0], c[1] - v.S[1], c[2] - v.S[2]};
     const double dw = fabs(dot3(w, vp.n));
-    double amin = a, amax = a, bmin = b, bmax = b, wmin = dw, wmax = dw;
-    // the 8 corners of a view are 8 consecutive lanes of one wave; corner
-    // t ^ (1 << k) is the other end of corner t's edge along axis k
-    double du[3], dv[3];
+    // the 8 corners of a view are 8 consecutive lanes of one wave (n * 8
+    // lanes take part, so every aligned group of 8 is whole): gather them
+    double ca[8], cb[8], cw[8];
+    int okall = 1;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      du[k] = fabs(a - __shfl_xor(a, 1 << k, 64));
-      dv[k] = fabs(b - __shfl_xor(b, 1 << k, 64));
-    }
-#pragma unroll
-    for (int off = 1; off < 8; off <<= 1) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        du[k] = fmax(du[k], __shfl_xor(du[k], off, 64));
-        dv[k] = fmax(dv[k], __shfl_xor(dv[k], off, 64));
-      }
-      amin = fmin(amin, __shfl_xor(amin, off, 64));
-      amax = fmax(amax, __shfl_xor(amax, off, 64));
-      bmin = fmin(bmin, __shfl_xor(bmin, off, 64));
-      bmax = fmax(bmax, __shfl_xor(bmax, off, 64));
-      wmin = fmin(wmin, __shfl_xor(wmin, off, 64));
-      wmax = fmax(wmax, __shfl_xor(wmax, off, 64));
-      ok &= __shfl_xor(ok, off, 64);
+    for (int s = 0; s < 8; ++s) {
+      ca[s] = __shfl(a, s, 8);
+      cb[s] = __shfl(b, s, 8);
+      cw[s] = __shfl(dw, s, 8);
+      okall &= __shfl(ok, s, 8);
     }
     if (t == 0) {
-      int u0 = 0, u1 = g.nu - 1, v0 = 0, v1 = g.nv - 1;
-      if (ok) {
-        // pixel index i has its ray through coordinate i exactly, so the rays
-        // that can meet the box are those with i in [amin, amax]; floor and
-        // ceil already give a pixel of slack against the FP64 projection's
-        // rounding (~1e-10 pixel)
-        const double fu0 = floor(amin), fu1 = ceil(amax);
-        const double fv0 = floor(bmin), fv1 = ceil(bmax);
-        u0 = fu0 < 0.0 ? 0 : (fu0 > g.nu ? g.nu : (int)fu0);
-        u1 = fu1 < -1.0 ? -1 : (fu1 > g.nu - 1 ? g.nu - 1 : (int)fu1);
-        v0 = fv0 < 0.0 ? 0 : (fv0 > g.nv ? g.nv : (int)fv0);
-        v1 = fv1 < -1.0 ? -1 : (fv1 > g.nv - 1 ? g.nv - 1 : (int)fv1);
-      }
-      st.rect[vi][0] = u0;
-      st.rect[vi][1] = u1;
-      st.rect[vi][2] = v0;
-      st.rect[vi][3] = v1;
+      int rect[4];
+      float wb;
+      box_view_stage(ca, cb, cw, okall != 0, lo, hi, g.nu, g.nv, rect, wb);
+      for (int k = 0; k < 4; ++k) st.rect[vi][k] = rect[k];
       st.vidx[vi] = va;
-      // a corner behind the source or a depth ratio above 2 across the box
-      // (the box next to the source) makes the bound loose enough to cost
-      // fixed-point precision: +inf sends the view's group to FP32 atomics
-      if (bound)
-        st.wbound[vi] = (ok && wmin > 0.0 && wmax <= 2.0 * wmin)
-                            ? weight_per_voxel_bound(true, du, dv, wmin, wmax, ext,
-                                                     u1 - u0 + 1, v1 - v0 + 1)
-                            : __int_as_float(0x7f800000);
+      if (bound) st.wbound[vi] = wb;
       st.vmax[vi] = 0u;
       for (int k = 0; k < 3; ++k) {
         st.fview[vi][k] = (float)v.S[k];
@@ -1113,6 +1051,10 @@ int launch_multi(bool adjoint, const dtv_instance* ih, const dtv_instance* id, i
   if (rc) return rc;
   if (!id || !wd) return DTV_ERR_NULL_POINTER;
   if (!adjoint) {
+    for (int i = 0; i < n_inst; ++i) {
+      rc = check_atomic_target(ih[i].out);
+      if (rc) return rc;
+    }
     size_t max_out = 0;
     for (int i = 0; i < n_inst; ++i) {
       const size_t no = (size_t)ih[i].n_views * ih[i].nu * ih[i].nv;

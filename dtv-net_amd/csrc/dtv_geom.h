@@ -332,4 +332,83 @@ DTV_HD inline bool project_point(const View& v, const ViewProj& p, const double 
   return true;
 }
 
+// Bound on sum_k w over the samples of one view that touch one voxel c of a
+// box (w = the sample's trilinear weight on c):
+//  * per ray, the weights on c are bounded by a 1-D tent along the ray's
+//    dominant axis sampled at spacing |d_ax| >= 1/sqrt(3): sum <= 3;
+//  * rays touching c pass through its support cube (side 2), so their pixels
+//    lie in the cube's projected footprint.  Along u that footprint is at
+//    most sum_k (2 / e_k) du_k times the depth ratio across the box
+//    (perspective), +10%, where du_k is the largest projected length of the
+//    box's edges along axis k and e_k the box's extent along k (triangle
+//    inequality over the cube's three edge directions); likewise along v.  A
+//    footprint of f pixels holds <= f + 1 pixel centres per axis (+0.5 slack).
+// Depths are |(corner - S).n|, proportional to the distance from the source
+// along the detector normal.  nur x nvr is the view's candidate rectangle.
+DTV_HD inline float weight_per_voxel_bound(bool all, const double du[3], const double dv[3],
+                                           double wmin, double wmax, const float ext[3],
+                                           int nur, int nvr) {
+  if (!all || !(wmin > 0.0)) return 3.0f * (float)nur * (float)nvr;
+  const double persp = wmax / wmin * 1.1;
+  double fu = 0.0, fv = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    fu += du[k] * 2.0 / ext[k];
+    fv += dv[k] * 2.0 / ext[k];
+  }
+  const float pu = (float)(fu * persp) + 1.5f, pv = (float)(fv * persp) + 1.5f;
+  return 3.0f * fminf(pu, (float)nur + 1.f) * fminf(pv, (float)nvr + 1.f);
+}
+
+// What the brick kernels stage per (box, view), from the projections of the
+// box [lo, hi]'s 8 corners (corner t at (t&1 ? hi : lo, t&2 ? hi : lo,
+// t&4 ? hi : lo) per axis): ca[t], cb[t] its continuous pixel coordinates
+// (project_point), cw[t] = |(corner - S).n| its depth, ok = every corner
+// strictly in front of the source.
+//  * rect = {u0, u1, v0, v1} (inclusive, u1 < u0 when empty): the pixels i
+//    in [floor(amin), ceil(amax)] (likewise v), clipped to the detector --
+//    pixel i's ray passes through coordinate i exactly, so these are the
+//    rays that can meet the box, and floor/ceil give a pixel of slack
+//    against the FP64 projection's rounding (~1e-10 pixel); the whole
+//    detector when some corner is not in front of the source.
+//  * wbound: weight_per_voxel_bound, or +inf when a corner is behind the
+//    source or the depth ratio across the box exceeds 2 (the box next to the
+//    source, where the bound would be loose): the matched adjoint then
+//    accumulates that view in FP32 instead of fixed point.
+DTV_HD inline void box_view_stage(const double ca[8], const double cb[8], const double cw[8],
+                                  bool ok, const double lo[3], const double hi[3], int nu,
+                                  int nv, int rect[4], float& wbound) {
+  double amin = ca[0], amax = ca[0], bmin = cb[0], bmax = cb[0], wmin = cw[0], wmax = cw[0];
+  double du[3] = {0.0, 0.0, 0.0}, dv[3] = {0.0, 0.0, 0.0};
+  for (int t = 0; t < 8; ++t) {
+    amin = fmin(amin, ca[t]);
+    amax = fmax(amax, ca[t]);
+    bmin = fmin(bmin, cb[t]);
+    bmax = fmax(bmax, cb[t]);
+    wmin = fmin(wmin, cw[t]);
+    wmax = fmax(wmax, cw[t]);
+    // corner t ^ (1 << k) is the other end of corner t's edge along axis k
+    for (int k = 0; k < 3; ++k) {
+      du[k] = fmax(du[k], fabs(ca[t] - ca[t ^ (1 << k)]));
+      dv[k] = fmax(dv[k], fabs(cb[t] - cb[t ^ (1 << k)]));
+    }
+  }
+  int u0 = 0, u1 = nu - 1, v0 = 0, v1 = nv - 1;
+  if (ok) {
+    const double fu0 = floor(amin), fu1 = ceil(amax);
+    const double fv0 = floor(bmin), fv1 = ceil(bmax);
+    u0 = fu0 < 0.0 ? 0 : (fu0 > nu ? nu : (int)fu0);
+    u1 = fu1 < -1.0 ? -1 : (fu1 > nu - 1 ? nu - 1 : (int)fu1);
+    v0 = fv0 < 0.0 ? 0 : (fv0 > nv ? nv : (int)fv0);
+    v1 = fv1 < -1.0 ? -1 : (fv1 > nv - 1 ? nv - 1 : (int)fv1);
+  }
+  rect[0] = u0;
+  rect[1] = u1;
+  rect[2] = v0;
+  rect[3] = v1;
+  const float ext[3] = {(float)(hi[0] - lo[0]), (float)(hi[1] - lo[1]), (float)(hi[2] - lo[2])};
+  wbound = (ok && wmin > 0.0 && wmax <= 2.0 * wmin)
+               ? weight_per_voxel_bound(true, du, dv, wmin, wmax, ext, u1 - u0 + 1, v1 - v0 + 1)
+               : INFINITY;
+}
+
 }  // namespace dtv

@@ -25,6 +25,11 @@ inline size_t adj_tile_bytes(int bx, int by, int bz) {
 }
 constexpr size_t kMaxTileBytes = 56 * 1024;
 
+// DTV_OK when p is coarse-grained device memory (hipMalloc), which the
+// kernels' hardware FP32 global atomics need; DTV_ERR_INVALID_ARGUMENT for
+// managed, host or fine-grained memory and pointers HIP does not know.
+int check_atomic_target(const void* p);
+
 int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int bx, int by,
                      int bz, int block, hipStream_t st);
 int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int bx, int by,

@@ -504,7 +504,9 @@ int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipS
 int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st,
                 const Launch& L) {
   if (L.algo == DTV_ALGO_BRICK) {
-    int rc = run_zero(sino, (size_t)batch * g.n_views * g.nu * g.nv, st);
+    int rc = check_atomic_target(sino);
+    if (rc) return rc;
+    rc = run_zero(sino, (size_t)batch * g.n_views * g.nu * g.nv, st);
     if (rc) return rc;
     return launch_fwd_brick(vol, sino, batch, g, L.bx, L.by, L.bz, L.block, st);
   }
@@ -523,7 +525,9 @@ int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipS
                 const Launch& L) {
   if (L.algo == DTV_ALGO_BRICK)
     return launch_adj_brick(sino, vol, batch, g, L.bx, L.by, L.bz, L.block, st);
-  int rc = run_zero(vol, (size_t)batch * g.nx * g.ny * g.nz, st);
+  int rc = check_atomic_target(vol);
+  if (rc) return rc;
+  rc = run_zero(vol, (size_t)batch * g.nx * g.ny * g.nz, st);
   if (rc) return rc;
   const dim3 block(kRayTileU, kRayTileV);
   const dim3 grid((g.nu + kRayTileU - 1) / kRayTileU, (g.nv + kRayTileV - 1) / kRayTileV,
@@ -798,3 +802,25 @@ const char* dtv_strerror(int code) {
 const char* dtv_version(void) { return "dtvproj 0.1.0 gfx950"; }
 
 }  // extern "C"
+
+namespace dtv {
+
+// Outputs that kernels accumulate into with hardware FP32 global atomics (A
+// on bricks, the ray-driven A^T, the multi-instance A) must be coarse-grained
+// device memory, as hipMalloc returns: on managed, host or fine-grained
+// memory the hardware atomic add is not performed and sums come out wrong
+// without any error.  Anything else, or a pointer HIP does not know, is
+// rejected.
+int check_atomic_target(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return DTV_ERR_INVALID_ARGUMENT;
+  }
+  // hipExtMallocWithFlags: 1 = fine-grained, 2 = signal memory, 3 = uncached
+  if (a.type != hipMemoryTypeDevice || a.isManaged || (a.allocationFlags & 3u) != 0)
+    return DTV_ERR_INVALID_ARGUMENT;
+  return DTV_OK;
+}
+
+}  // namespace dtv

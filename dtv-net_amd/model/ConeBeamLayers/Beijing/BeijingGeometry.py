@@ -12,11 +12,13 @@ one angle consecutive; the reference's public layout interleaves them as
 [angles, V, planes, U] (the permutes at BeijingGeometry.py:18, 23, 31, 39),
 which is kept here for any batch size (the reference fixes batch 1).
 
-``adjoint_mode``: ``"matched"`` (default) uses the exact transpose of
-``sampleInterval * A`` as A^T, so autograd gradients are exact;
-``"reference"`` uses the reference's backprojector and its factor placement
-(``sampleInterval`` in ForwardProjection.forward and BackProjection.backward
-only, BeijingGeometry.py:17, 22-25, 38).
+``adjoint_mode``: ``"reference"`` (default) uses the reference's
+backprojector and its factor placement (``sampleInterval`` in
+ForwardProjection.forward and BackProjection.backward only,
+BeijingGeometry.py:17, 22-25, 38), so reference callers keep their tuned
+step sizes (``lamb = 10e-3``); ``"matched"`` uses the exact transpose of
+``sampleInterval * A`` as A^T, so autograd gradients are exact (a reference
+step size then needs rescaling to the scale of A^T A).
 """
 import os
 
@@ -51,7 +53,7 @@ def _load_parameters():
 parameters = _load_parameters()
 volumeSize = torch.IntTensor(beijingVolumeSize)
 detectorSize = torch.IntTensor(beijingSubDetectorSize)
-adjoint_mode = "matched"
+adjoint_mode = "reference"
 
 
 def _args(device):

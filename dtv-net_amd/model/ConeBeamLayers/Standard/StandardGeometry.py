@@ -8,11 +8,16 @@ at call time, so callers may reassign them as the reference's drivers do.
 
 ``adjoint_mode`` selects the operator used as A^T:
 
-* ``"matched"`` (default) -- the exact transpose of A, so the backward of
-  ``ForwardProjection`` is the true gradient and vice versa;
-* ``"reference"`` -- the reference's FDK-weighted voxel-driven
-  backprojector (backwardKernel.cu:22-53), reproducing its iterate
-  trajectories (its step sizes, e.g. ``lamb = 1.0``, were tuned for it).
+* ``"reference"`` (default) -- the reference's FDK-weighted voxel-driven
+  backprojector (backwardKernel.cu:22-53), as the reference wires it
+  (StandardGeometry.py:17-21, 30-34), so callers written for the reference
+  keep their iterate trajectories and hand-tuned step sizes (``lamb = 1.0``
+  here; ASD-POCS ``beta``, DTV-CP ``tau``/``sigma``);
+* ``"matched"`` -- the exact transpose of A, so the backward of
+  ``ForwardProjection`` is the true gradient and vice versa.  Its scale is
+  that of A^T A (about ``unroll.power_norm`` of the geometry), not of the
+  FDK backprojector, so a reference step size must be rescaled, e.g.
+  ``lamb = 1 / power_norm``.
 """
 import numpy as np
 import torch
@@ -25,7 +30,7 @@ from options import standardAngleNum, standardDetectorSize, standardVolumeSize, 
 angles = torch.from_numpy(np.linspace(0, 360, standardAngleNum, False))
 volumeSize = torch.tensor(standardVolumeSize)
 detectorSize = torch.tensor(standardDetectorSize)
-adjoint_mode = "matched"
+adjoint_mode = "reference"
 
 
 def _args(device):

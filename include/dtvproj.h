@@ -5,7 +5,11 @@
  *   volume   [batch][Z][Y][X]        (x fastest; the reference's [B,1,Z,Y,X])
  *   sinogram [batch][views][V][U]    (u fastest; the reference's [B,1,A,V,U])
  * Outputs are caller-allocated and fully overwritten; the library keeps no
- * pointer after a call returns and allocates no device memory.  Work is
+ * pointer after a call returns and allocates no device memory.  Outputs
+ * must be coarse-grained device memory (hipMalloc, as torch allocates):
+ * some kernels add into them with hardware FP32 atomics, which are not
+ * performed on managed, host or fine-grained memory, so the projector entry
+ * points reject such outputs with DTV_ERR_INVALID_ARGUMENT.  Work is
  * enqueued on ``stream`` (a hipStream_t; NULL = the null stream) and runs
  * on the device that is current on the calling thread.  Every entry point
  * returns 0 on success or a DTV_ERR_* code; dtv_strerror() describes it.

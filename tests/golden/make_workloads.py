@@ -1,0 +1,128 @@
+"""Writes tests/golden/oracle_workloads.npz: FP64-oracle outputs at two
+full-size workloads, sampled at seeded voxels so the file stays small.
+
+* ``s256_*`` -- the matched adjoint A^T over all 60 views of the sparse-view
+  workload (256^3 at 0.5 mm, 60 views, 384x384 detector at 0.8 mm, SOD 600,
+  SDD 1100) applied to the i.i.d. N(0,1) sinogram of seed 21
+  (``dtvproj.phantom.gaussian``, float32), evaluated by
+  ``oracle.cone.circ_adjoint_at`` at 4096 seeded voxels (3584 uniform, 512
+  on the faces of the volume).  The sinogram itself (35 MB) is not stored:
+  its FP64 sum and sum of squares are, so a test can prove it regenerated
+  the same input.
+* ``u16_*`` -- 20 FISTA-TV iterations (smoothed TV, Neumann differences,
+  lam = eps = 1e-3, non-negativity, FISTA momentum of FISTA.py:23-25) on the
+  smallest-work instance of the 16-instance uneven workload (103x103x75 at
+  1 mm, 38 views, 153x91 detector, SOD 500 / SDD 1000), run entirely in FP64
+  on the oracle's circ_forward / circ_adjoint and oracle.tv.tv_eps_grad,
+  from the sinogram of its seeded 10-ellipsoid phantom (seed 102).  The step
+  is 1 / (L + 12 lam / eps) with L from 12 FP64 power iterations of A^T A.
+  Stored: the step, the iterate at 32768 seeded voxels, its peak, sum and
+  norm.
+
+Takes several minutes on 8 cores.  Run from the repository root:
+    python tests/golden/make_workloads.py
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "dtv-net_amd"))
+
+from oracle import cone, tv  # noqa: E402
+from dtvproj import phantom  # noqa: E402
+
+OUT = os.path.join(ROOT, "tests", "golden", "oracle_workloads.npz")
+
+S256 = dict(vol=(256, 256, 256), voxel=0.5, det=(384, 384), pitch=0.8, n_views=60,
+            sod=600.0, sdd=1100.0, sino_seed=21, n_uniform=3584, n_face=512, vox_seed=2021)
+U16 = dict(vol=(103, 103, 75), voxel=1.0, det=(153, 91), pitch=2.0, n_views=38, sod=500.0,
+           sdd=1000.0, phantom_seed=102, n_ellipsoids=10, lam=1e-3, eps=1e-3, iters=20,
+           power_iters=12, n_vox=32768, vox_seed=2022)
+
+
+def angles(n):
+    return np.linspace(0.0, 360.0, n, endpoint=False)
+
+
+def s256_voxels():
+    X, Y, Z = S256["vol"]
+    rng = np.random.default_rng(S256["vox_seed"])
+    N = np.array([X, Y, Z])
+    uni = rng.integers(0, N, size=(S256["n_uniform"], 3))
+    face = rng.integers(0, N, size=(S256["n_face"], 3))
+    ax = rng.integers(0, 3, size=S256["n_face"])
+    hi = rng.integers(0, 2, size=S256["n_face"])
+    face[np.arange(S256["n_face"]), ax] = np.where(hi == 1, N[ax] - 1, 0)
+    return np.concatenate([uni, face]).astype(np.int32)
+
+
+def u16_voxels():
+    X, Y, Z = U16["vol"]
+    rng = np.random.default_rng(U16["vox_seed"])
+    return np.sort(rng.choice(X * Y * Z, size=U16["n_vox"], replace=False)).astype(np.int64)
+
+
+def build_s256():
+    c = S256
+    X, Y, Z = c["vol"]
+    U, V = c["det"]
+    y = phantom.gaussian((1, 1, c["n_views"], V, U), c["sino_seed"]).double()
+    vox = s256_voxels()
+    out = cone.circ_adjoint_at(y, angles(c["n_views"]), c["vol"], c["voxel"], c["det"],
+                               c["pitch"], c["sod"], c["sdd"], vox)
+    return dict(s256_voxels=vox, s256_adjoint=out[0].numpy(),
+                s256_sino_sum=np.float64(y.sum()), s256_sino_sumsq=np.float64((y * y).sum()))
+
+
+def build_u16(log=print):
+    c = U16
+    args = (angles(c["n_views"]), c["vol"], c["voxel"], c["det"], c["pitch"], c["sod"], c["sdd"])
+    A = lambda x: cone.circ_forward(x, *args)          # noqa: E731
+    AT = lambda s: cone.circ_adjoint(s, *args)         # noqa: E731
+    ph = phantom.phantom(c["vol"], c["voxel"], c["n_ellipsoids"], seed=c["phantom_seed"])
+    b = A(ph.double())
+    # power iteration for L = ||A^T A|| from a seeded start
+    v = phantom.gaussian(ph.shape, 7, dtype=torch.float64)
+    v = v / v.norm()
+    lip = 0.0
+    for k in range(c["power_iters"]):
+        w = AT(A(v))
+        lip = float(w.norm())
+        v = w / lip
+        log(f"power iteration {k}: {lip:.6e}")
+    lam, eps = c["lam"], c["eps"]
+    step = 1.0 / (lip + lam * 12.0 / eps)
+    x = torch.zeros(ph.shape, dtype=torch.float64)
+    y = x.clone()
+    t = 1.0
+    for k in range(c["iters"]):
+        tn = (1.0 + math.sqrt(1.0 + 4.0 * t * t)) / 2.0       # FISTA.py:23-25
+        beta = (t - 1.0) / tn
+        t = tn
+        g = AT(A(y) - b).numpy()
+        gtv = tv.tv_eps_grad(y.numpy()[0, 0], eps, c["vol"], "neumann")[None, None]
+        xn = np.maximum(y.numpy() - step * (g + lam * gtv), 0.0)
+        xn = torch.from_numpy(xn)
+        y = xn + beta * (xn - x)
+        x = xn
+        log(f"fista iteration {k}: |x| = {float(x.norm()):.6e}")
+    flat = x.reshape(-1)
+    idx = u16_voxels()
+    return dict(u16_step=np.float64(step), u16_lip=np.float64(lip), u16_voxels=idx,
+                u16_values=flat[torch.from_numpy(idx)].numpy(),
+                u16_peak=np.float64(flat.abs().max()), u16_sum=np.float64(flat.sum()),
+                u16_norm=np.float64(flat.norm()))
+
+
+if __name__ == "__main__":
+    torch.set_num_threads(os.cpu_count() or 1)
+    fx = build_s256()
+    print(f"s256: {fx['s256_adjoint'].shape[0]} voxels", flush=True)
+    fx.update(build_u16(lambda s: print(s, flush=True)))
+    np.savez_compressed(OUT, **fx)
+    print(f"wrote {OUT}: {len(fx)} arrays, {os.path.getsize(OUT)} bytes")

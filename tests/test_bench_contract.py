@@ -48,8 +48,16 @@ def test_bench_prints_one_json_line_with_the_contract_fields(cuda):
     roof = line["roofline"]
     for k in ("bound", "achieved", "peak", "unit", "frac", "traffic"):
         assert k in roof, k
-    assert roof["bound"] == "hbm" and roof["unit"] == "GB/s" and roof["peak"] == 8000.0
+    # bound = the binding resource against its physical peak, never above it
+    assert roof["bound"] in ("valu", "hbm")
+    assert 0 < roof["frac"] <= 1
     assert abs(roof["frac"] - roof["achieved"] / roof["peak"]) < 1e-9
+    assert roof[{"valu": "valu_issue", "hbm": "hbm"}[roof["bound"]]]["frac"] == roof["frac"]
+    assert roof["hbm"]["peak"] == 8000.0 and 0 < roof["hbm"]["frac"] <= 1
+    assert roof["valu_issue"]["frac"] <= 1
+    assert roof["frac"] >= max(roof["hbm"]["frac"], roof["valu_issue"]["frac"])
+    # the algorithmic (LDS-served) tap figure is reported, not used as a bound
+    assert roof["tap_gbs"] > 0 and roof["algorithmic_bytes"]["per_sample"] == 32
     # value = rays per pair / seconds per pair
     assert abs(line["value"] - 8847360 / (line["ms_per_step"] * 1e-3) / 1e9) \
         < 1e-6 * line["value"]

@@ -136,3 +136,37 @@ def test_beijing_backprojector_skipped_views(cuda, sid):
     got = ops.bp_ref(y.to(cuda), g)
     want = cone.bj_bp_ref(y, P, vs, ds)
     assert rel_l2(got, want) <= TOL
+
+
+def test_outputs_accumulated_with_atomics_must_be_device_memory(cuda):
+    """The brick A and the ray-driven A^T add into their outputs with
+    hardware FP32 atomics, which are not performed on host or managed
+    memory: such an output is rejected with DTV_ERR_INVALID_ARGUMENT
+    instead of giving silently wrong sums."""
+    import ctypes
+    from dtvproj import _capi, configs, ops
+    g = configs.fd_32()
+    print("[case] atomic outputs must be device memory", flush=True)
+    L = _capi.lib()
+    vol = torch.zeros(g.volume_shape(1), device=cuda)
+    sino = torch.zeros(g.sino_shape(1), device=cuda)
+    views = g.device_views(cuda)
+    sc = _capi.scheme_struct(g.scheme)
+    host_sino = torch.zeros(g.sino_shape(1)).pin_memory()
+    host_vol = torch.zeros(g.volume_shape(1)).pin_memory()
+    brick = _capi.launch_struct(2, (16, 16, 8), 256)
+    ray = _capi.launch_struct(1)
+    args = (1, _capi.int3(g.volume_size), _capi.int2(g.detector_size), views.data_ptr(),
+            g.n_views, ctypes.byref(sc))
+    st = ops._stream(cuda)
+    assert L.dtv_cone_forward_ex(vol.data_ptr(), sino.data_ptr(), *args, ctypes.byref(brick),
+                                 st) == 0
+    assert L.dtv_cone_forward_ex(vol.data_ptr(), host_sino.data_ptr(), *args,
+                                 ctypes.byref(brick), st) == 1
+    assert L.dtv_cone_adjoint_ex(sino.data_ptr(), vol.data_ptr(), *args, ctypes.byref(ray),
+                                 st) == 0
+    assert L.dtv_cone_adjoint_ex(sino.data_ptr(), host_vol.data_ptr(), *args,
+                                 ctypes.byref(ray), st) == 1
+    torch.cuda.synchronize()
+    # the rejection leaves no error behind for the next call
+    assert torch.isfinite(ops.forward(vol, g)).all()

@@ -94,9 +94,11 @@ def test_standard_fbp_module(cuda, std_small, monkeypatch, mode):
     assert rel_l2(out, back) <= TOL
 
 
-def test_standard_step_module(cuda, std_small):
+@pytest.mark.parametrize("mode", ["matched", "reference"])
+def test_standard_step_module(cuda, std_small, monkeypatch, mode):
     S = std_small
-    print("[case] standard step module", flush=True)
+    monkeypatch.setattr(S, "adjoint_mode", mode)
+    print(f"[case] standard step module mode={mode}", flush=True)
     X, Y, Z = SMALL_STD["vol"]
     U, V = SMALL_STD["det"]
     n = SMALL_STD["n_views"]
@@ -105,8 +107,76 @@ def test_standard_step_module(cuda, std_small):
     p = torch.randn((2, 1, n, V, U), generator=g)
     out = S.StandardGeometry().to(cuda)(x.to(cuda), p.to(cuda))
     args = (S.angles.numpy(), SMALL_STD["vol"], SMALL_STD["det"], S.standardSID, S.standardSDD)
-    ref = x.double() - cone.std_adjoint(cone.std_forward(x, *args) - p.double(), *args)
+    back = cone.std_adjoint if mode == "matched" else cone.std_bp_ref
+    ref = x.double() - back(cone.std_forward(x, *args) - p.double(), *args)
     assert rel_l2(out, ref) <= TOL
+
+
+@pytest.mark.parametrize("mode", ["matched", "reference"])
+def test_standard_functions_backward(cuda, std_small, monkeypatch, mode):
+    """autograd through the drop-in Functions, batch 2: the backward of
+    ForwardProjection is A^T of the mode (the reference's FDK backprojector,
+    StandardGeometry.py:17-21, or the matched transpose), the backward of
+    BackProjection is A (:30-34)."""
+    S = std_small
+    monkeypatch.setattr(S, "adjoint_mode", mode)
+    print(f"[case] standard functions backward mode={mode}", flush=True)
+    X, Y, Z = SMALL_STD["vol"]
+    U, V = SMALL_STD["det"]
+    n = SMALL_STD["n_views"]
+    g = torch.Generator().manual_seed(16)
+    x = torch.randn((2, 1, Z, Y, X), generator=g)
+    w = torch.randn((2, 1, n, V, U), generator=g)
+    c = torch.randn((2, 1, Z, Y, X), generator=g)
+    args = (S.angles.numpy(), SMALL_STD["vol"], SMALL_STD["det"], S.standardSID, S.standardSDD)
+    back = cone.std_adjoint if mode == "matched" else cone.std_bp_ref
+    xg = x.to(cuda).requires_grad_(True)
+    out = S.ForwardProjection.apply(xg)
+    assert rel_l2(out, cone.std_forward(x, *args)) <= TOL
+    (gx,) = torch.autograd.grad((out * w.to(cuda)).sum(), xg)
+    assert rel_l2(gx, back(w.double(), *args)) <= TOL
+    wg = w.to(cuda).requires_grad_(True)
+    vol = S.BackProjection.apply(wg)
+    assert rel_l2(vol, back(w.double(), *args)) <= TOL
+    vol.backward(c.to(cuda))
+    assert rel_l2(wg.grad, cone.std_forward(c.double(), *args)) <= TOL
+
+
+def test_standard_reference_caller_step_sizes(cuda, std_small, monkeypatch):
+    """A reference caller iterating StandardGeometry (x <- x - lamb B(Ax - p),
+    lamb = 1.0 as tuned for the FDK backprojector, StandardGeometry.py:36-43):
+    in the default (reference) mode five iterates meet the same iteration on
+    the oracle's A and B_ref; in matched mode the same step would diverge
+    (||A^T A|| >> 2), and the documented rescaling lamb = 1 / ||A^T A||
+    makes the iteration reduce the residual."""
+    from dtvproj import geometry as G, unroll
+    S = std_small
+    print("[case] standard reference caller", flush=True)
+    X, Y, Z = SMALL_STD["vol"]
+    U, V = SMALL_STD["det"]
+    n = SMALL_STD["n_views"]
+    args = (S.angles.numpy(), SMALL_STD["vol"], SMALL_STD["det"], S.standardSID, S.standardSDD)
+    x_true = torch.randn((1, 1, Z, Y, X), generator=torch.Generator().manual_seed(17)).abs()
+    p64 = cone.std_forward(x_true.double(), *args)
+    p = p64.float().to(cuda)
+    m = S.StandardGeometry().to(cuda)
+    x = torch.zeros_like(x_true).to(cuda)
+    x64 = torch.zeros_like(x_true, dtype=torch.float64)
+    for _ in range(5):
+        x = m(x, p)
+        x64 = x64 - 1.0 * cone.std_bp_ref(cone.std_forward(x64, *args) - p64, *args)
+        assert rel_l2(x, x64) <= TOL
+    monkeypatch.setattr(S, "adjoint_mode", "matched")
+    geom = G.standard(S.angles.numpy(), SMALL_STD["vol"], SMALL_STD["det"], S.standardSID,
+                      S.standardSDD)
+    L = unroll.power_norm(geom, device=cuda)
+    assert L > 2.0      # the reference step would diverge on the matched pair
+    m.lamb = 1.0 / L
+    x = torch.zeros_like(x_true).to(cuda)
+    r0 = float(p.norm())
+    for _ in range(20):
+        x = m(x, p)
+    assert float((S.ForwardProjection.apply(x) - p).norm()) < 0.5 * r0
 
 
 # ---- Beijing modules --------------------------------------------------------
@@ -171,6 +241,47 @@ def test_beijing_step_and_fbp_modules(cuda, bj_small, monkeypatch, mode):
     r = _correlate_rows(r, m.ramp.cpu().numpy(), m.cosWeight[0, 0, 0].cpu().numpy())
     r = torch.from_numpy(r).reshape(1, 1, A * P, V, U)
     assert rel_l2(out, x.double() - back(r)) <= TOL
+
+
+@pytest.mark.parametrize("mode", ["matched", "reference"])
+def test_beijing_functions_backward(cuda, bj_small, monkeypatch, mode):
+    """autograd through the Beijing drop-in Functions, batch 2, with the
+    public (angles, V, planes, U) interleave.  Reference mode keeps the
+    reference's factor placement: sampleInterval in ForwardProjection.forward
+    and BackProjection.backward, none in ForwardProjection.backward or
+    BackProjection.forward (BeijingGeometry.py:17, 20-25, 29-32, 38)."""
+    B, vec = bj_small
+    monkeypatch.setattr(B, "adjoint_mode", mode)
+    print(f"[case] beijing functions backward mode={mode}", flush=True)
+    X, Y, Z = SMALL_BJ["vol"]
+    U, V = SMALL_BJ["sub"]
+    A, P = SMALL_BJ["angles"], SMALL_BJ["planes"]
+    vs, ds = SMALL_BJ["vol"], SMALL_BJ["sub"]
+    si = B.sampleInterval
+    g = torch.Generator().manual_seed(18)
+    x = torch.randn((2, 1, Z, Y, X), generator=g)
+    w = torch.randn((2, 1, A * P, V, U), generator=g)      # public layout
+    c = torch.randn((2, 1, Z, Y, X), generator=g)
+
+    def fwd(v):
+        return _bj_public(cone.bj_forward(v.double(), vec, vs, ds) * si, A, P, V, U)
+
+    def back(s, with_si):
+        rp = _bj_projector(s.double(), A, P, V, U)
+        if mode == "matched":
+            return cone.bj_adjoint(rp, vec, vs, ds) * si
+        return cone.bj_bp_ref(rp, vec, vs, ds) * (si if with_si else 1.0)
+
+    xg = x.to(cuda).requires_grad_(True)
+    out = B.ForwardProjection.apply(xg)
+    assert rel_l2(out, fwd(x)) <= TOL
+    (gx,) = torch.autograd.grad((out * w.to(cuda)).sum(), xg)
+    assert rel_l2(gx, back(w, with_si=False)) <= TOL
+    wg = w.to(cuda).requires_grad_(True)
+    vol = B.BackProjection.apply(wg)
+    assert rel_l2(vol, back(w, with_si=False)) <= TOL
+    vol.backward(c.to(cuda))
+    assert rel_l2(wg.grad, fwd(c)) <= TOL
 
 
 def test_simulate_loader_projects_on_gpu(cuda, bj_small, tmp_path):

@@ -1,0 +1,95 @@
+"""GPU: the full-size workloads against FP64-oracle fixtures
+(tests/golden/oracle_workloads.npz, written by tests/golden/make_workloads.py).
+
+* the matched A^T over all 60 views of the 256^3 sparse-view workload (the
+  headline call: two 32-view chunks and many fixed-point view groups per
+  brick) at 4096 seeded voxels, rel L2 <= 1e-5;
+* the 16-instance uneven FISTA-TV workload: 20 iterations batched in one
+  launch per operator, eager and hipGraph replay, against the plain
+  per-volume loop (rel L2 <= 1e-6), and its smallest-work instance against
+  the same 20 iterations run in FP64 on the oracle (RMSE / peak <= 1e-4)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+
+from dtvproj import batched, configs, ops, phantom, unroll
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIX = os.path.join(ROOT, "tests", "golden", "oracle_workloads.npz")
+
+
+@pytest.fixture(scope="module")
+def wfx():
+    with np.load(FIX) as f:
+        return {k: f[k] for k in f.files}
+
+
+def _case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+@pytest.mark.parametrize("launch", [None, dict(algorithm=1),
+                                    dict(algorithm=2, brick=(16, 16, 8), block=256)])
+def test_sparse256_full_adjoint_vs_oracle_fixture(cuda, wfx, launch):
+    g = configs.sparse_256()
+    _case("sparse_256 full A^T vs oracle voxels", launch=launch)
+    y = phantom.gaussian(g.sino_shape(1), 21)
+    y64 = y.double()
+    assert abs(float(y64.sum()) - float(wfx["s256_sino_sum"])) <= 1e-9 * float(wfx["s256_sino_sumsq"])
+    assert abs(float((y64 * y64).sum()) / float(wfx["s256_sino_sumsq"]) - 1.0) <= 1e-12
+    vol = ops.adjoint(y.to(cuda), g, launch)
+    vox = torch.from_numpy(wfx["s256_voxels"].astype(np.int64))
+    X, Y, Z = g.volume_size
+    flat = (vox[:, 2] * Y + vox[:, 1]) * X + vox[:, 0]
+    got = vol.reshape(-1)[flat.to(cuda)]
+    ref = torch.from_numpy(wfx["s256_adjoint"])
+    assert rel_l2(got, ref) <= 1e-5
+    # the 512 face voxels on their own (brick edges and the volume border)
+    assert rel_l2(got[-512:], ref[-512:]) <= 1e-5
+
+
+def test_uneven16_batched_fista_tv(cuda, wfx):
+    geoms = configs.uneven_16()
+    g2 = geoms[2]
+    assert g2.volume_size == (103, 103, 75) and g2.detector_size == (153, 91)
+    assert g2.n_views == 38 and g2.ref["pitch_mm"] == (2.0, 2.0)
+    _case("uneven_16 batched FISTA-TV", instances=len(geoms), iters=20)
+    b = []
+    for i, g in enumerate(geoms):
+        ph = phantom.phantom(g.volume_size, 1.0, 10, seed=100 + i).to(cuda)
+        b.append(ops.forward(ph, g))
+    lam, eps, iters = 1e-3, 1e-3, 20
+    steps = [1.0 / (unroll.power_norm(g, iters=8, device=cuda) + lam * 12.0 / eps)
+             for g in geoms]
+    steps[2] = float(wfx["u16_step"])
+    plain = batched.plain_fista_tv(geoms, b, iters, lam, eps, steps)
+    bf = batched.BatchedFistaTV(geoms, b, iters, lam, eps, steps)
+    bf.reset()
+    eager = [x.clone() for x in bf.run()]
+    for i, (e, p) in enumerate(zip(eager, plain)):
+        assert rel_l2(e, p) <= 1e-6, f"instance {i}"
+    bf.capture()
+    for rep in range(2):
+        torch.cuda.synchronize()
+        junk = torch.randn(1000, device=cuda).sum()   # eager work between replays
+        out = bf.replay()
+        torch.cuda.synchronize()
+        for i, (o, e) in enumerate(zip(out, eager)):
+            assert rel_l2(o, e) <= 1e-6, f"replay {rep} instance {i}"
+        float(junk)
+    # the smallest-work instance against the FP64 oracle iteration
+    x = eager[2].reshape(-1).double().cpu()
+    idx = torch.from_numpy(wfx["u16_voxels"])
+    ref = torch.from_numpy(wfx["u16_values"])
+    peak = float(wfx["u16_peak"])
+    rmse = float(((x[idx] - ref) ** 2).mean().sqrt())
+    print(f"uneven_16 instance 2: RMSE/peak {rmse / peak:.3e}", flush=True)
+    assert rmse / peak <= 1e-4
+    assert abs(float(x.sum()) - float(wfx["u16_sum"])) <= 1e-4 * float(wfx["u16_sum"])
+    assert abs(float(x.abs().max()) - peak) <= 1e-4 * peak

@@ -21,6 +21,13 @@ def _bj():
 
 # ---- FBP weights -------------------------------------------------------------
 
+def test_default_adjoint_mode_is_the_reference_wiring():
+    """Reference callers keep their step sizes: both drop-in modules default
+    to the reference's backprojector as A^T."""
+    S, B = _std(), _bj()
+    assert S.adjoint_mode == "reference" and B.adjoint_mode == "reference"
+
+
 def test_standard_cos_weight_is_cosine_to_the_central_ray():
     S = _std()
     from options import standardSDD

@@ -251,3 +251,25 @@ def test_reference_backprojectors_at_selected_voxels():
     full = cone.bj_bp_ref(y, P, vs, ds)
     sub = cone.bj_bp_ref(y, P, vs, ds, voxels=vox)
     assert torch.equal(sub, full[:, 0][:, vox[:, 2], vox[:, 1], vox[:, 0]])
+
+
+@pytest.mark.parametrize("case", [
+    ((8, 8, 8), 1.0, (12, 10), 1.2, 6, 40.0, 80.0),
+    # odd detector: the central column's ray has a zero direction component
+    ((9, 7, 5), 1.1, (13, 9), 1.0, 5, 30.0, 120.0),
+    ((16, 12, 10), 0.7, (21, 15), (0.9, 1.3), 7, 25.0, 60.0),
+])
+def test_matched_adjoint_at_selected_voxels(case):
+    """``circ_adjoint_at`` (the per-voxel evaluation used for the full-size
+    fixtures) equals the full matched adjoint at every voxel."""
+    vs, vx, ds, pitch, n, sod, sdd = case
+    X, Y, Z = vs
+    U, V = ds
+    ang = G.uniform_angles(n)
+    y = phantom.gaussian((2, 1, n, V, U), 5, dtype=torch.float64)
+    full = cone.circ_adjoint(y, ang, vs, vx, ds, pitch, sod, sdd).reshape(2, -1)
+    vox = np.stack(np.meshgrid(np.arange(X), np.arange(Y), np.arange(Z), indexing="ij"),
+                   -1).reshape(-1, 3)
+    sub = cone.circ_adjoint_at(y, ang, vs, vx, ds, pitch, sod, sdd, vox)
+    ref = full[:, (vox[:, 2] * Y + vox[:, 1]) * X + vox[:, 0]]
+    assert float((sub - ref).abs().max()) <= 1e-13 * float(ref.abs().max())

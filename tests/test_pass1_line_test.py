@@ -16,7 +16,13 @@ from dtvproj import configs, geometry as G
 SLACK = np.float32(0.0625)  # the kernels' kLineSlack
 
 
-def _line_hits(views, iu, iv, blo, bhi):
+def _line_hits(views, iu, iv, blo, bhi, rcp_ulp=False):
+    """The kernels' FP32 slab test.  The kernels take 1/rc from v_rcp_f32,
+    whose result is within 1 ulp of the correctly rounded reciprocal (AMD
+    CDNA ISA, V_RCP_F32); with ``rcp_ulp`` every axis's reciprocal may be
+    the neighbour on either side, and a ray counts as a hit only if it hits
+    for every such choice (the adversarial combination: the largest entry
+    and the smallest exit over both neighbours)."""
     fv = views.astype(np.float32)
     tin = np.full(iu.shape, -3.0e38, np.float32)
     tout = np.full(iu.shape, 3.0e38, np.float32)
@@ -24,10 +30,15 @@ def _line_hits(views, iu, iv, blo, bhi):
         rc = (fv[:, 3 + c] + iu * fv[:, 6 + c]) + iv * fv[:, 9 + c] - fv[:, c]
         rc = np.where(rc != 0, rc, np.float32(1e-30)).astype(np.float32)
         ir = (np.float32(1.0) / rc).astype(np.float32)
-        t0 = ((blo[c] - fv[:, c]) * ir).astype(np.float32)
-        t1 = ((bhi[c] - fv[:, c]) * ir).astype(np.float32)
-        tin = np.maximum(tin, np.minimum(t0, t1))
-        tout = np.minimum(tout, np.maximum(t0, t1))
+        irs = [ir]
+        if rcp_ulp:
+            irs = [np.nextafter(ir, np.float32(-np.inf)), ir,
+                   np.nextafter(ir, np.float32(np.inf))]
+        for r in irs:
+            t0 = ((blo[c] - fv[:, c]) * r).astype(np.float32)
+            t1 = ((bhi[c] - fv[:, c]) * r).astype(np.float32)
+            tin = np.maximum(tin, np.minimum(t0, t1))
+            tout = np.minimum(tout, np.maximum(t0, t1))
     return tout >= tin
 
 
@@ -92,6 +103,9 @@ def test_pass1_line_test_keeps_every_ray_with_a_sample(name):
         bhi = np.array([fhi[c] + 1 + SLACK for c in range(3)], np.float32)
         hit = _line_hits(views, iu, iv, blo, bhi)
         assert not (has & ~hit).any(), (name, trial, int((has & ~hit).sum()))
+        # and with v_rcp_f32's 1-ulp reciprocal, adversarially on every axis
+        hit_rcp = _line_hits(views, iu, iv, blo, bhi, rcp_ulp=True)
+        assert not (has & ~hit_rcp).any(), (name, trial, int((has & ~hit_rcp).sum()))
         # the check has teeth: a box shrunk by half a voxel drops such rays
         hit_in = _line_hits(views, iu, iv, blo + np.float32(0.5625), bhi - np.float32(0.5625))
         missed_at_negative += int((has & ~hit_in).sum())
