@@ -13,8 +13,11 @@
 //                  D_y/D_z stop N - s before the end; rows without +1 give
 //                  d = -f[i].
 // One HBM pass: reads y (19-point stencil, cached), g, x_prev; writes x, y'.
-// Work items are (instance, voxel) over a descriptor table, so uneven
-// instances share one launch.
+// Uneven instances share one launch: each owns a contiguous range of
+// workgroups of a one-dimensional grid, found by a binary search over the
+// ranges (kernel arguments), so no workgroup falls outside its volume.
+// Lanes take voxels of 64 x 4 x-y tiles (dtv_stencil.h), so neighbours and
+// boundary tests come from coordinates, with no 64-bit division.
 //
 // The TV descent step of ASD-POCS (model/PDHG/ASDPOCS.py:37-43) uses the
 // same stencil with another normaliser:
@@ -30,50 +33,19 @@
 // extrapolation fbar = 2 f' - f.  28 B per voxel each.
 #include <hip/hip_runtime.h>
 
+#include "dtv_stencil.h"
 #include "dtvproj.h"
+
+using namespace dtv;
 
 namespace {
 
-struct Stencil {
-  const float* __restrict__ f;
-  int nx, ny, nz;
-  int64_t n;
-  int64_t s[3];
-  int mode;
-};
-
-// d_a at flat index i (i must be a valid voxel)
-__device__ __forceinline__ float diff(const Stencil& S, int64_t i, int a) {
-  const float fi = S.f[i];
-  if (S.mode == 0) {
-    bool ok;
-    if (a == 0) ok = (i % S.nx) != S.nx - 1;
-    else if (a == 1) ok = ((i / S.nx) % S.ny) != S.ny - 1;
-    else ok = (i / ((int64_t)S.nx * S.ny)) != S.nz - 1;
-    return ok ? S.f[i + S.s[a]] - fi : 0.f;
-  }
-  bool ok;
-  if (a == 0) ok = ((i + 1) % S.nx) != 0;
-  else ok = i < S.n - S.s[a];
-  return ok ? S.f[i + S.s[a]] - fi : -fi;
-}
-
-// does row i - s_a of D_a carry a +1 at column i?
-__device__ __forceinline__ bool has_prev(const Stencil& S, int64_t i, int a) {
-  if (S.mode == 0) {
-    if (a == 0) return (i % S.nx) != 0;
-    if (a == 1) return ((i / S.nx) % S.ny) != 0;
-    return i / ((int64_t)S.nx * S.ny) != 0;
-  }
-  if (a == 0) return (i % S.nx) != 0;
-  return i >= S.s[a];
-}
-
-// q = D f / n at voxel i: n = sqrt(|D f|^2 + eps^2) (kAsd false) or
-// |D f| + eps (kAsd true)
+// q = D f / n at p: n = sqrt(|D f|^2 + eps^2) (kAsd false) or |D f| + eps
+// (kAsd true)
 template <bool kAsd>
-__device__ __forceinline__ void qvec(const Stencil& S, int64_t i, float eps, float q[3]) {
-  const float dx = diff(S, i, 0), dy = diff(S, i, 1), dz = diff(S, i, 2);
+__device__ __forceinline__ void qvec(const Vol3& v, const float* __restrict__ f, const Pos& p,
+                                     float eps, float q[3]) {
+  const float dx = diff(v, f, p, 0), dy = diff(v, f, p, 1), dz = diff(v, f, p, 2);
   const float ss = dx * dx + dy * dy + dz * dz;
   const float r = kAsd ? 1.f / (sqrtf(ss) + eps) : rsqrtf(ss + eps * eps);
   q[0] = dx * r;
@@ -81,81 +53,90 @@ __device__ __forceinline__ void qvec(const Stencil& S, int64_t i, float eps, flo
   q[2] = dz * r;
 }
 
-__device__ __forceinline__ Stencil make_stencil(const float* f, int nx, int ny, int nz,
-                                                int mode) {
-  Stencil S;
-  S.f = f;
-  S.nx = nx;
-  S.ny = ny;
-  S.nz = nz;
-  S.n = (int64_t)nx * ny * nz;
-  S.s[0] = 1;
-  S.s[1] = mode == 0 ? nx : ny;
-  S.s[2] = (int64_t)nx * ny;
-  S.mode = mode;
-  return S;
-}
-
-// sum_a D_a^T q_a at voxel i
+// sum_a D_a^T q_a at p
 template <bool kAsd>
-__device__ __forceinline__ float tv_grad_at(const Stencil& S, int64_t i, float eps) {
+__device__ __forceinline__ float tv_grad_at(const Vol3& v, const float* __restrict__ f,
+                                            const Pos& p, float eps) {
   float q[3];
-  qvec<kAsd>(S, i, eps, q);
+  qvec<kAsd>(v, f, p, eps, q);
   float gtv = -(q[0] + q[1] + q[2]);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    if (has_prev(S, i, a)) {
+    if (has_prev(v, p, a)) {
       float qp[3];
-      qvec<kAsd>(S, i - S.s[a], eps, qp);
+      qvec<kAsd>(v, f, back(v, p, a), eps, qp);
       gtv += qp[a];
     }
   }
   return gtv;
 }
 
-__device__ __forceinline__ void tv_point(const dtv_tv_item& it, int mode, int64_t off,
-                                         int64_t i) {
-  const Stencil S = make_stencil(it.y + off, it.nx, it.ny, it.nz, mode);
-  const float gtv = tv_grad_at<false>(S, i, it.eps);
+__device__ __forceinline__ void tv_point(const dtv_tv_item& it, const Vol3& v, int64_t off,
+                                         const Pos& p) {
+  const float* y = it.y + off;
+  const float gtv = tv_grad_at<false>(v, y, p, it.eps);
   // read every input at i before writing: x_prev may share storage with an
   // output (y may not: neighbours read it)
-  const float yv = S.f[i];
-  const float gv = it.g[off + i];
-  const float xp = it.xprev[off + i];
+  const int64_t i = off + p.i;
+  const float yv = y[p.i];
+  const float gv = it.g[i];
+  const float xp = it.xprev[i];
   float x = yv - it.step * (gv + it.lam * gtv);
   if (it.nonneg) x = fmaxf(x, 0.f);
-  it.xnew[off + i] = x;
-  it.ynew[off + i] = x + it.beta * (x - xp);
+  it.xnew[i] = x;
+  it.ynew[i] = x + it.beta * (x - xp);
 }
 
-// one instance, blockIdx.y = batch item (volumes of equal size, contiguous)
-__global__ __launch_bounds__(256) void k_tv_fista_one(dtv_tv_item it, int mode) {
-  const int64_t nvox = (int64_t)it.nx * it.ny * it.nz;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nvox) return;
-  tv_point(it, mode, (int64_t)blockIdx.y * nvox, i);
+// one instance, blockIdx.z = batch item (volumes of equal size, contiguous)
+__global__ __launch_bounds__(256) void k_tv_fista_one(dtv_tv_item it, Vol3 v) {
+  Pos p;
+  if (!vol3_pos(v, p)) return;
+  tv_point(it, v, (int64_t)blockIdx.z * v.n, p);
 }
 
-// uneven instances from a device descriptor table, blockIdx.y = instance
+// Uneven instances in one launch: instance k owns the blocks
+// [off[k], off[k+1]) of a one-dimensional grid, (x-y tiles) x z of its own
+// volume, so no workgroup is launched outside an instance.
+constexpr int kMaxTvItems = 256;
+struct TvOffsets {
+  int n;
+  int off[kMaxTvItems + 1];
+};
+
 __global__ __launch_bounds__(256) void k_tv_fista_multi(const dtv_tv_item* __restrict__ items,
-                                                        int mode) {
-  const dtv_tv_item it = items[blockIdx.y];
-  const int64_t nvox = (int64_t)it.nx * it.ny * it.nz;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nvox) return;
-  tv_point(it, mode, 0, i);
+                                                        int mode, TvOffsets offs) {
+  const int b = (int)blockIdx.x;
+  int lo = 0, hi = offs.n;  // last k with off[k] <= b
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offs.off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const dtv_tv_item it = items[lo];
+  const Vol3 v = make_vol3(it.nx, it.ny, it.nz, mode);
+  const int tiles = v.tiles_x * ((v.ny + kTileY - 1) / kTileY);
+  const int t = b - offs.off[lo];
+  const int z = t / tiles;
+  const int tile = t - z * tiles;
+  const int ty = tile / v.tiles_x, tx = tile - ty * v.tiles_x;
+  Pos p;
+  p.x = tx * kTileX + (int)(threadIdx.x & (kTileX - 1));
+  p.y = ty * kTileY + (int)(threadIdx.x / kTileX);
+  p.z = z;
+  p.i = ((int64_t)p.z * v.ny + p.y) * v.nx + p.x;
+  if (p.x >= v.nx || p.y >= v.ny) return;
+  tv_point(it, v, 0, p);
 }
 
-// out = f - step * TV gradient (ASD normaliser), blockIdx.y = batch item
+// out = f - step * TV gradient (ASD normaliser), blockIdx.z = batch item
 __global__ __launch_bounds__(256) void k_tv_descent(const float* __restrict__ f,
-                                                    float* __restrict__ out, int nx, int ny,
-                                                    int nz, float step, float eps, int mode) {
-  const int64_t nvox = (int64_t)nx * ny * nz;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nvox) return;
-  const int64_t off = (int64_t)blockIdx.y * nvox;
-  const Stencil S = make_stencil(f + off, nx, ny, nz, mode);
-  out[off + i] = S.f[i] - step * tv_grad_at<true>(S, i, eps);
+                                                    float* __restrict__ out, Vol3 v,
+                                                    float step, float eps) {
+  Pos p;
+  if (!vol3_pos(v, p)) return;
+  const int64_t off = (int64_t)blockIdx.z * v.n;
+  const float* fb = f + off;
+  out[off + p.i] = fb[p.i] - step * tv_grad_at<true>(v, fb, p, eps);
 }
 
 // one PDHG step of the directional-TV unroll, after A and A^T.  Dual half:
@@ -163,55 +144,57 @@ __global__ __launch_bounds__(256) void k_tv_descent(const float* __restrict__ f,
 // (p: three planes [3][batch][Z][Y][X], updated in place; each lane reads
 // only its own p entries).
 __global__ __launch_bounds__(256) void k_pdhg_dual(const float* __restrict__ fbar,
-                                                   float* __restrict__ p, int nx, int ny,
-                                                   int nz, int64_t plane, float sigma, float l0,
-                                                   float l1, float l2, int mode) {
-  const int64_t nvox = (int64_t)nx * ny * nz;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nvox) return;
-  const int64_t off = (int64_t)blockIdx.y * nvox;
-  const Stencil S = make_stencil(fbar + off, nx, ny, nz, mode);
+                                                   float* __restrict__ p, Vol3 v,
+                                                   int64_t plane, float sigma, float l0,
+                                                   float l1, float l2) {
+  Pos q;
+  if (!vol3_pos(v, q)) return;
+  const int64_t off = (int64_t)blockIdx.z * v.n;
+  const float* fb = fbar + off;
   const float lam[3] = {l0, l1, l2};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     float* pa = p + a * plane + off;
-    const float v = pa[i] + sigma * diff(S, i, a);
-    pa[i] = fminf(fmaxf(v, -lam[a]), lam[a]);
+    const float val = pa[q.i] + sigma * diff(v, fb, q, a);
+    pa[q.i] = fminf(fmaxf(val, -lam[a]), lam[a]);
   }
 }
 
 // Primal half: g = atw + sum_a D_a^T p_a;  f_new = max(f - tau g, 0);
-// fbar <- 2 f_new - f;  f <- f_new.  D_a^T p at i is p_a(i - s_a) where row
-// i - s_a of D_a has its +1 at i, minus p_a(i) where row i is not empty (in
-// the reference boundary mode every row has its -1).
+// fbar <- 2 f_new - f;  f <- f_new.
 __global__ __launch_bounds__(256) void k_pdhg_primal(float* __restrict__ f,
                                                      float* __restrict__ fbar,
                                                      const float* __restrict__ atw,
-                                                     const float* __restrict__ p, int nx,
-                                                     int ny, int nz, int64_t plane, float tau,
-                                                     int mode) {
-  const int64_t nvox = (int64_t)nx * ny * nz;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nvox) return;
-  const int64_t off = (int64_t)blockIdx.y * nvox;
-  const Stencil S = make_stencil(nullptr, nx, ny, nz, mode);
-  float g = atw[off + i];
+                                                     const float* __restrict__ p, Vol3 v,
+                                                     int64_t plane, float tau) {
+  Pos q;
+  if (!vol3_pos(v, q)) return;
+  const int64_t off = (int64_t)blockIdx.z * v.n;
+  const int64_t i = off + q.i;
+  float g = atw[i];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float* pa = p + a * plane + off;
-    float dt = has_prev(S, i, a) ? pa[i - S.s[a]] : 0.f;
-    bool own;
-    if (mode != 0) own = true;
-    else if (a == 0) own = (i % nx) != nx - 1;
-    else if (a == 1) own = ((i / nx) % ny) != ny - 1;
-    else own = (i / ((int64_t)nx * ny)) != nz - 1;
-    if (own) dt -= pa[i];
+    float dt = has_prev(v, q, a) ? pa[q.i - stride(v, a)] : 0.f;
+    if (has_self(v, q, a)) dt -= pa[q.i];
     g += dt;
   }
-  const float fo = f[off + i];
+  const float fo = f[i];
   const float fn = fmaxf(fo - tau * g, 0.f);
-  f[off + i] = fn;
-  fbar[off + i] = 2.f * fn - fo;
+  f[i] = fn;
+  fbar[i] = 2.f * fn - fo;
+}
+
+// checks shared by the single-volume entry points; v = the volume's shape
+int vol_args(const int32_t* volume_size, int32_t batch, int32_t mode, Vol3& v) {
+  if (!volume_size) return DTV_ERR_NULL_POINTER;
+  if (batch < 1 || batch > 65535 || (mode != 0 && mode != 1)) return DTV_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < 3; ++i)
+    if (volume_size[i] < 1) return DTV_ERR_INVALID_ARGUMENT;
+  if (volume_size[2] > 65535) return DTV_ERR_TOO_LARGE;
+  v = make_vol3(volume_size[0], volume_size[1], volume_size[2], mode);
+  if ((int64_t)v.tiles_x * ((v.ny + kTileY - 1) / kTileY) > 0x7fffffff) return DTV_ERR_TOO_LARGE;
+  return DTV_OK;
 }
 
 }  // namespace
@@ -222,70 +205,71 @@ int dtv_tv_fista_step(const float* y, const float* g, const float* xprev, float*
                       float* ynew, int32_t batch, const int32_t volume_size[3], float step,
                       float lam, float eps, float beta, int32_t nonneg, int32_t mode,
                       void* stream) {
-  if (!y || !g || !xprev || !xnew || !ynew || !volume_size) return DTV_ERR_NULL_POINTER;
-  if (batch < 1 || batch > 65535 || (mode != 0 && mode != 1)) return DTV_ERR_INVALID_ARGUMENT;
-  for (int i = 0; i < 3; ++i)
-    if (volume_size[i] < 1) return DTV_ERR_INVALID_ARGUMENT;
+  if (!y || !g || !xprev || !xnew || !ynew) return DTV_ERR_NULL_POINTER;
+  Vol3 v;
+  int rc = vol_args(volume_size, batch, mode, v);
+  if (rc) return rc;
   dtv_tv_item it{y, g, xprev, xnew, ynew, volume_size[0], volume_size[1], volume_size[2],
                  nonneg ? 1 : 0, step, lam, eps, beta};
-  const int64_t nvox = (int64_t)volume_size[0] * volume_size[1] * volume_size[2];
-  const int64_t blocks = (nvox + 255) / 256;
-  if (blocks > 0x7fffffff) return DTV_ERR_TOO_LARGE;
-  hipLaunchKernelGGL(k_tv_fista_one, dim3((unsigned)blocks, batch), dim3(256), 0,
-                     (hipStream_t)stream, it, mode);
+  hipLaunchKernelGGL(k_tv_fista_one, vol3_grid(v, batch), dim3(256), 0, (hipStream_t)stream,
+                     it, v);
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
 }
 
-int dtv_tv_fista_step_multi(const dtv_tv_item* items, int32_t n_items, int64_t max_voxels,
-                            int32_t mode, void* stream) {
-  if (!items) return DTV_ERR_NULL_POINTER;
-  if (n_items < 1 || n_items > 65535 || max_voxels < 1 || (mode != 0 && mode != 1))
+int dtv_tv_fista_step_multi(const dtv_tv_item* items_host, const dtv_tv_item* items_dev,
+                            int32_t n_items, int32_t mode, void* stream) {
+  if (!items_host || !items_dev) return DTV_ERR_NULL_POINTER;
+  if (n_items < 1 || n_items > kMaxTvItems || (mode != 0 && mode != 1))
     return DTV_ERR_INVALID_ARGUMENT;
-  const int64_t blocks = (max_voxels + 255) / 256;
-  if (blocks > 0x7fffffff) return DTV_ERR_TOO_LARGE;
-  hipLaunchKernelGGL(k_tv_fista_multi, dim3((unsigned)blocks, n_items), dim3(256), 0,
-                     (hipStream_t)stream, items, mode);
+  TvOffsets offs;
+  offs.n = n_items;
+  int64_t total = 0;
+  for (int k = 0; k < n_items; ++k) {
+    const dtv_tv_item& it = items_host[k];
+    if (!it.y || !it.g || !it.xprev || !it.xnew || !it.ynew) return DTV_ERR_NULL_POINTER;
+    if (it.nx < 1 || it.ny < 1 || it.nz < 1) return DTV_ERR_INVALID_ARGUMENT;
+    const Vol3 v = make_vol3(it.nx, it.ny, it.nz, mode);
+    offs.off[k] = (int)total;
+    total += (int64_t)v.tiles_x * ((v.ny + kTileY - 1) / kTileY) * v.nz;
+    if (total > 0x7fffffff) return DTV_ERR_TOO_LARGE;
+  }
+  offs.off[n_items] = (int)total;
+  hipLaunchKernelGGL(k_tv_fista_multi, dim3((unsigned)total), dim3(256), 0, (hipStream_t)stream,
+                     items_dev, mode, offs);
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
 }
 
 int dtv_tv_descent(const float* f, float* out, int32_t batch, const int32_t volume_size[3],
                    float step, float eps, int32_t mode, void* stream) {
-  if (!f || !out || !volume_size) return DTV_ERR_NULL_POINTER;
-  if (batch < 1 || batch > 65535 || (mode != 0 && mode != 1) || !(eps > 0.f) || f == out)
-    return DTV_ERR_INVALID_ARGUMENT;
-  for (int i = 0; i < 3; ++i)
-    if (volume_size[i] < 1) return DTV_ERR_INVALID_ARGUMENT;
-  const int64_t nvox = (int64_t)volume_size[0] * volume_size[1] * volume_size[2];
-  const int64_t blocks = (nvox + 255) / 256;
-  if (blocks > 0x7fffffff) return DTV_ERR_TOO_LARGE;
-  hipLaunchKernelGGL(k_tv_descent, dim3((unsigned)blocks, batch), dim3(256), 0,
-                     (hipStream_t)stream, f, out, volume_size[0], volume_size[1],
-                     volume_size[2], step, eps, mode);
+  if (!f || !out) return DTV_ERR_NULL_POINTER;
+  Vol3 v;
+  int rc = vol_args(volume_size, batch, mode, v);
+  if (rc) return rc;
+  if (!(eps > 0.f) || f == out) return DTV_ERR_INVALID_ARGUMENT;
+  hipLaunchKernelGGL(k_tv_descent, vol3_grid(v, batch), dim3(256), 0, (hipStream_t)stream, f,
+                     out, v, step, eps);
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
 }
 
 int dtv_pdhg_dtv_step(float* f, float* fbar, const float* atw, float* p, int32_t batch,
                       const int32_t volume_size[3], float sigma, float tau, const float lam[3],
                       int32_t mode, void* stream) {
-  if (!f || !fbar || !atw || !p || !volume_size || !lam) return DTV_ERR_NULL_POINTER;
-  if (batch < 1 || batch > 65535 || (mode != 0 && mode != 1))
-    return DTV_ERR_INVALID_ARGUMENT;
+  if (!f || !fbar || !atw || !p || !lam) return DTV_ERR_NULL_POINTER;
+  Vol3 v;
+  int rc = vol_args(volume_size, batch, mode, v);
+  if (rc) return rc;
   if ((const void*)f == fbar || (const void*)f == atw || (const void*)fbar == atw ||
       (const void*)p == f || (const void*)p == fbar || (const void*)p == atw)
     return DTV_ERR_INVALID_ARGUMENT;
   for (int i = 0; i < 3; ++i)
-    if (volume_size[i] < 1 || !(lam[i] >= 0.f)) return DTV_ERR_INVALID_ARGUMENT;
-  const int64_t nvox = (int64_t)volume_size[0] * volume_size[1] * volume_size[2];
-  const int64_t blocks = (nvox + 255) / 256;
-  if (blocks > 0x7fffffff) return DTV_ERR_TOO_LARGE;
-  const int64_t plane = nvox * batch;
+    if (!(lam[i] >= 0.f)) return DTV_ERR_INVALID_ARGUMENT;
+  const int64_t plane = v.n * batch;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_pdhg_dual, dim3((unsigned)blocks, batch), dim3(256), 0, st, fbar, p,
-                     volume_size[0], volume_size[1], volume_size[2], plane, sigma, lam[0],
-                     lam[1], lam[2], mode);
+  const dim3 grid = vol3_grid(v, batch);
+  hipLaunchKernelGGL(k_pdhg_dual, grid, dim3(256), 0, st, fbar, p, v, plane, sigma, lam[0],
+                     lam[1], lam[2]);
   if (hipGetLastError() != hipSuccess) return DTV_ERR_HIP;
-  hipLaunchKernelGGL(k_pdhg_primal, dim3((unsigned)blocks, batch), dim3(256), 0, st, f, fbar,
-                     atw, p, volume_size[0], volume_size[1], volume_size[2], plane, tau, mode);
+  hipLaunchKernelGGL(k_pdhg_primal, grid, dim3(256), 0, st, f, fbar, atw, p, v, plane, tau);
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
 }
 

@@ -83,11 +83,18 @@ _SIGS = {
     "dtv_tv_fista_step": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32_p,
                           ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                           c_int32, c_int32, c_void_p],
-    "dtv_tv_fista_step_multi": [c_void_p, c_int32, ctypes.c_int64, c_int32, c_void_p],
+    "dtv_tv_fista_step_multi": [c_void_p, c_void_p, c_int32, c_int32, c_void_p],
     "dtv_tv_descent": [c_void_p, c_void_p, c_int32, c_int32_p, ctypes.c_float, ctypes.c_float,
                        c_int32, c_void_p],
     "dtv_pdhg_dtv_step": [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32_p,
                           ctypes.c_float, ctypes.c_float, c_float_p, c_int32, c_void_p],
+    "dtv_l1ball": [c_void_p, c_int32, c_int32, c_double, c_int32, c_void_p, ctypes.c_size_t,
+                   c_void_p],
+    "dtv_shrink_step": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64,
+                        ctypes.c_float, ctypes.c_float, ctypes.c_float, c_void_p],
+    "dtv_fdpg_step": [c_void_p, c_void_p, c_void_p, c_int32, c_int32_p, ctypes.c_float,
+                      c_float_p, c_float_p, ctypes.c_float, c_int32, c_void_p, ctypes.c_size_t,
+                      c_void_p],
     "dtv_row_filter": [c_void_p, c_void_p, ctypes.c_int64, c_int32, c_void_p, c_int32, c_void_p,
                        c_int32, c_void_p],
     "dtv_views_standard": [c_double_p, c_int32, c_int32_p, c_int32_p, c_double, c_double,
@@ -96,7 +103,10 @@ _SIGS = {
                           ctypes.POINTER(DtvScheme)],
 }
 
-EXPORTED = tuple(_SIGS) + ("dtv_strerror", "dtv_version")
+# entry points returning a size (no GPU work)
+_SIZE_FNS = {"dtv_l1ball_workspace_bytes": [c_int32], "dtv_fdpg_workspace_bytes": []}
+
+EXPORTED = tuple(_SIGS) + tuple(_SIZE_FNS) + ("dtv_strerror", "dtv_version")
 
 
 def lib():
@@ -116,6 +126,10 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = argtypes
             fn.restype = c_int32
+        for name, argtypes in _SIZE_FNS.items():
+            fn = getattr(L, name)
+            fn.argtypes = argtypes
+            fn.restype = ctypes.c_size_t
         L.dtv_strerror.argtypes = [c_int32]
         L.dtv_strerror.restype = ctypes.c_char_p
         L.dtv_version.argtypes = []

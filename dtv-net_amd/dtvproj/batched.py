@@ -19,7 +19,6 @@ from __future__ import annotations
 
 import ctypes
 
-import numpy as np
 import torch
 
 from . import _capi, autotune, ops, tv, unroll
@@ -180,7 +179,6 @@ class BatchedFistaTV:
         self.fwd = [self.mp.bind_forward(self.y[p], self.r, self.b) for p in range(2)]
         self.adj = self.mp.bind_adjoint(self.r, self.g)
         betas = unroll.fista_momentum(iters)
-        self.max_vox = max(int(np.prod(g.volume_size)) for g in self.geoms)
         self.tv_tables = []
         for k in range(iters):
             p = k % 2
@@ -197,7 +195,7 @@ class BatchedFistaTV:
                 it.nonneg = 1
                 it.step, it.lam, it.eps, it.beta = steps[i], lam, eps, betas[k]
             self.tv_tables.append(
-                torch.frombuffer(bytearray(items), dtype=torch.uint8).to(dev))
+                (items, torch.frombuffer(bytearray(items), dtype=torch.uint8).to(dev)))
         self.graph = None
 
     def reset(self):
@@ -212,8 +210,9 @@ class BatchedFistaTV:
             p = k % 2
             self.mp.run(self.fwd[p], adjoint=False)
             self.mp.run(self.adj, adjoint=True)
-            rc = L.dtv_tv_fista_step_multi(self.tv_tables[k].data_ptr(), len(self.geoms),
-                                           self.max_vox, self.mode, stream)
+            host, dev_table = self.tv_tables[k]
+            rc = L.dtv_tv_fista_step_multi(ctypes.addressof(host), dev_table.data_ptr(),
+                                           len(self.geoms), self.mode, stream)
             _capi.check(rc, "dtv_tv_fista_step_multi")
         return self.x
 

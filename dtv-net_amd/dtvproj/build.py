@@ -20,7 +20,7 @@ LIB_NAME = "libdtvproj.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 ARCH = "gfx950"
 
-SOURCES = ["dtv_project.hip", "dtv_brick.hip", "dtv_tv.hip", "dtv_fbp.hip"]
+SOURCES = ["dtv_project.hip", "dtv_brick.hip", "dtv_tv.hip", "dtv_fbp.hip", "dtv_shrink.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -198,16 +198,72 @@ def pdhg_dtv_step(f, fbar, atw, p, sigma, tau, lam, mode="neumann"):
     return f, fbar, p
 
 
-def l1ball(v, b, tol=1e-4, max_iter=100):
+class L1Item(ctypes.Structure):
+    """One vector of a batched l1-ball projection (mirrors dtv_l1_item)."""
+    _fields_ = [("v", ctypes.c_void_p), ("out", ctypes.c_void_p), ("n", ctypes.c_int64),
+                ("radius", ctypes.c_double), ("sigma", ctypes.c_float),
+                ("reserved", ctypes.c_int32)]
+
+
+L1_MAX_ITEMS = 16
+
+
+def _l1ball_device(vs, radii, outs, mode, sigma=1.0, tol=1e-4, max_iter=100):
+    """Run ``dtv_l1ball`` over the (contiguous FP32 CUDA) tensors ``vs``."""
+    n = len(vs)
+    if not 1 <= n <= L1_MAX_ITEMS:
+        raise RuntimeError(f"l1ball takes 1..{L1_MAX_ITEMS} vectors per call")
+    dev = vs[0].device
+    items = (L1Item * n)()
+    for k, (v, o, b) in enumerate(zip(vs, outs, radii)):
+        check_input(v, "v")
+        check_input(o, "out")
+        if v.dtype != torch.float32 or o.dtype != torch.float32 or o.numel() != v.numel():
+            raise RuntimeError("l1ball takes float32 tensors, out of the size of v")
+        if v.device != dev or o.device != dev:
+            raise RuntimeError("l1ball vectors must be on one device")
+        items[k].v = v.data_ptr()
+        items[k].out = o.data_ptr()
+        items[k].n = v.numel()
+        items[k].radius = float(b)
+        items[k].sigma = float(sigma)
+    L = _capi.lib()
+    nbytes = L.dtv_l1ball_workspace_bytes(n)
+    ws = torch.empty(nbytes // 8 + 1, dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        rc = L.dtv_l1ball(ctypes.addressof(items), n, mode, float(tol), int(max_iter),
+                          ws.data_ptr(), ws.numel() * 8, _stream(dev))
+    _capi.check(rc, "dtv_l1ball")
+    return outs
+
+
+def l1ball(v, b, tol=1e-4, max_iter=100, out=None):
     """Shrink v onto the l1 ball of radius b as model/PDHG/Utils/L1ball.py:4-17
     does: if sum|v| < b, v itself; else the threshold lam solving
     sum relu(|v| - lam) = b by the reference's damped Newton iteration from
     lam = 0 (slope -count(|v| > lam) + 0.001, stop once |objective| <= tol or
     after max_iter steps), then sign(v) relu(|v| - lam).
 
-    The iteration runs on v's device with lam a 0-d FP64 tensor (the
-    reference keeps it as a Python float) and checks for convergence on the
-    host every 8 steps, where the reference synchronises on every step."""
+    Runs on the fused gfx950 kernels (``dtv_l1ball``: the whole iteration on
+    the GPU, no host synchronisation); ``v`` is a contiguous FP32 CUDA
+    tensor.  :func:`l1ball_torch` is the same iteration as torch ops."""
+    out = torch.empty_like(v) if out is None else out
+    return _l1ball_device([v], [b], [out], 0, tol=tol, max_iter=max_iter)[0]
+
+
+def l1ball_dual(ps, sigma, radii, tol=1e-4, max_iter=100):
+    """The dual update of model/PDHG/DTVCP.py:44-46 for every tensor in ``ps``
+    (up to 16, updated in place): p <- p - sigma sign(p) l1ball(|p| / sigma,
+    b), one ``dtv_l1ball`` call (mode 1) for all of them."""
+    _l1ball_device(list(ps), list(radii), list(ps), 1, sigma=sigma, tol=tol,
+                   max_iter=max_iter)
+    return ps
+
+
+def l1ball_torch(v, b, tol=1e-4, max_iter=100):
+    """:func:`l1ball` as torch ops on v's device, lam a 0-d FP64 tensor (the
+    reference keeps it as a Python float), with a convergence check on the
+    host every 8 steps (the reference synchronises on every step)."""
     a = v.abs()
     total = a.sum()
     if float(total) < b:
@@ -229,6 +285,72 @@ def l1ball(v, b, tol=1e-4, max_iter=100):
 def shrink(v, thr):
     """Soft threshold sign(v) max(|v| - thr, 0)."""
     return torch.sign(v) * torch.clamp(v.abs() - thr, min=0)
+
+
+def shrink_step(y, g, xprev, step, thr, beta, xnew=None, ynew=None):
+    """Fused soft-threshold FISTA step (``dtv_shrink_step``, model/FISTA/FISTA.py:21-25):
+    d = y - step g; x = sign(d) relu(|d| - thr); y' = x + beta (x - xprev).
+    Returns (x, y'); xnew may be xprev (updated in place)."""
+    for t, n in ((y, "y"), (g, "g"), (xprev, "xprev")):
+        check_input(t, n)
+        if t.dtype != torch.float32 or t.shape != y.shape:
+            raise RuntimeError(f"{n} must be float32 with the shape of y")
+    xnew = torch.empty_like(y) if xnew is None else xnew
+    ynew = torch.empty_like(y) if ynew is None else ynew
+    if ynew.data_ptr() == xprev.data_ptr():
+        raise RuntimeError("ynew must not share storage with xprev")
+    with torch.cuda.device(y.device):
+        rc = _capi.lib().dtv_shrink_step(y.data_ptr(), g.data_ptr(), xprev.data_ptr(),
+                                         xnew.data_ptr(), ynew.data_ptr(), y.numel(),
+                                         float(step), float(thr), float(beta), _stream(y.device))
+    _capi.check(rc, "dtv_shrink_step")
+    return xnew, ynew
+
+
+def shrink_step_torch(y, g, xprev, step, thr, beta):
+    """Unfused torch formulation of :func:`shrink_step`."""
+    x = shrink(y - step * g, thr)
+    return x, x + beta * (x - xprev)
+
+
+def fdpg_step(f, g, sigma, v, t, beta, mode="reference", out=None):
+    """One fused step of the directional-TV dual-gradient unroll
+    (``dtv_fdpg_step``, model/FISTA/DTVFISTA.py:29-41) after g = A^T(A f - b):
+    z = relu(f - sigma g); m_a = max(|D_a z| - v_a) over the whole batch;
+    f_new = z + sum_a t_a D_a^T (sign(D_a z) m_a); returns
+    f_new + beta (f_new - f) (a new tensor, or ``out``)."""
+    for x, n in ((f, "f"), (g, "g")):
+        check_input(x, n)
+        if x.dtype != torch.float32 or x.shape != f.shape:
+            raise RuntimeError(f"{n} must be float32 with the shape of f")
+    out = torch.empty_like(f) if out is None else out
+    if out.data_ptr() in (f.data_ptr(), g.data_ptr()):
+        raise RuntimeError("out must not share storage with f or g")
+    Z, Y, X = f.shape[-3:]
+    B = f.numel() // (X * Y * Z)
+    L = _capi.lib()
+    ws = torch.empty(L.dtv_fdpg_workspace_bytes() // 4 + 1, dtype=torch.int32, device=f.device)
+    v3 = (ctypes.c_float * 3)(*[float(x) for x in v])
+    t3 = (ctypes.c_float * 3)(*[float(x) for x in t])
+    with torch.cuda.device(f.device):
+        rc = L.dtv_fdpg_step(f.data_ptr(), g.data_ptr(), out.data_ptr(), B, _capi.int3((X, Y, Z)),
+                             float(sigma), v3, t3, float(beta), MODES[mode], ws.data_ptr(),
+                             ws.numel() * 4, _stream(f.device))
+    _capi.check(rc, "dtv_fdpg_step")
+    return out
+
+
+def fdpg_step_torch(f, g, sigma, v, t, beta, mode="reference"):
+    """Unfused torch formulation of :func:`fdpg_step` (the reference's
+    statements with D_a / D_a^T in place of the sparse matrices)."""
+    z = torch.relu(f - sigma * g)
+    axes = ("x", "y", "z")
+    fnew = z
+    for i, a in enumerate(axes):
+        d = D(z, a, mode)
+        p = torch.sign(d) * torch.max(d.abs() - v[i])
+        fnew = fnew + t[i] * DT(p, a, mode)
+    return fnew + beta * (fnew - f)
 
 
 # --------------------------------------------------------------------------

@@ -18,7 +18,12 @@
 * :func:`asd_pocs` -- ASD-POCS as model/PDHG/ASDPOCS.py:28-47 runs it
   (data step, then ``ng`` TV descent steps on the fused gfx950 kernel).
 * :func:`dtv_cp` -- the directional-TV Chambolle-Pock iteration of
-  model/PDHG/DTVCP.py:29-60 with the reference's l1-ball dual constraints.
+  model/PDHG/DTVCP.py:29-60 with the reference's l1-ball dual constraints
+  (the Newton l1-ball projection on the fused gfx950 kernel).
+* :func:`fista_l1` -- the soft-threshold FISTA of model/FISTA/FISTA.py:16-27
+  (fused shrink + momentum kernel).
+* :func:`fdpg_dtv` -- the directional-TV dual-gradient iteration of
+  model/FISTA/DTVFISTA.py:23-43 (fused kernel per step).
 * :func:`power_method` -- HyperParams.py:6-13 for an operator pair.
 """
 from __future__ import annotations
@@ -227,7 +232,8 @@ def asd_pocs(image, sino, project, backproject, cascades: int = 30, beta: float 
 
 @torch.no_grad()
 def dtv_cp(image, sino, project, backproject, cascades: int = 30, v=(1.0, 1.0, 1.0),
-           mu: float = 1.0, sigma: float = 1e-7, tau: float = 1e-7, mode: str = "reference"):
+           mu: float = 1.0, sigma: float = 1e-7, tau: float = 1e-7, mode: str = "reference",
+           fused: bool = True):
     """Directional-TV Chambolle-Pock of model/PDHG/DTVCP.py:29-60.
 
     Every 10 cascades the bounds t_a = sum|D_a f| are refreshed (:34-37).
@@ -237,7 +243,10 @@ def dtv_cp(image, sino, project, backproject, cascades: int = 30, v=(1.0, 1.0, 1
     c = -relu(-c - sigma mu f_bar) (:47); f_new = f - r - tau (sum_a v_a
     D_a^T p_a + mu c) (:48-52 -- the reference does not scale r by tau);
     f_bar = 2 f_new - f.  ``v`` = (v1, v2, v3): the reference sets
-    v_a = 1e5 / ||D_a||, mu = 1e5 / ||I||, sigma = tau = 1e-7 (:18-27)."""
+    v_a = 1e5 / ||D_a||, mu = 1e5 / ||I||, sigma = tau = 1e-7 (:18-27).
+    With ``fused`` the three l1-ball projections and dual updates run as one
+    ``dtv_l1ball`` call (the Newton iterations on the GPU, no host
+    synchronisation); otherwise as torch ops (:func:`tv.l1ball_torch`)."""
     w = torch.zeros_like(sino)
     p = [torch.zeros_like(image) for _ in range(3)]
     c = torch.zeros_like(image)
@@ -251,12 +260,69 @@ def dtv_cp(image, sino, project, backproject, cascades: int = 30, v=(1.0, 1.0, 1
         res = project(fbar) - sino
         w = (w + sigma * res) / (1 + sigma)
         recon = backproject(w)
-        for i, a in enumerate(axes):
-            pa = p[i] + v[i] * sigma * tv.D(fbar, a, mode)
-            p[i] = pa - sigma * torch.sign(pa) * tv.l1ball(pa.abs() / sigma, v[i] * t[i])
+        pas = [p[i] + v[i] * sigma * tv.D(fbar, a, mode) for i, a in enumerate(axes)]
+        if fused:
+            p = tv.l1ball_dual([pa.contiguous() for pa in pas], sigma,
+                               [v[i] * t[i] for i in range(3)])
+        else:
+            p = [pa - sigma * torch.sign(pa) * tv.l1ball_torch(pa.abs() / sigma, v[i] * t[i])
+                 for i, pa in enumerate(pas)]
         c = -F.relu(-c - sigma * mu * fbar)
         reg = sum(v[i] * tv.DT(p[i], a, mode) for i, a in enumerate(axes)) + mu * c
         fnew = f - recon - tau * reg
         fbar = 2 * fnew - f
         f = fnew
+    return f
+
+
+@torch.no_grad()
+def fista_l1(image, sino, project, backproject, cascades: int = 30, lamb: float = 10e3,
+             L: float = 1e-4, fused: bool = True):
+    """Soft-threshold FISTA of model/FISTA/FISTA.py:16-27 (its constants:
+    lamb = 10e3, L = 1e-4, tuned for the reference's FDK backprojector).
+
+    Per cascade: d = y - L B(A y - sino); I = sign(d) relu(|d| - lamb L);
+    t' = (1 + sqrt(1 + 4 t^2)) / 2; y = I + (t - 1)/t' (I - I_prev); returns I.
+    With ``fused`` the shrink and momentum are one ``dtv_shrink_step``
+    launch; otherwise torch ops."""
+    t = 1.0
+    x = image.clone()          # I = Ip = image
+    y = image.clone()
+    for _ in range(cascades):
+        g = backproject(project(y) - sino)
+        tp = (1.0 + math.sqrt(1.0 + 4.0 * t * t)) / 2.0
+        beta = (t - 1.0) / tp
+        if fused:
+            x, y = tv.shrink_step(y.contiguous(), g.contiguous(), x, L, lamb * L, beta,
+                                  xnew=x, ynew=torch.empty_like(y))
+        else:
+            x, y = tv.shrink_step_torch(y, g, x, L, lamb * L, beta)
+        t = tp
+    return x
+
+
+@torch.no_grad()
+def fdpg_dtv(image, sino, project, backproject, v, cascades: int = 30, sigma: float = 1e-3,
+             t=(1e-3, 1e-3, 1e-3), mode: str = "reference", fused: bool = True):
+    """Directional-TV dual-gradient iteration of model/FISTA/DTVFISTA.py:23-43.
+
+    Per cascade: z = relu(f - sigma B(A f - sino)); per axis
+    p_a = sign(D_a z) max(|D_a z| - v_a) (torch.max without a dimension: one
+    scalar over the whole tensor); f_new = sum_a t_a D_a^T p_a + z;
+    f = f_new + (t - 1)/t' (f_new - f).  The reference computes t' from t but
+    never assigns t, so t stays 1 and the momentum factor is 0; that is
+    kept.  ``v`` = (v1, v2, v3): the reference sets v_a = 0.88 ||D_a|| /
+    1e-3 (``power_method`` of its sparse matrices, :16-18).  With ``fused``
+    each step after A and A^T is one ``dtv_fdpg_step`` (three launches);
+    otherwise torch ops."""
+    f = image
+    tm = 1.0
+    for _ in range(cascades):
+        tp = (1.0 + math.sqrt(1.0 + 4.0 * tm * tm)) / 2.0
+        beta = (tm - 1.0) / tp
+        g = backproject(project(f) - sino)
+        if fused:
+            f = tv.fdpg_step(f.contiguous(), g.contiguous(), sigma, v, t, beta, mode)
+        else:
+            f = tv.fdpg_step_torch(f, g, sigma, v, t, beta, mode)
     return f

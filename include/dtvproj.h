@@ -18,6 +18,7 @@
 #ifndef DTVPROJ_H
 #define DTVPROJ_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -211,10 +212,13 @@ int dtv_tv_fista_step(const float* y, const float* g, const float* xprev, float*
                       float lam, float eps, float beta, int32_t nonneg, int32_t mode,
                       void* stream);
 
-/* Same step over uneven instances in one launch; ``items`` is a device array
- * of n_items descriptors, max_voxels the largest instance's voxel count. */
-int dtv_tv_fista_step_multi(const dtv_tv_item* items, int32_t n_items, int64_t max_voxels,
-                            int32_t mode, void* stream);
+/* Same step over uneven instances in one launch.  items_dev is a device
+ * array of n_items (<= 256) descriptors, items_host its host copy (read on
+ * the host to size the launch: each instance gets exactly its own
+ * workgroups; the kernel reads the device copy, so the call can be captured
+ * in a graph). */
+int dtv_tv_fista_step_multi(const dtv_tv_item* items_host, const dtv_tv_item* items_dev,
+                            int32_t n_items, int32_t mode, void* stream);
 
 /* One TV descent step of ASD-POCS (model/PDHG/ASDPOCS.py:37-43, whose
  * gradTV is built from the sparse matrices of Gradients.py:8-47):
@@ -239,6 +243,62 @@ int dtv_tv_descent(const float* f, float* out, int32_t batch, const int32_t volu
 int dtv_pdhg_dtv_step(float* f, float* fbar, const float* atw, float* p, int32_t batch,
                       const int32_t volume_size[3], float sigma, float tau, const float lam[3],
                       int32_t mode, void* stream);
+
+/* ---- Fused shrinkage steps ----------------------------------------------- */
+
+/* One vector of a batched l1-ball projection (device pointers). */
+#define DTV_L1_MAX_ITEMS 16
+typedef struct dtv_l1_item {
+  const float* v;   /* input, n values */
+  float* out;       /* output, n values (may be v) */
+  int64_t n;
+  double radius;    /* b */
+  float sigma;      /* mode 1 only: the dual step size (> 0) */
+  int32_t reserved;
+} dtv_l1_item;
+
+/* Workspace of dtv_l1ball for n_items vectors (0 when n_items is out of
+ * range); device memory, 8-byte aligned, no initialisation needed. */
+size_t dtv_l1ball_workspace_bytes(int32_t n_items);
+
+/* l1-ball projection of up to DTV_L1_MAX_ITEMS vectors in one call,
+ * following model/PDHG/Utils/L1ball.py:4-17: with a_i = |v_i| (mode 0) or
+ * |v_i| / sigma (mode 1), if sum a_i < b the ball holds the point;
+ * otherwise lam solves sum relu(a_i - lam) = b by the reference's Newton
+ * iteration from lam = 0 (slope 0.001 - #{a_i > lam}, stop once
+ * |sum relu(a_i - lam) - b| <= tol or after max_iter steps; the reference
+ * has tol = 1e-4, max_iter = 100).  Writes
+ *   mode 0: out = v if inside, else sign(v) relu(|v| - lam)
+ *   mode 1: out = v - sigma sign(v) (a if inside, else relu(a - lam)),
+ *           the dual update p = p' - sigma sign(p') l1ball(|p'| / sigma, b)
+ *           of model/PDHG/DTVCP.py:44-46.
+ * Runs entirely on ``stream`` with no host synchronisation (one launch per
+ * Newton step; steps after convergence return at once).  No reference
+ * counterpart on the device: the reference loops in Python with a host
+ * synchronisation per step. */
+int dtv_l1ball(const dtv_l1_item* items, int32_t n_items, int32_t mode, double tol,
+               int32_t max_iter, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Soft threshold with FISTA momentum over n values (model/FISTA/FISTA.py:21-25):
+ *   d = y - step * g;  xnew = sign(d) relu(|d| - thr);  ynew = xnew + beta (xnew - xprev)
+ * xnew may be xprev; ynew must differ from xprev. */
+int dtv_shrink_step(const float* y, const float* g, const float* xprev, float* xnew,
+                    float* ynew, int64_t n, float step, float thr, float beta, void* stream);
+
+/* Workspace of dtv_fdpg_step (device memory, 4-byte aligned). */
+size_t dtv_fdpg_workspace_bytes(void);
+
+/* One step of the directional-TV dual-gradient unroll of
+ * model/FISTA/DTVFISTA.py:29-41 after the caller's g = A^T (A f - sino):
+ *   z = relu(f - sigma g);  m_a = max over all voxels of the batch of |D_a z| - v_a;
+ *   p_a = sign(D_a z) m_a;  fnew = z + sum_a t_a D_a^T p_a;
+ *   fout = fnew + beta (fnew - f)
+ * (D_a as in dtv_tv_fista_step, mode 0 Neumann / 1 reference matrices).
+ * f, g, fout: [batch][Z][Y][X], fout distinct from f and g.  Three launches. */
+int dtv_fdpg_step(const float* f, const float* g, float* fout, int32_t batch,
+                  const int32_t volume_size[3], float sigma, const float v[3], const float t[3],
+                  float beta, int32_t mode, void* workspace, size_t workspace_bytes,
+                  void* stream);
 
 /* ---- FBP front end ------------------------------------------------------ */
 

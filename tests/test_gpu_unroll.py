@@ -239,10 +239,72 @@ def test_dtv_cp_matches_fp64_oracle_run():
     kw = dict(cascades=5, v=(0.5, 0.5, 0.5), mu=1.0, sigma=0.5 / lip, tau=0.5 / lip)
     got = unroll.dtv_cp(start.float().to(cuda), b64.float().to(cuda),
                         lambda x: ops.forward(x, g), lambda y: ops.adjoint(y, g), **kw)
-    ref = unroll.dtv_cp(start, b64, A64, AT64, **kw)
+    plain = unroll.dtv_cp(start.float().to(cuda), b64.float().to(cuda),
+                          lambda x: ops.forward(x, g), lambda y: ops.adjoint(y, g),
+                          fused=False, **kw)
+    ref = unroll.dtv_cp(start, b64, A64, AT64, fused=False, **kw)
     peak = float(ref.abs().max())
-    rmse = float(((got.double().cpu() - ref) ** 2).mean().sqrt())
-    assert rmse / peak <= 1e-4
+    for x in (got, plain):
+        rmse = float(((x.double().cpu() - ref) ** 2).mean().sqrt())
+        assert rmse / peak <= 1e-4
     # the iteration moved the data residual down from the start
     r0 = float((A64(start) - b64).norm())
     assert float((A64(ref) - b64).norm()) < r0
+
+
+def test_fista_l1_matches_fp64_oracle_run():
+    """8 cascades of the soft-threshold FISTA of FISTA.py:16-27 (step 1/L of
+    the matched pair, threshold lamb L): fused shrink + momentum kernel and
+    torch ops on the GPU vs the same cascades on the FP64 oracle operators."""
+    cuda = torch.device("cuda", 0)
+    g = configs.fd_32()
+    _case("fista_l1 fd_32")
+    A64, AT64 = _oracle_ops(g)
+    ph = phantom.phantom((32, 32, 32), 1.0, 6, seed=12)
+    b64 = A64(ph.double())
+    lip = unroll.power_norm(g, device=cuda)
+    kw = dict(cascades=8, L=1.0 / lip, lamb=0.05 * lip)
+    proj = lambda x: ops.forward(x, g)
+    back = lambda y: ops.adjoint(y, g)
+    x0 = torch.zeros(g.volume_shape(1), device=cuda)
+    fused = unroll.fista_l1(x0, b64.float().to(cuda), proj, back, **kw)
+    plain = unroll.fista_l1(x0, b64.float().to(cuda), proj, back, fused=False, **kw)
+    ref = unroll.fista_l1(torch.zeros(g.volume_shape(1), dtype=torch.float64), b64, A64, AT64,
+                          fused=False, **kw)
+    peak = float(ref.abs().max())
+    assert float((ref == 0).double().mean()) > 0.05      # the threshold acts
+    for x in (fused, plain):
+        rmse = float(((x.double().cpu() - ref) ** 2).mean().sqrt())
+        assert rmse / peak <= 1e-4
+    assert float(x0.abs().max()) == 0.0                   # the start is not modified
+
+
+def test_fdpg_dtv_matches_fp64_oracle_run():
+    """6 cascades of the directional-TV dual-gradient iteration of
+    DTVFISTA.py:23-43 (reference-mode difference matrices): fused step and
+    torch ops on the GPU vs the same cascades on the FP64 oracle operators."""
+    cuda = torch.device("cuda", 0)
+    g = configs.fd_32()
+    _case("fdpg_dtv fd_32")
+    A64, AT64 = _oracle_ops(g)
+    ph = phantom.phantom((32, 32, 32), 1.0, 6, seed=13)
+    b64 = A64(ph.double())
+    lip = unroll.power_norm(g, device=cuda)
+    kw = dict(cascades=6, sigma=1.0 / lip, v=(0.02, 0.03, 0.02), t=(0.05, 0.05, 0.04))
+    proj = lambda x: ops.forward(x, g)
+    back = lambda y: ops.adjoint(y, g)
+    # a noisy start: the iteration takes sign(D_a z), which FP32 and FP64
+    # would only disagree on where D_a z rounds to about zero -- with noise
+    # of 0.05 almost nowhere
+    start = 0.5 * ph + 0.05 * phantom.gaussian(ph.shape, 14)
+    fused = unroll.fdpg_dtv(start.to(cuda), b64.float().to(cuda), proj, back, **kw)
+    plain = unroll.fdpg_dtv(start.to(cuda), b64.float().to(cuda), proj, back, fused=False, **kw)
+    ref = unroll.fdpg_dtv(start.double(), b64, A64, AT64, fused=False, **kw)
+    peak = float(ref.abs().max())
+    for x in (fused, plain):
+        rmse = float(((x.double().cpu() - ref) ** 2).mean().sqrt())
+        assert rmse / peak <= 1e-4
+    # the TV part moves the result away from the plain projected gradient step
+    no_tv = unroll.fdpg_dtv(start.to(cuda), b64.float().to(cuda), proj, back,
+                            **dict(kw, t=(0.0, 0.0, 0.0)))
+    assert rel_l2(no_tv, ref) > 1e-4

@@ -132,8 +132,8 @@ def test_l1ball_matches_oracle():
     v = phantom.gaussian((1, 1, 6, 7, 8), 14, dtype=torch.float64)
     for b in (5.0, 60.0, 1e4):
         want = otv.l1ball(v.numpy(), b)
-        np.testing.assert_allclose(tv.l1ball(v, b).numpy(), want, atol=1e-12)
-        got32 = tv.l1ball(v.float(), b).double().numpy()
+        np.testing.assert_allclose(tv.l1ball_torch(v, b).numpy(), want, atol=1e-12)
+        got32 = tv.l1ball_torch(v.float(), b).double().numpy()
         assert np.linalg.norm(got32 - want) <= 1e-5 * max(np.linalg.norm(want), 1e-30)
 
 
@@ -232,13 +232,117 @@ def test_fused_pdhg_dtv_step(cuda, mode, shp):
 
 
 @pytest.mark.gpu
-def test_l1ball_on_the_gpu_matches_oracle(cuda):
-    print("[case] l1ball gpu", flush=True)
-    v = phantom.gaussian((1, 1, 16, 16, 16), 22)
-    for b in (10.0, 400.0, 1e5):
+@pytest.mark.parametrize("n", [1, 5, 4096, 1 << 21])
+def test_l1ball_on_the_gpu_matches_oracle(cuda, n):
+    """dtv_l1ball (the Newton iteration on the device) vs the FP64 oracle
+    restatement of L1ball.py:4-17, from one value to 2M values (every
+    workgroup of the reduction busy), inside and outside the ball."""
+    print(f"[case] l1ball gpu n={n}", flush=True)
+    v = phantom.gaussian((n,), 22)
+    total = float(v.double().abs().sum())
+    for b in (0.0, 0.01 * total, 0.5 * total, 1.1 * total):
         want = torch.from_numpy(otv.l1ball(v.double().numpy(), b))
         got = tv.l1ball(v.to(cuda), b)
+        if b == 0.0:
+            # the radius-0 ball: what remains is below the stopping tolerance
+            assert float(got.double().abs().sum()) <= 1e-3
+            assert float(want.abs().sum()) <= 1e-3
+        else:
+            assert rel_l2(got, want) <= 1e-5, b
+        if b > total:
+            assert torch.equal(got.cpu(), v)   # inside: v itself (L1ball.py:5-6)
+
+
+@pytest.mark.gpu
+def test_l1ball_batched_and_dual_update(cuda):
+    """Three vectors of different size and radius in one call, as DTV-CP's
+    dual update (mode 1): p <- p - sigma sign(p) l1ball(|p| / sigma, b),
+    against the same composition on the FP64 oracle; and the projection mode
+    on the same vectors."""
+    print("[case] l1ball batched dual", flush=True)
+    sigma = 1e-3
+    ps = [phantom.gaussian((1, 1, 5, 6, 7), 41) * 1e-3, phantom.gaussian((70000,), 42) * 2e-3,
+          phantom.gaussian((3, 4), 43) * 1e-3]
+    radii = [0.2 * float((p.double().abs() / sigma).sum()) for p in ps[:2]] + [1e9]
+    want = []
+    for p, b in zip(ps, radii):
+        pd = p.double().numpy()
+        want.append(pd - sigma * np.sign(pd) * otv.l1ball(np.abs(pd) / sigma, b))
+    got = tv.l1ball_dual([p.to(cuda) for p in ps], sigma, radii)
+    for g, w in zip(got, want):
+        if np.linalg.norm(w) == 0.0:
+            assert float(g.abs().max()) <= 1e-9
+        else:
+            assert rel_l2(g, torch.from_numpy(w)) <= 1e-5
+    outs = [torch.empty_like(p).to(cuda) for p in ps]
+    tv._l1ball_device([p.to(cuda) for p in ps], radii, outs, 0)
+    for o, p, b in zip(outs, ps, radii):
+        w = torch.from_numpy(otv.l1ball(p.double().numpy(), b))
+        assert rel_l2(o, w) <= 1e-5
+
+
+@pytest.mark.gpu
+def test_l1ball_iteration_count_and_tolerance_follow_the_reference(cuda):
+    """max_iter and tol are the reference's loop controls: with max_iter = 1
+    only the first Newton step (from the sums at lam = 0) is taken, as in the
+    oracle run with max_iter = 1."""
+    print("[case] l1ball max_iter", flush=True)
+    v = phantom.gaussian((10000,), 44)
+    b = 0.3 * float(v.double().abs().sum())
+    for it in (1, 2, 3):
+        want = torch.from_numpy(otv.l1ball(v.double().numpy(), b, max_iter=it))
+        got = tv.l1ball(v.to(cuda), b, max_iter=it)
         assert rel_l2(got, want) <= 1e-5
+    full = torch.from_numpy(otv.l1ball(v.double().numpy(), b))
+    assert rel_l2(tv.l1ball(v.to(cuda), b, max_iter=1), full) > 1e-3
+
+
+@pytest.mark.gpu
+def test_fused_shrink_step_closed_form(cuda):
+    print("[case] shrink step", flush=True)
+    y = phantom.gaussian((2, 1, 9, 10, 11), 51)
+    g = phantom.gaussian((2, 1, 9, 10, 11), 52)
+    xp = phantom.gaussian((2, 1, 9, 10, 11), 53)
+    step, thr, beta = 0.3, 0.25, 0.6
+    d = y.double() - step * g.double()
+    x = torch.sign(d) * torch.clamp(d.abs() - thr, min=0)
+    assert float((x == 0).double().mean()) > 0.1
+    yn = x + beta * (x - xp.double())
+    gx, gy = tv.shrink_step(y.to(cuda), g.to(cuda), xp.to(cuda), step, thr, beta)
+    assert rel_l2(gx, x) <= 1e-6 and rel_l2(gy, yn) <= 1e-6
+    xpd = xp.to(cuda)
+    gx2, gy2 = tv.shrink_step(y.to(cuda), g.to(cuda), xpd, step, thr, beta, xnew=xpd)
+    assert rel_l2(gx2, x) <= 1e-6 and rel_l2(gy2, yn) <= 1e-6
+
+
+def _fdpg_fp64(f, g, sigma, v, t, beta, mode):
+    """One DTVFISTA.py:29-41 step in FP64 on the explicit sparse matrices."""
+    Z, Y, X = f.shape[-3:]
+    mats = [(otv.reference_matrix if mode == "reference" else otv.neumann_matrix)((X, Y, Z), a)
+            for a in "xyz"]
+    fb = f.reshape(-1, X * Y * Z)
+    z = np.maximum(fb - sigma * g.reshape(-1, X * Y * Z), 0.0)
+    d = [np.stack([M @ zi for zi in z]) for M in mats]
+    m = [float((np.abs(da) - va).max()) for da, va in zip(d, v)]
+    fnew = z.copy()
+    for M, da, ma, ta in zip(mats, d, m, t):
+        fnew = fnew + ta * np.stack([M.T @ (np.sign(di) * ma) for di in da])
+    return (fnew + beta * (fnew - fb)).reshape(f.shape)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["reference", "neumann"])
+@pytest.mark.parametrize("shp", [(2, 1, 7, 9, 11), (1, 1, 1, 1, 6), (1, 1, 5, 70, 3)])
+def test_fused_fdpg_step_matches_sparse_matrices(cuda, mode, shp):
+    print(f"[case] fdpg step {mode} {shp}", flush=True)
+    f = phantom.gaussian(shp, 61) + 0.5
+    g = phantom.gaussian(shp, 62)
+    sigma, v, t, beta = 0.3, (0.4, 0.2, 0.7), (0.05, 0.08, 0.03), 0.25
+    want = _fdpg_fp64(f.double().numpy(), g.double().numpy(), sigma, v, t, beta, mode)
+    got = tv.fdpg_step(f.to(cuda), g.to(cuda), sigma, v, t, beta, mode)
+    assert rel_l2(got, torch.from_numpy(want)) <= 1e-6
+    got_t = tv.fdpg_step_torch(f.double(), g.double(), sigma, v, t, beta, mode)
+    assert rel_l2(got_t, torch.from_numpy(want)) <= 1e-12
 
 
 def test_asd_pocs_step_schedule(monkeypatch):
@@ -284,5 +388,5 @@ def test_dtv_cp_without_regularisation_solves_the_data_term():
     back = lambda y: (A.T @ y.reshape(-1)).reshape(1, 1, 2, 3, 4)
     L = float(torch.linalg.matrix_norm(A, 2)) ** 2
     out = unroll.dtv_cp(torch.zeros_like(x), b, proj, back, cascades=2000, v=(0.0, 0.0, 0.0),
-                        mu=0.0, sigma=0.5 / L, tau=0.5 / L)
+                        mu=0.0, sigma=0.5 / L, tau=0.5 / L, fused=False)
     assert float((out - x).abs().max()) <= 1e-8
