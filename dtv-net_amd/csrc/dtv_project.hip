@@ -792,7 +792,7 @@ const char* dtv_strerror(int code) {
     case DTV_OK: return "success";
     case DTV_ERR_INVALID_ARGUMENT: return "invalid argument (sizes, batch, view count or geometry)";
     case DTV_ERR_NULL_POINTER: return "null pointer argument";
-    case DTV_ERR_TOO_LARGE: return "problem too large (32-bit per-volume indexing or the kernel's LDS budget)";
+    case DTV_ERR_TOO_LARGE: return "problem too large (32-bit per-volume indexing, the kernel's LDS budget or a fixed scratch capacity)";
     case DTV_ERR_HIP: return "HIP runtime error (launch or device selection failed)";
     case DTV_ERR_VIEW_COUNT: return "view count must be a multiple of 21 for the Beijing backprojector";
     default: return "unknown dtvproj error";

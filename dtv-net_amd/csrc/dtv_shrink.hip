@@ -31,6 +31,8 @@
 
 #include <math.h>
 
+#include <mutex>
+
 #include "dtv_stencil.h"
 #include "dtvproj.h"
 
@@ -288,24 +290,14 @@ unsigned flat_blocks(int64_t n) {
   return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
 }
 
-}  // namespace
-
-extern "C" {
-
-size_t dtv_l1ball_workspace_bytes(int32_t n_items) {
-  if (n_items < 1 || n_items > DTV_L1_MAX_ITEMS) return 0;
-  return sizeof(L1State) * DTV_L1_MAX_ITEMS +
-         sizeof(double) * 2 * (size_t)kL1Blocks * (size_t)n_items;
-}
-
-int dtv_l1ball(const dtv_l1_item* items, int32_t n_items, int32_t mode, double tol,
-               int32_t max_iter, void* workspace, size_t workspace_bytes, void* stream) {
-  if (!items || !workspace) return DTV_ERR_NULL_POINTER;
+// dtv_l1ball after the workspace checks: validates the items and enqueues
+// the init, Newton-step and apply launches on st (ws: state + partial sums).
+int run_l1ball(const dtv_l1_item* items, int32_t n_items, int32_t mode, double tol,
+               int32_t max_iter, void* ws, hipStream_t st) {
+  if (!items) return DTV_ERR_NULL_POINTER;
   if (n_items < 1 || n_items > DTV_L1_MAX_ITEMS || (mode != 0 && mode != 1) ||
       max_iter < 0 || max_iter > 100000 || !(tol >= 0.0))
     return DTV_ERR_INVALID_ARGUMENT;
-  if (workspace_bytes < dtv_l1ball_workspace_bytes(n_items)) return DTV_ERR_INVALID_ARGUMENT;
-  if ((uintptr_t)workspace % 8 != 0) return DTV_ERR_INVALID_ARGUMENT;
   L1Args A{};
   int64_t nmax = 0;
   for (int k = 0; k < n_items; ++k) {
@@ -320,9 +312,8 @@ int dtv_l1ball(const dtv_l1_item* items, int32_t n_items, int32_t mode, double t
   A.mode = mode;
   A.max_iter = max_iter;
   A.tol = tol;
-  hipStream_t st = (hipStream_t)stream;
-  L1State* state = (L1State*)workspace;
-  double* part = (double*)((char*)workspace + sizeof(L1State) * DTV_L1_MAX_ITEMS);
+  L1State* state = (L1State*)ws;
+  double* part = (double*)((char*)ws + sizeof(L1State) * DTV_L1_MAX_ITEMS);
   const unsigned nb = (unsigned)std::min<int64_t>(kL1Blocks, (nmax + 1023) / 1024);
   hipLaunchKernelGGL(k_l1_init, dim3(1), dim3(64), 0, st, state, n_items);
   if (hipGetLastError() != hipSuccess) return DTV_ERR_HIP;
@@ -334,6 +325,75 @@ int dtv_l1ball(const dtv_l1_item* items, int32_t n_items, int32_t mode, double t
   hipLaunchKernelGGL(k_l1_apply, dim3(std::min<unsigned>(flat_blocks(nmax), 4096), n_items),
                      dim3(256), 0, st, A, state);
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
+}
+
+}  // namespace
+
+// ---- allocation-free l1-ball entry point --------------------------------------
+//
+// dtv_l1ball_noalloc keeps the Newton state and the partial sums in this
+// file-scope device array instead of a caller workspace: room for
+// DTV_L1_MAX_ITEMS vectors of any length (the reduction uses at most
+// kL1Blocks workgroups per vector), 131,584 bytes, one copy per device for
+// the life of the process.  Host code must not pass the array's name to a
+// kernel: its device address comes from hipGetSymbolAddress, looked up once
+// per device with that device current.
+static __device__ __align__(256) unsigned char scratch[sizeof(L1State) * DTV_L1_MAX_ITEMS +
+                                                       sizeof(double) * 2 * kL1Blocks *
+                                                           DTV_L1_MAX_ITEMS];
+static_assert(sizeof(scratch) == 131584, "the documented size of the l1-ball scratch array");
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices];
+
+// The current device's address of `scratch`.
+int scratch_address(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return DTV_ERR_HIP;
+  if (dev < 0 || dev >= kMaxDevices) return DTV_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return DTV_ERR_HIP;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return DTV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dtv_l1ball_noalloc(const dtv_l1_item* items, int32_t n_items, int32_t mode, double tol,
+                       int32_t max_iter, void* stream) {
+  if (n_items > DTV_L1_MAX_ITEMS) return DTV_ERR_TOO_LARGE;  // beyond the scratch array
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc) return rc;
+  return run_l1ball(items, n_items, mode, tol, max_iter, ws, (hipStream_t)stream);
+}
+
+size_t dtv_l1ball_workspace_bytes(int32_t n_items) {
+  if (n_items < 1 || n_items > DTV_L1_MAX_ITEMS) return 0;
+  return sizeof(L1State) * DTV_L1_MAX_ITEMS +
+         sizeof(double) * 2 * (size_t)kL1Blocks * (size_t)n_items;
+}
+
+int dtv_l1ball(const dtv_l1_item* items, int32_t n_items, int32_t mode, double tol,
+               int32_t max_iter, void* workspace, size_t workspace_bytes, void* stream) {
+  if (!workspace) return DTV_ERR_NULL_POINTER;
+  if (n_items >= 1 && n_items <= DTV_L1_MAX_ITEMS &&
+      workspace_bytes < dtv_l1ball_workspace_bytes(n_items))
+    return DTV_ERR_INVALID_ARGUMENT;
+  if ((uintptr_t)workspace % 8 != 0) return DTV_ERR_INVALID_ARGUMENT;
+  return run_l1ball(items, n_items, mode, tol, max_iter, workspace, (hipStream_t)stream);
 }
 
 int dtv_shrink_step(const float* y, const float* g, const float* xprev, float* xnew,

@@ -90,6 +90,7 @@ _SIGS = {
                           ctypes.c_float, ctypes.c_float, c_float_p, c_int32, c_void_p],
     "dtv_l1ball": [c_void_p, c_int32, c_int32, c_double, c_int32, c_void_p, ctypes.c_size_t,
                    c_void_p],
+    "dtv_l1ball_noalloc": [c_void_p, c_int32, c_int32, c_double, c_int32, c_void_p],
     "dtv_shrink_step": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int64,
                         ctypes.c_float, ctypes.c_float, ctypes.c_float, c_void_p],
     "dtv_fdpg_step": [c_void_p, c_void_p, c_void_p, c_int32, c_int32_p, ctypes.c_float,

@@ -279,6 +279,18 @@ size_t dtv_l1ball_workspace_bytes(int32_t n_items);
 int dtv_l1ball(const dtv_l1_item* items, int32_t n_items, int32_t mode, double tol,
                int32_t max_iter, void* workspace, size_t workspace_bytes, void* stream);
 
+/* dtv_l1ball without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  No counterpart in the reference.  The state and
+ * partial sums live in a static device array of 131,584 bytes (room for
+ * DTV_L1_MAX_ITEMS vectors of any length), one per device, held until the
+ * process ends; the call allocates nothing.  n_items > DTV_L1_MAX_ITEMS
+ * returns DTV_ERR_TOO_LARGE.  Every call on one device shares that array
+ * (each call clears the state it needs first): calls that overlap on one
+ * device, e.g. on two streams, race; calls on one stream do not. */
+int dtv_l1ball_noalloc(const dtv_l1_item* items, int32_t n_items, int32_t mode, double tol,
+                       int32_t max_iter, void* stream);
+
 /* Soft threshold with FISTA momentum over n values (model/FISTA/FISTA.py:21-25):
  *   d = y - step * g;  xnew = sign(d) relu(|d| - thr);  ynew = xnew + beta (xnew - xprev)
  * xnew may be xprev; ynew must differ from xprev. */
