@@ -233,10 +233,11 @@ def asd_pocs(image, sino, project, backproject, cascades: int = 30, beta: float 
 @torch.no_grad()
 def dtv_cp(image, sino, project, backproject, cascades: int = 30, v=(1.0, 1.0, 1.0),
            mu: float = 1.0, sigma: float = 1e-7, tau: float = 1e-7, mode: str = "reference",
-           fused: bool = True):
+           fused: bool = True, refresh: int = 10):
     """Directional-TV Chambolle-Pock of model/PDHG/DTVCP.py:29-60.
 
-    Every 10 cascades the bounds t_a = sum|D_a f| are refreshed (:34-37).
+    Every ``refresh`` cascades (10, as the reference) the bounds
+    t_a = sum|D_a f| are refreshed (:34-37).
     Per cascade: res = A f_bar - sino; w = (w + sigma res)/(1 + sigma);
     r = B w; for each axis p_a' = p_a + v_a sigma D_a f_bar and
     p_a = p_a' - sigma sign(p_a') l1ball(|p_a'| / sigma, v_a t_a) (:41-46);
@@ -255,7 +256,7 @@ def dtv_cp(image, sino, project, backproject, cascades: int = 30, v=(1.0, 1.0, 1
     t = [0.0, 0.0, 0.0]
     axes = ("x", "y", "z")
     for k in range(cascades):
-        if k % 10 == 0:
+        if k % refresh == 0:
             t = [float(tv.D(f, a, mode).abs().sum()) for a in axes]
         res = project(fbar) - sino
         w = (w + sigma * res) / (1 + sigma)
