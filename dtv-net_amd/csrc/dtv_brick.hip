@@ -167,24 +167,39 @@ __device__ void stage_chunk(const Geo& g, int c0, int n, const double lo[3], con
     const double c[3] = {(t & 1) ? hi[0] : lo[0], (t & 2) ? hi[1] : lo[1],
                          (t & 4) ? hi[2] : lo[2]};
     double a = 0.0, b = 0.0;
-    const int ok = project_point(v, vp, c, a, b) ? 1 : 0;
+    int ok = project_point(v, vp, c, a, b) ? 1 : 0;
     const double w[3] = {c[0] - v.S[0], c[1] - v.S[1], c[2] - v.S[2]};
     const double dw = fabs(dot3(w, vp.n));
+    double amin = a, amax = a, bmin = b, bmax = b, wmin = dw, wmax = dw;
     // the 8 corners of a view are 8 consecutive lanes of one wave (n * 8
-    // lanes take part, so every aligned group of 8 is whole): gather them
-    double ca[8], cb[8], cw[8];
-    int okall = 1;
+    // lanes take part, so every aligned group of 8 is whole); corner
+    // t ^ (1 << k) is the other end of corner t's edge along axis k
+    double du[3], dv[3];
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      ca[s] = __shfl(a, s, 8);
-      cb[s] = __shfl(b, s, 8);
-      cw[s] = __shfl(dw, s, 8);
-      okall &= __shfl(ok, s, 8);
+    for (int k = 0; k < 3; ++k) {
+      du[k] = fabs(a - __shfl_xor(a, 1 << k, 64));
+      dv[k] = fabs(b - __shfl_xor(b, 1 << k, 64));
+    }
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        du[k] = fmax(du[k], __shfl_xor(du[k], off, 64));
+        dv[k] = fmax(dv[k], __shfl_xor(dv[k], off, 64));
+      }
+      amin = fmin(amin, __shfl_xor(amin, off, 64));
+      amax = fmax(amax, __shfl_xor(amax, off, 64));
+      bmin = fmin(bmin, __shfl_xor(bmin, off, 64));
+      bmax = fmax(bmax, __shfl_xor(bmax, off, 64));
+      wmin = fmin(wmin, __shfl_xor(wmin, off, 64));
+      wmax = fmax(wmax, __shfl_xor(wmax, off, 64));
+      ok &= __shfl_xor(ok, off, 64);
     }
     if (t == 0) {
       int rect[4];
       float wb;
-      box_view_stage(ca, cb, cw, okall != 0, lo, hi, g.nu, g.nv, rect, wb);
+      box_view_finish(amin, amax, bmin, bmax, wmin, wmax, du, dv, ok != 0, lo, hi, g.nu, g.nv,
+                      rect, wb);
       for (int k = 0; k < 4; ++k) st.rect[vi][k] = rect[k];
       st.vidx[vi] = va;
       if (bound) st.wbound[vi] = wb;

@@ -361,9 +361,11 @@ DTV_HD inline float weight_per_voxel_bound(bool all, const double du[3], const d
 
 // What the brick kernels stage per (box, view), from the projections of the
 // box [lo, hi]'s 8 corners (corner t at (t&1 ? hi : lo, t&2 ? hi : lo,
-// t&4 ? hi : lo) per axis): ca[t], cb[t] its continuous pixel coordinates
-// (project_point), cw[t] = |(corner - S).n| its depth, ok = every corner
-// strictly in front of the source.
+// t&4 ? hi : lo) per axis), reduced over the corners: amin..bmax the range
+// of their continuous pixel coordinates (project_point), wmin/wmax of their
+// depths |(corner - S).n|, du[k]/dv[k] the largest projected length of the
+// box's edges along axis k (corner t to corner t ^ (1 << k)), ok = every
+// corner strictly in front of the source.
 //  * rect = {u0, u1, v0, v1} (inclusive, u1 < u0 when empty): the pixels i
 //    in [floor(amin), ceil(amax)] (likewise v), clipped to the detector --
 //    pixel i's ray passes through coordinate i exactly, so these are the
@@ -374,24 +376,13 @@ DTV_HD inline float weight_per_voxel_bound(bool all, const double du[3], const d
 //    source or the depth ratio across the box exceeds 2 (the box next to the
 //    source, where the bound would be loose): the matched adjoint then
 //    accumulates that view in FP32 instead of fixed point.
-DTV_HD inline void box_view_stage(const double ca[8], const double cb[8], const double cw[8],
-                                  bool ok, const double lo[3], const double hi[3], int nu,
-                                  int nv, int rect[4], float& wbound) {
-  double amin = ca[0], amax = ca[0], bmin = cb[0], bmax = cb[0], wmin = cw[0], wmax = cw[0];
-  double du[3] = {0.0, 0.0, 0.0}, dv[3] = {0.0, 0.0, 0.0};
-  for (int t = 0; t < 8; ++t) {
-    amin = fmin(amin, ca[t]);
-    amax = fmax(amax, ca[t]);
-    bmin = fmin(bmin, cb[t]);
-    bmax = fmax(bmax, cb[t]);
-    wmin = fmin(wmin, cw[t]);
-    wmax = fmax(wmax, cw[t]);
-    // corner t ^ (1 << k) is the other end of corner t's edge along axis k
-    for (int k = 0; k < 3; ++k) {
-      du[k] = fmax(du[k], fabs(ca[t] - ca[t ^ (1 << k)]));
-      dv[k] = fmax(dv[k], fabs(cb[t] - cb[t ^ (1 << k)]));
-    }
-  }
+// The kernels reduce over the corners with wave shuffles (one corner per
+// lane) and call this; box_view_stage reduces with loops and calls it.
+DTV_HD inline void box_view_finish(double amin, double amax, double bmin, double bmax,
+                                   double wmin, double wmax, const double du[3],
+                                   const double dv[3], bool ok, const double lo[3],
+                                   const double hi[3], int nu, int nv, int rect[4],
+                                   float& wbound) {
   int u0 = 0, u1 = nu - 1, v0 = 0, v1 = nv - 1;
   if (ok) {
     const double fu0 = floor(amin), fu1 = ceil(amax);
@@ -409,6 +400,28 @@ DTV_HD inline void box_view_stage(const double ca[8], const double cb[8], const 
   wbound = (ok && wmin > 0.0 && wmax <= 2.0 * wmin)
                ? weight_per_voxel_bound(true, du, dv, wmin, wmax, ext, u1 - u0 + 1, v1 - v0 + 1)
                : INFINITY;
+}
+
+// box_view_finish from the 8 corners' values (ca, cb: pixel coordinates,
+// cw: depths), reduced with loops.
+DTV_HD inline void box_view_stage(const double ca[8], const double cb[8], const double cw[8],
+                                  bool ok, const double lo[3], const double hi[3], int nu,
+                                  int nv, int rect[4], float& wbound) {
+  double amin = ca[0], amax = ca[0], bmin = cb[0], bmax = cb[0], wmin = cw[0], wmax = cw[0];
+  double du[3] = {0.0, 0.0, 0.0}, dv[3] = {0.0, 0.0, 0.0};
+  for (int t = 0; t < 8; ++t) {
+    amin = fmin(amin, ca[t]);
+    amax = fmax(amax, ca[t]);
+    bmin = fmin(bmin, cb[t]);
+    bmax = fmax(bmax, cb[t]);
+    wmin = fmin(wmin, cw[t]);
+    wmax = fmax(wmax, cw[t]);
+    for (int k = 0; k < 3; ++k) {
+      du[k] = fmax(du[k], fabs(ca[t] - ca[t ^ (1 << k)]));
+      dv[k] = fmax(dv[k], fabs(cb[t] - cb[t ^ (1 << k)]));
+    }
+  }
+  box_view_finish(amin, amax, bmin, bmax, wmin, wmax, du, dv, ok, lo, hi, nu, nv, rect, wbound);
 }
 
 }  // namespace dtv

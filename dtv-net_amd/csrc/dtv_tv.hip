@@ -139,6 +139,200 @@ __global__ __launch_bounds__(256) void k_tv_descent(const float* __restrict__ f,
   out[off + p.i] = fb[p.i] - step * tv_grad_at<true>(v, fb, p, eps);
 }
 
+// ---- z-marching form of the TV gradient ---------------------------------------
+//
+// The gradient at i needs q = D f / n at i and at its three back neighbours
+// i - s_a.  Evaluated per output voxel that is four q's (and 13 loads of f);
+// here every q is evaluated once: a workgroup of 64 x 8 lanes covers the x-y
+// region [x0 - 1, x0 + 63) x [y0 - 1, y0 + 7) and marches z through a chunk
+// of slices.  Per slice each lane evaluates q at its point, q_x and q_y go to
+// LDS for the lanes at x + 1 and y + 1, q_z stays in a register for the next
+// slice, and the 63 x 7 lanes off the region's low edges produce their voxel.
+// That needs i - s_1 to be the voxel at y - 1: always in Neumann mode; in the
+// reference mode (s_1 = Y) when X == Y, where at y = 0 the back neighbour is
+// (x, Y - 1, z - 1), evaluated directly.  Reference mode with X != Y takes
+// the per-voxel kernels above.
+constexpr int kMX = 64, kMY = 8, kMZ = 32;  // march region and z chunk
+
+__host__ __device__ inline bool march_ok(const Vol3& v) {
+  return (v.mode == 0 || v.nx == v.ny) && v.n < (1ll << 31);
+}
+__host__ __device__ inline int march_tiles_x(const Vol3& v) { return (v.nx + kMX - 2) / (kMX - 1); }
+__host__ __device__ inline int march_tiles_y(const Vol3& v) { return (v.ny + kMY - 2) / (kMY - 1); }
+__host__ __device__ inline int march_chunks(const Vol3& v) { return (v.nz + kMZ - 1) / kMZ; }
+__host__ __device__ inline int march_blocks(const Vol3& v) {
+  return march_tiles_x(v) * march_tiles_y(v) * march_chunks(v);
+}
+
+// q = D f / n at a point from its value fc and those of its three forward
+// neighbours (read only where the point has them).
+template <bool kAsd>
+__device__ __forceinline__ void qvec_vals(const Vol3& v, bool hx, bool hy, bool hz, float fc,
+                                          float fx, float fy, float fz, float eps, float q[3]) {
+  const float edge = v.mode == 0 ? 0.f : -fc;
+  const float dx = hx ? fx - fc : edge, dy = hy ? fy - fc : edge, dz = hz ? fz - fc : edge;
+  const float ss = dx * dx + dy * dy + dz * dz;
+  const float r = kAsd ? 1.f / (sqrtf(ss) + eps) : rsqrtf(ss + eps * eps);
+  q[0] = dx * r;
+  q[1] = dy * r;
+  q[2] = dz * r;
+}
+
+// Marches block b of volume f (march_blocks(v) blocks per volume).  Per
+// voxel the lane produces, out.load(i) fetches what the update reads besides
+// f (issued before the slice's barrier) and out.store(i, fc, gtv, pre)
+// writes it; gtv = sum_a D_a^T q_a and fc = f(i).  The slice's f values at
+// z + 1 carry to the next slice, so each slice loads three values per lane;
+// offsets are 32-bit (volumes below 2^31 voxels) from the volume's base, and
+// the q's of two consecutive slices use alternate LDS buffers, so one
+// barrier per slice orders them.
+template <bool kAsd, typename Out>
+__device__ __forceinline__ void tv_march(const Vol3& v, const float* __restrict__ f, float eps,
+                                         int b, Out& out) {
+  __shared__ float sq[2][2][kMY][kMX];
+  const int txy = march_tiles_x(v) * march_tiles_y(v);
+  const int chunk = b / txy, t = b - chunk * txy;
+  const int ty = t / march_tiles_x(v), tx = t - ty * march_tiles_x(v);
+  const int lx = threadIdx.x & (kMX - 1), ly = threadIdx.x / kMX;
+  const int x = tx * (kMX - 1) - 1 + lx, y = ty * (kMY - 1) - 1 + ly;
+  const bool in_xy = x >= 0 && x < v.nx && y >= 0 && y < v.ny;
+  const bool out_lane = in_xy && lx >= 1 && ly >= 1;
+  const int z0 = chunk * kMZ, z1 = min(v.nz, z0 + kMZ);
+  const unsigned s1 = (unsigned)v.s1, s2 = (unsigned)v.s2, n = (unsigned)v.n;
+  const unsigned row = (unsigned)(y * v.nx + x);
+  const bool hx = x != v.nx - 1;
+  const bool ref = v.mode != 0;
+  // has_next on D_y: Neumann y != Y - 1, reference i < n - s_1
+  auto hy_at = [&](unsigned i) { return ref ? i < n - s1 : y != v.ny - 1; };
+  float qzp = 0.f, fc = 0.f;
+  if (in_xy) {
+    const unsigned i0 = (unsigned)z0 * s2 + row;
+    fc = f[i0];
+    if (z0 > 0) {
+      const unsigned ib = i0 - s2;
+      float q[3];
+      qvec_vals<kAsd>(v, hx, hy_at(ib), true, f[ib], hx ? f[ib + 1] : 0.f,
+                      hy_at(ib) ? f[ib + s1] : 0.f, fc, eps, q);
+      qzp = q[2];
+    }
+  }
+  // the values of slice z are loaded during slice z - 1 (software pipelining:
+  // their latency overlaps the previous slice's barrier and update)
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  typename Out::Pre pre{};
+  auto fetch = [&](int z, float& gx, float& gy, float& gz, typename Out::Pre& gp) {
+    const unsigned i = (unsigned)z * s2 + row;
+    if (hx) gx = f[i + 1];
+    if (hy_at(i)) gy = f[i + s1];
+    if (z != v.nz - 1) gz = f[i + s2];
+    if (out_lane) gp = out.load(i);
+  };
+  if (in_xy && z0 < z1) fetch(z0, fx, fy, fz, pre);
+  for (int z = z0; z < z1; ++z) {
+    const unsigned i = (unsigned)z * s2 + row;
+    const bool hy = hy_at(i), hz = z != v.nz - 1;
+    float nfx = 0.f, nfy = 0.f, nfz = 0.f;
+    typename Out::Pre npre{};
+    if (in_xy && z + 1 < z1) fetch(z + 1, nfx, nfy, nfz, npre);
+    float q[3] = {0.f, 0.f, 0.f};
+    if (in_xy) qvec_vals<kAsd>(v, hx, hy, hz, fc, fx, fy, fz, eps, q);
+    const int buf = z & 1;
+    sq[buf][0][ly][lx] = q[0];
+    sq[buf][1][ly][lx] = q[1];
+    __syncthreads();
+    if (out_lane) {
+      float gtv = -(q[0] + q[1] + q[2]);
+      if (x > 0) gtv += sq[buf][0][ly][lx - 1];
+      if (y > 0) {
+        gtv += sq[buf][1][ly - 1][lx];
+      } else if (ref && z > 0) {
+        // reference D_y with X == Y: row i - Y is (x, Y - 1, z - 1)
+        const unsigned ib = i - s1;
+        const bool hyb = hy_at(ib);
+        float qb[3];
+        qvec_vals<kAsd>(v, hx, hyb, true, f[ib], hx ? f[ib + 1] : 0.f, hyb ? f[ib + s1] : 0.f,
+                        f[ib + s2], eps, qb);
+        gtv += qb[1];
+      }
+      if (z > 0) gtv += qzp;
+      out.store(i, fc, gtv, pre);
+    }
+    qzp = q[2];
+    fc = fz;
+    fx = nfx;
+    fy = nfy;
+    fz = nfz;
+    pre = npre;
+  }
+}
+
+// FISTA-TV update of one volume (base pointers of this instance / item)
+struct FistaOut {
+  struct Pre {
+    float g, xp;
+  };
+  const float* g;
+  const float* xprev;
+  float* xnew;
+  float* ynew;
+  float step, lam, beta;
+  int nonneg;
+  __device__ __forceinline__ Pre load(unsigned i) const { return Pre{g[i], xprev[i]}; }
+  __device__ __forceinline__ void store(unsigned i, float yv, float gtv, const Pre& p) const {
+    float x = yv - step * (p.g + lam * gtv);
+    if (nonneg) x = fmaxf(x, 0.f);
+    xnew[i] = x;
+    ynew[i] = x + beta * (x - p.xp);
+  }
+};
+
+__device__ __forceinline__ FistaOut fista_out(const dtv_tv_item& it, int64_t off) {
+  return FistaOut{it.g + off, it.xprev + off, it.xnew + off, it.ynew + off, it.step, it.lam,
+                  it.beta, it.nonneg};
+}
+
+// FISTA-TV step, one instance: blockIdx.x = march block, blockIdx.y = batch item
+__global__ __launch_bounds__(512) void k_tv_fista_march(dtv_tv_item it, Vol3 v) {
+  const int64_t off = (int64_t)blockIdx.y * v.n;
+  FistaOut o = fista_out(it, off);
+  tv_march<false>(v, it.y + off, it.eps, (int)blockIdx.x, o);
+}
+
+// uneven instances: instance k owns march blocks [off[k], off[k+1])
+__global__ __launch_bounds__(512) void k_tv_fista_march_multi(
+    const dtv_tv_item* __restrict__ items, int mode, TvOffsets offs) {
+  const int b = (int)blockIdx.x;
+  int lo = 0, hi = offs.n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offs.off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const dtv_tv_item it = items[lo];
+  const Vol3 v = make_vol3(it.nx, it.ny, it.nz, mode);
+  FistaOut o = fista_out(it, 0);
+  tv_march<false>(v, it.y, it.eps, b - offs.off[lo], o);
+}
+
+struct DescentOut {
+  struct Pre {};
+  float* out;
+  float step;
+  __device__ __forceinline__ Pre load(unsigned) const { return Pre{}; }
+  __device__ __forceinline__ void store(unsigned i, float fv, float gtv, const Pre&) const {
+    out[i] = fv - step * gtv;
+  }
+};
+
+// ASD-POCS TV descent: blockIdx.x = march block, blockIdx.y = batch item
+__global__ __launch_bounds__(512) void k_tv_descent_march(const float* __restrict__ f,
+                                                          float* __restrict__ out, Vol3 v,
+                                                          float step, float eps) {
+  const int64_t off = (int64_t)blockIdx.y * v.n;
+  DescentOut o{out + off, step};
+  tv_march<true>(v, f + off, eps, (int)blockIdx.x, o);
+}
+
 // one PDHG step of the directional-TV unroll, after A and A^T.  Dual half:
 //   p_a <- clamp(p_a + sigma * d_a(fbar), -lam_a, lam_a)
 // (p: three planes [3][batch][Z][Y][X], updated in place; each lane reads
@@ -211,8 +405,12 @@ int dtv_tv_fista_step(const float* y, const float* g, const float* xprev, float*
   if (rc) return rc;
   dtv_tv_item it{y, g, xprev, xnew, ynew, volume_size[0], volume_size[1], volume_size[2],
                  nonneg ? 1 : 0, step, lam, eps, beta};
-  hipLaunchKernelGGL(k_tv_fista_one, vol3_grid(v, batch), dim3(256), 0, (hipStream_t)stream,
-                     it, v);
+  if (march_ok(v))
+    hipLaunchKernelGGL(k_tv_fista_march, dim3((unsigned)march_blocks(v), batch), dim3(512), 0,
+                       (hipStream_t)stream, it, v);
+  else
+    hipLaunchKernelGGL(k_tv_fista_one, vol3_grid(v, batch), dim3(256), 0, (hipStream_t)stream,
+                       it, v);
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
 }
 
@@ -223,19 +421,29 @@ int dtv_tv_fista_step_multi(const dtv_tv_item* items_host, const dtv_tv_item* it
     return DTV_ERR_INVALID_ARGUMENT;
   TvOffsets offs;
   offs.n = n_items;
-  int64_t total = 0;
+  bool march = true;
   for (int k = 0; k < n_items; ++k) {
     const dtv_tv_item& it = items_host[k];
     if (!it.y || !it.g || !it.xprev || !it.xnew || !it.ynew) return DTV_ERR_NULL_POINTER;
     if (it.nx < 1 || it.ny < 1 || it.nz < 1) return DTV_ERR_INVALID_ARGUMENT;
+    march = march && march_ok(make_vol3(it.nx, it.ny, it.nz, mode));
+  }
+  int64_t total = 0;
+  for (int k = 0; k < n_items; ++k) {
+    const dtv_tv_item& it = items_host[k];
     const Vol3 v = make_vol3(it.nx, it.ny, it.nz, mode);
     offs.off[k] = (int)total;
-    total += (int64_t)v.tiles_x * ((v.ny + kTileY - 1) / kTileY) * v.nz;
+    total += march ? (int64_t)march_blocks(v)
+                   : (int64_t)v.tiles_x * ((v.ny + kTileY - 1) / kTileY) * v.nz;
     if (total > 0x7fffffff) return DTV_ERR_TOO_LARGE;
   }
   offs.off[n_items] = (int)total;
-  hipLaunchKernelGGL(k_tv_fista_multi, dim3((unsigned)total), dim3(256), 0, (hipStream_t)stream,
-                     items_dev, mode, offs);
+  if (march)
+    hipLaunchKernelGGL(k_tv_fista_march_multi, dim3((unsigned)total), dim3(512), 0,
+                       (hipStream_t)stream, items_dev, mode, offs);
+  else
+    hipLaunchKernelGGL(k_tv_fista_multi, dim3((unsigned)total), dim3(256), 0,
+                       (hipStream_t)stream, items_dev, mode, offs);
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
 }
 
@@ -246,8 +454,12 @@ int dtv_tv_descent(const float* f, float* out, int32_t batch, const int32_t volu
   int rc = vol_args(volume_size, batch, mode, v);
   if (rc) return rc;
   if (!(eps > 0.f) || f == out) return DTV_ERR_INVALID_ARGUMENT;
-  hipLaunchKernelGGL(k_tv_descent, vol3_grid(v, batch), dim3(256), 0, (hipStream_t)stream, f,
-                     out, v, step, eps);
+  if (march_ok(v))
+    hipLaunchKernelGGL(k_tv_descent_march, dim3((unsigned)march_blocks(v), batch), dim3(512), 0,
+                       (hipStream_t)stream, f, out, v, step, eps);
+  else
+    hipLaunchKernelGGL(k_tv_descent, vol3_grid(v, batch), dim3(256), 0, (hipStream_t)stream, f,
+                       out, v, step, eps);
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
 }
 

@@ -169,9 +169,14 @@ def test_asd_pocs_without_tv_steps_is_landweber_to_the_solution():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["neumann", "reference"])
-def test_fused_fista_tv_step(cuda, mode):
-    print(f"[case] fused_fista_tv_step {mode}", flush=True)
-    shp = (2, 1, 13, 11, 17)
+@pytest.mark.parametrize("shp", [(2, 1, 13, 11, 17), (1, 1, 20, 16, 16), (1, 1, 35, 70, 70),
+                                 (2, 1, 3, 9, 130)])
+def test_fused_fista_tv_step(cuda, mode, shp):
+    """The fused step vs the torch formulation (whose D and D^T meet the
+    sparse matrices); the shapes cover several x-y tiles and z chunks of the
+    z-marching kernel (Neumann, and reference mode with X == Y) and the
+    per-voxel kernel (reference mode with X != Y)."""
+    print(f"[case] fused_fista_tv_step {mode} {shp}", flush=True)
     y = phantom.gaussian(shp, 1)
     g = phantom.gaussian(shp, 2)
     xp = phantom.gaussian(shp, 3)
@@ -189,7 +194,7 @@ def test_fused_fista_tv_step(cuda, mode):
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["reference", "neumann"])
 @pytest.mark.parametrize("shp", [(2, 1, 13, 11, 17), (1, 1, 1, 1, 5), (1, 1, 9, 1, 1),
-                                 (3, 1, 4, 6, 5)])
+                                 (3, 1, 4, 6, 5), (1, 1, 20, 16, 16), (2, 1, 35, 70, 70)])
 def test_fused_tv_descent_matches_oracle(cuda, mode, shp):
     print(f"[case] tv_descent {mode} {shp}", flush=True)
     f = phantom.gaussian(shp, 21)
