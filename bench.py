@@ -314,7 +314,7 @@ def roofline_record(dom, dom_t, dom_b, acct, t_fwd, t_adj, med_fwd, med_adj, log
                 pmc_source="profiles/pmc_sparse_256.json")
 
 
-LEGS = ("beijing", "batched", "pdhg512")
+LEGS = ("beijing", "batched", "pdhg512", "dense")
 
 
 def main():
@@ -355,9 +355,16 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda", 0)
     if args.leg is not None:
+        def dense_leg():
+            g = configs.sparse_256()
+            vol = phantom.phantom((256, 256, 256), 0.5, 20, seed=2).to(dev)
+            return dense_inputs(g, ops.forward(vol, g), roofline.pair_bytes(g),
+                                max(1, args.steps), log)
+
         res = {"beijing": lambda: beijing_loop(max(1, args.steps), log),
                "batched": lambda: batched_unroll(log),
-               "pdhg512": lambda: pdhg_512(max(1, args.steps), log)}[args.leg]()
+               "pdhg512": lambda: pdhg_512(max(1, args.steps), log),
+               "dense": dense_leg}[args.leg]()
         print(json.dumps({"leg": args.leg, "result": res}), flush=True)
         return
     _capi.lib()
