@@ -298,22 +298,6 @@ __global__ __launch_bounds__(512) void k_tv_fista_march(dtv_tv_item it, Vol3 v) 
   tv_march<false>(v, it.y + off, it.eps, (int)blockIdx.x, o);
 }
 
-// uneven instances: instance k owns march blocks [off[k], off[k+1])
-__global__ __launch_bounds__(512) void k_tv_fista_march_multi(
-    const dtv_tv_item* __restrict__ items, int mode, TvOffsets offs) {
-  const int b = (int)blockIdx.x;
-  int lo = 0, hi = offs.n;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (offs.off[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  const dtv_tv_item it = items[lo];
-  const Vol3 v = make_vol3(it.nx, it.ny, it.nz, mode);
-  FistaOut o = fista_out(it, 0);
-  tv_march<false>(v, it.y, it.eps, b - offs.off[lo], o);
-}
-
 struct DescentOut {
   struct Pre {};
   float* out;
@@ -331,6 +315,247 @@ __global__ __launch_bounds__(512) void k_tv_descent_march(const float* __restric
   const int64_t off = (int64_t)blockIdx.y * v.n;
   DescentOut o{out + off, step};
   tv_march<true>(v, f + off, eps, (int)blockIdx.x, o);
+}
+
+// ---- z march, four voxels per lane ------------------------------------------------
+//
+// With X a multiple of 4 (and 16-byte aligned arrays) each lane of the march
+// takes four consecutive x voxels as one float4: a 64 x 8 workgroup covers
+// [x0, x0 + 256) x [y0 - 1, y0 + 7); the lanes of one wave are one row, so
+// q_x of the voxel before a quad and f of the voxel after it come from the
+// neighbouring lanes by shuffles (the first lane evaluates that q_x itself,
+// the last lane loads that f), and only q_y goes through LDS.
+__host__ __device__ inline int march4_tiles_x(const Vol3& v) { return (v.nx + 255) / 256; }
+// taken when X is a multiple of 4 and the 256-wide tiles keep at least as
+// many lanes busy as the one-voxel march's 63-wide tiles (up to 5%): a row
+// of 124 voxels, say, would leave half of a wave idle
+__host__ __device__ inline bool march4_ok(const Vol3& v) {
+  if (!march_ok(v) || v.nx % 4 != 0) return false;
+  const float u4 = (float)v.nx / (256.f * (float)march4_tiles_x(v));
+  const float u1 = (float)v.nx / (63.f * (float)((v.nx + 62) / 63));
+  return u4 >= 0.95f * u1;
+}
+__host__ __device__ inline int march4_blocks(const Vol3& v) {
+  return march4_tiles_x(v) * march_tiles_y(v) * march_chunks(v);
+}
+
+__device__ __forceinline__ float4 ld4(const float* __restrict__ f, unsigned i) {
+  return *reinterpret_cast<const float4*>(f + i);
+}
+__device__ __forceinline__ float el(const float4& a, int e) {
+  return e == 0 ? a.x : (e == 1 ? a.y : (e == 2 ? a.z : a.w));
+}
+
+// q of a quad's four voxels from its values c, the value after it (xa, used
+// when the last voxel has an x neighbour: hx3), the quad one step along y
+// (yn, when hy) and along z (zn, when hz)
+template <bool kAsd>
+__device__ __forceinline__ void qvec4(bool ref, bool hx3, bool hy, bool hz, const float4& c,
+                                      float xa, const float4& yn, const float4& zn, float eps,
+                                      float qx[4], float qy[4], float qz[4]) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const float ce = el(c, e);
+    const float edge = ref ? -ce : 0.f;
+    const float xn = e < 3 ? el(c, e + 1) : xa;
+    const float dx = (e < 3 || hx3) ? xn - ce : edge;
+    const float dy = hy ? el(yn, e) - ce : edge;
+    const float dz = hz ? el(zn, e) - ce : edge;
+    const float ss = dx * dx + dy * dy + dz * dz;
+    const float r = kAsd ? 1.f / (sqrtf(ss) + eps) : rsqrtf(ss + eps * eps);
+    qx[e] = dx * r;
+    qy[e] = dy * r;
+    qz[e] = dz * r;
+  }
+}
+
+template <bool kAsd, typename Out>
+__device__ __forceinline__ void tv_march4(const Vol3& v, const float* __restrict__ f, float eps,
+                                          int b, Out& out) {
+  __shared__ float4 sqy[2][kMY][kMX];
+  const int tq = march4_tiles_x(v);
+  const int txy = tq * march_tiles_y(v);
+  const int chunk = b / txy, t = b - chunk * txy;
+  const int ty = t / tq, tx = t - ty * tq;
+  const int lx = threadIdx.x & (kMX - 1), ly = threadIdx.x / kMX;
+  const int x = tx * 256 + 4 * lx, y = ty * (kMY - 1) - 1 + ly;
+  const bool in_xy = x < v.nx && y >= 0 && y < v.ny;
+  const bool out_lane = in_xy && ly >= 1;
+  const int z0 = chunk * kMZ, z1 = min(v.nz, z0 + kMZ);
+  const unsigned s1 = (unsigned)v.s1, s2 = (unsigned)v.s2, n = (unsigned)v.n;
+  const unsigned row = in_xy ? (unsigned)(y * v.nx + x) : 0u;
+  const bool hx3 = x + 4 < v.nx;
+  const bool ref = v.mode != 0;
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto hy_at = [&](unsigned i) { return ref ? i < n - s1 : y != v.ny - 1; };
+  float qzp[4] = {0.f, 0.f, 0.f, 0.f};
+  float4 c = zero4;
+  if (in_xy) {
+    const unsigned i0 = (unsigned)z0 * s2 + row;
+    c = ld4(f, i0);
+    if (z0 > 0) {
+      const unsigned ib = i0 - s2;
+      const bool hyb = hy_at(ib);
+      float qx[4], qy[4];
+      qvec4<kAsd>(ref, hx3, hyb, true, ld4(f, ib), hx3 ? f[ib + 4] : 0.f,
+                  hyb ? ld4(f, ib + s1) : zero4, c, eps, qx, qy, qzp);
+    }
+  }
+  // the values of slice z are loaded during slice z - 1
+  float4 yn = zero4, zn = zero4;
+  float xa = 0.f;
+  typename Out::Pre pre{};
+  auto fetch = [&](int z, float4& gy, float4& gz, float& gx, typename Out::Pre& gp) {
+    const unsigned i = (unsigned)z * s2 + row;
+    if (hy_at(i)) gy = ld4(f, i + s1);
+    if (z != v.nz - 1) gz = ld4(f, i + s2);
+    if (lx == kMX - 1 && hx3) gx = f[i + 4];
+    if (out_lane) gp = out.load(i);
+  };
+  if (in_xy && z0 < z1) fetch(z0, yn, zn, xa, pre);
+  for (int z = z0; z < z1; ++z) {
+    const unsigned i = (unsigned)z * s2 + row;
+    const bool hy = in_xy && hy_at(i), hz = z != v.nz - 1;
+    float4 nyn = zero4, nzn = zero4;
+    float nxa = 0.f;
+    typename Out::Pre npre{};
+    if (in_xy && z + 1 < z1) fetch(z + 1, nyn, nzn, nxa, npre);
+    // the value after the quad: the next lane's first (the last lane loaded it)
+    const float after = __shfl_down(c.x, 1, kMX);
+    float qx[4] = {0.f, 0.f, 0.f, 0.f}, qy[4] = {0.f, 0.f, 0.f, 0.f}, qz[4] = {0.f, 0.f, 0.f, 0.f};
+    if (in_xy)
+      qvec4<kAsd>(ref, hx3, hy, hz, c, lx == kMX - 1 ? xa : after, yn, zn, eps, qx, qy, qz);
+    // q_x of the voxel before the quad: the previous lane's last, or for the
+    // first lane of the row evaluated here
+    float qx_before = __shfl_up(qx[3], 1, kMX);
+    if (lx == 0 && out_lane && x > 0) {
+      const unsigned ia = i - 1;
+      const bool hya = hy_at(ia);
+      const float fa = f[ia];
+      const float edge = ref ? -fa : 0.f;
+      const float dx = c.x - fa;
+      const float dy = hya ? f[ia + s1] - fa : edge;
+      const float dz = hz ? f[ia + s2] - fa : edge;
+      const float ss = dx * dx + dy * dy + dz * dz;
+      qx_before = dx * (kAsd ? 1.f / (sqrtf(ss) + eps) : rsqrtf(ss + eps * eps));
+    }
+    const int buf = z & 1;
+    sqy[buf][ly][lx] = make_float4(qy[0], qy[1], qy[2], qy[3]);
+    __syncthreads();
+    if (out_lane) {
+      float qyb[4] = {0.f, 0.f, 0.f, 0.f};
+      bool has_yb = false;
+      if (y > 0) {
+        const float4 w = sqy[buf][ly - 1][lx];
+        qyb[0] = w.x;
+        qyb[1] = w.y;
+        qyb[2] = w.z;
+        qyb[3] = w.w;
+        has_yb = true;
+      } else if (ref && z > 0) {
+        // reference D_y with X == Y: row i - Y is (x, Y - 1, z - 1)
+        const unsigned ib = i - s1;
+        float bx[4], bz[4];
+        qvec4<kAsd>(ref, hx3, hy_at(ib), true, ld4(f, ib), hx3 ? f[ib + 4] : 0.f, c,
+                    ld4(f, ib + s2), eps, bx, qyb, bz);
+        has_yb = true;
+      }
+      float gtv[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float gv = -(qx[e] + qy[e] + qz[e]);
+        if (e > 0) gv += qx[e - 1];
+        else if (x > 0) gv += qx_before;
+        if (has_yb) gv += qyb[e];
+        if (z > 0) gv += qzp[e];
+        gtv[e] = gv;
+      }
+      out.store4(i, c, gtv, pre);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) qzp[e] = qz[e];
+    c = zn;
+    yn = nyn;
+    zn = nzn;
+    xa = nxa;
+    pre = npre;
+  }
+}
+
+struct FistaOut4 {
+  struct Pre {
+    float4 g, xp;
+  };
+  const float* g;
+  const float* xprev;
+  float* xnew;
+  float* ynew;
+  float step, lam, beta;
+  int nonneg;
+  __device__ __forceinline__ Pre load(unsigned i) const { return Pre{ld4(g, i), ld4(xprev, i)}; }
+  __device__ __forceinline__ void store4(unsigned i, const float4& yv, const float gtv[4],
+                                         const Pre& p) const {
+    float x[4], yy[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float xe = el(yv, e) - step * (el(p.g, e) + lam * gtv[e]);
+      if (nonneg) xe = fmaxf(xe, 0.f);
+      x[e] = xe;
+      yy[e] = xe + beta * (xe - el(p.xp, e));
+    }
+    *reinterpret_cast<float4*>(xnew + i) = make_float4(x[0], x[1], x[2], x[3]);
+    *reinterpret_cast<float4*>(ynew + i) = make_float4(yy[0], yy[1], yy[2], yy[3]);
+  }
+};
+
+__device__ __forceinline__ FistaOut4 fista_out4(const dtv_tv_item& it, int64_t off) {
+  return FistaOut4{it.g + off, it.xprev + off, it.xnew + off, it.ynew + off, it.step, it.lam,
+                   it.beta, it.nonneg};
+}
+
+__global__ __launch_bounds__(512) void k_tv_fista_march4(dtv_tv_item it, Vol3 v) {
+  const int64_t off = (int64_t)blockIdx.y * v.n;
+  FistaOut4 o = fista_out4(it, off);
+  tv_march4<false>(v, it.y + off, it.eps, (int)blockIdx.x, o);
+}
+
+// uneven instances: instance k owns march blocks [off[k], off[k+1]).  All
+// take the one-voxel march: one kernel holding both march bodies carries the
+// larger one's registers and LDS for every workgroup, which measured slower
+// on the 16-instance workload than the one-voxel march alone.
+__global__ __launch_bounds__(512) void k_tv_fista_march_multi(
+    const dtv_tv_item* __restrict__ items, int mode, TvOffsets offs) {
+  const int b = (int)blockIdx.x;
+  int lo = 0, hi = offs.n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offs.off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const dtv_tv_item it = items[lo];
+  const Vol3 v = make_vol3(it.nx, it.ny, it.nz, mode);
+  FistaOut o = fista_out(it, 0);
+  tv_march<false>(v, it.y, it.eps, b - offs.off[lo], o);
+}
+
+struct DescentOut4 {
+  struct Pre {};
+  float* out;
+  float step;
+  __device__ __forceinline__ Pre load(unsigned) const { return Pre{}; }
+  __device__ __forceinline__ void store4(unsigned i, const float4& fv, const float gtv[4],
+                                         const Pre&) const {
+    *reinterpret_cast<float4*>(out + i) = make_float4(fv.x - step * gtv[0], fv.y - step * gtv[1],
+                                                      fv.z - step * gtv[2], fv.w - step * gtv[3]);
+  }
+};
+
+__global__ __launch_bounds__(512) void k_tv_descent_march4(const float* __restrict__ f,
+                                                           float* __restrict__ out, Vol3 v,
+                                                           float step, float eps) {
+  const int64_t off = (int64_t)blockIdx.y * v.n;
+  DescentOut4 o{out + off, step};
+  tv_march4<true>(v, f + off, eps, (int)blockIdx.x, o);
 }
 
 // one PDHG step of the directional-TV unroll, after A and A^T.  Dual half:
@@ -379,6 +604,135 @@ __global__ __launch_bounds__(256) void k_pdhg_primal(float* __restrict__ f,
   fbar[i] = 2.f * fn - fo;
 }
 
+// ---- four voxels per lane --------------------------------------------------------
+//
+// When X (and in reference mode Y) is a multiple of 4, rows and the D_y / D_z
+// strides keep 16-byte alignment, so each lane takes four consecutive x
+// voxels with float4 loads and stores: workgroups of 64 x 4 lanes cover 256
+// x-voxels of 4 rows, blockIdx.y = z, blockIdx.z = batch item.  The boundary
+// tests on D_y and D_z hold for all four voxels at once (their strides and
+// the array ends are multiples of 4); D_x needs the next quad's first value.
+struct Quad {
+  int x, y, z;  // x = first of the four
+  unsigned i;   // flat index of (x, y, z) within the volume
+};
+
+__host__ __device__ inline bool quad_ok(const Vol3& v) {
+  return v.nx % 4 == 0 && (v.mode == 0 || v.ny % 4 == 0) && v.n < (1ll << 31);
+}
+__host__ __device__ inline int quad_tiles_x(const Vol3& v) { return (v.nx + 255) / 256; }
+inline dim3 quad_grid(const Vol3& v, int batch) {
+  return dim3((unsigned)(quad_tiles_x(v) * ((v.ny + 3) / 4)), (unsigned)v.nz, (unsigned)batch);
+}
+__device__ __forceinline__ bool quad_pos(const Vol3& v, Quad& q) {
+  const int tq = quad_tiles_x(v);
+  const int ty = (int)blockIdx.x / tq, tx = (int)blockIdx.x - ty * tq;
+  q.x = tx * 256 + 4 * (int)(threadIdx.x & 63);
+  q.y = ty * 4 + (int)(threadIdx.x >> 6);
+  q.z = (int)blockIdx.y;
+  q.i = (unsigned)((q.z * v.ny + q.y) * v.nx + q.x);
+  return q.x < v.nx && q.y < v.ny;
+}
+// has_next on D_y / D_z (the same for the quad's four voxels)
+__device__ __forceinline__ bool quad_next(const Vol3& v, const Quad& q, int a) {
+  if (v.mode == 0) return a == 1 ? q.y != v.ny - 1 : q.z != v.nz - 1;
+  return (int64_t)q.i < v.n - stride(v, a);
+}
+__device__ __forceinline__ bool quad_prev(const Vol3& v, const Quad& q, int a) {
+  if (v.mode == 0) return a == 1 ? q.y != 0 : q.z != 0;
+  return (int64_t)q.i >= stride(v, a);
+}
+
+__global__ __launch_bounds__(256) void k_pdhg_dual4(const float* __restrict__ fbar,
+                                                    float* __restrict__ p, Vol3 v,
+                                                    int64_t plane, float sigma, float l0,
+                                                    float l1, float l2) {
+  Quad q;
+  if (!quad_pos(v, q)) return;
+  const int64_t off = (int64_t)blockIdx.z * v.n;
+  const float* fb = fbar + off;
+  const float4 c = *reinterpret_cast<const float4*>(fb + q.i);
+  const float edge0 = v.mode == 0 ? 0.f : -c.w;
+  // D_x: within the quad, and from the next quad's first value at the end
+  const bool last = q.x + 4 == v.nx;
+  const float4 dx = make_float4(c.y - c.x, c.z - c.y, c.w - c.z, last ? edge0 : fb[q.i + 4] - c.w);
+  float4 dy, dz;
+  const float4 cn = make_float4(-c.x, -c.y, -c.z, -c.w);
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (quad_next(v, q, 1)) {
+    const float4 ny = *reinterpret_cast<const float4*>(fb + q.i + (unsigned)v.s1);
+    dy = make_float4(ny.x - c.x, ny.y - c.y, ny.z - c.z, ny.w - c.w);
+  } else {
+    dy = v.mode == 0 ? zero : cn;
+  }
+  if (quad_next(v, q, 2)) {
+    const float4 nz = *reinterpret_cast<const float4*>(fb + q.i + (unsigned)v.s2);
+    dz = make_float4(nz.x - c.x, nz.y - c.y, nz.z - c.z, nz.w - c.w);
+  } else {
+    dz = v.mode == 0 ? zero : cn;
+  }
+  const float4 d[3] = {dx, dy, dz};
+  const float lam[3] = {l0, l1, l2};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float4* pa = reinterpret_cast<float4*>(p + a * plane + off + q.i);
+    float4 w = *pa;
+    const float l = lam[a];
+    w.x = fminf(fmaxf(w.x + sigma * d[a].x, -l), l);
+    w.y = fminf(fmaxf(w.y + sigma * d[a].y, -l), l);
+    w.z = fminf(fmaxf(w.z + sigma * d[a].z, -l), l);
+    w.w = fminf(fmaxf(w.w + sigma * d[a].w, -l), l);
+    *pa = w;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pdhg_primal4(float* __restrict__ f,
+                                                      float* __restrict__ fbar,
+                                                      const float* __restrict__ atw,
+                                                      const float* __restrict__ p, Vol3 v,
+                                                      int64_t plane, float tau) {
+  Quad q;
+  if (!quad_pos(v, q)) return;
+  const int64_t off = (int64_t)blockIdx.z * v.n;
+  float4 g = *reinterpret_cast<const float4*>(atw + off + q.i);
+  // D_x^T p: p_x(i - 1) where x != 0, minus p_x(i) where row i has its -1
+  {
+    const float* px = p + off;
+    const float4 c = *reinterpret_cast<const float4*>(px + q.i);
+    const float prev = q.x > 0 ? px[q.i - 1] : 0.f;
+    const bool last = q.x + 4 == v.nx;
+    g.x += prev - c.x;
+    g.y += c.x - c.y;
+    g.z += c.y - c.z;
+    g.w += c.z - ((v.mode != 0 || !last) ? c.w : 0.f);
+  }
+#pragma unroll
+  for (int a = 1; a < 3; ++a) {
+    const float* pa = p + a * plane + off;
+    const float4 c = *reinterpret_cast<const float4*>(pa + q.i);
+    if (quad_prev(v, q, a)) {
+      const float4 b = *reinterpret_cast<const float4*>(pa + q.i - (unsigned)stride(v, a));
+      g.x += b.x;
+      g.y += b.y;
+      g.z += b.z;
+      g.w += b.w;
+    }
+    if (v.mode != 0 || quad_next(v, q, a)) {
+      g.x -= c.x;
+      g.y -= c.y;
+      g.z -= c.z;
+      g.w -= c.w;
+    }
+  }
+  float4* fp = reinterpret_cast<float4*>(f + off + q.i);
+  const float4 fo = *fp;
+  const float4 fn = make_float4(fmaxf(fo.x - tau * g.x, 0.f), fmaxf(fo.y - tau * g.y, 0.f),
+                                fmaxf(fo.z - tau * g.z, 0.f), fmaxf(fo.w - tau * g.w, 0.f));
+  *fp = fn;
+  *reinterpret_cast<float4*>(fbar + off + q.i) =
+      make_float4(2.f * fn.x - fo.x, 2.f * fn.y - fo.y, 2.f * fn.z - fo.z, 2.f * fn.w - fo.w);
+}
+
 // checks shared by the single-volume entry points; v = the volume's shape
 int vol_args(const int32_t* volume_size, int32_t batch, int32_t mode, Vol3& v) {
   if (!volume_size) return DTV_ERR_NULL_POINTER;
@@ -405,7 +759,12 @@ int dtv_tv_fista_step(const float* y, const float* g, const float* xprev, float*
   if (rc) return rc;
   dtv_tv_item it{y, g, xprev, xnew, ynew, volume_size[0], volume_size[1], volume_size[2],
                  nonneg ? 1 : 0, step, lam, eps, beta};
-  if (march_ok(v))
+  const bool al = ((uintptr_t)y | (uintptr_t)g | (uintptr_t)xprev | (uintptr_t)xnew |
+                   (uintptr_t)ynew) % 16 == 0;
+  if (march4_ok(v) && al)
+    hipLaunchKernelGGL(k_tv_fista_march4, dim3((unsigned)march4_blocks(v), batch), dim3(512), 0,
+                       (hipStream_t)stream, it, v);
+  else if (march_ok(v))
     hipLaunchKernelGGL(k_tv_fista_march, dim3((unsigned)march_blocks(v), batch), dim3(512), 0,
                        (hipStream_t)stream, it, v);
   else
@@ -454,7 +813,10 @@ int dtv_tv_descent(const float* f, float* out, int32_t batch, const int32_t volu
   int rc = vol_args(volume_size, batch, mode, v);
   if (rc) return rc;
   if (!(eps > 0.f) || f == out) return DTV_ERR_INVALID_ARGUMENT;
-  if (march_ok(v))
+  if (march4_ok(v) && ((uintptr_t)f | (uintptr_t)out) % 16 == 0)
+    hipLaunchKernelGGL(k_tv_descent_march4, dim3((unsigned)march4_blocks(v), batch), dim3(512),
+                       0, (hipStream_t)stream, f, out, v, step, eps);
+  else if (march_ok(v))
     hipLaunchKernelGGL(k_tv_descent_march, dim3((unsigned)march_blocks(v), batch), dim3(512), 0,
                        (hipStream_t)stream, f, out, v, step, eps);
   else
@@ -477,6 +839,16 @@ int dtv_pdhg_dtv_step(float* f, float* fbar, const float* atw, float* p, int32_t
     if (!(lam[i] >= 0.f)) return DTV_ERR_INVALID_ARGUMENT;
   const int64_t plane = v.n * batch;
   hipStream_t st = (hipStream_t)stream;
+  // float4 access needs 16-byte aligned bases as well
+  const bool aligned = ((uintptr_t)f | (uintptr_t)fbar | (uintptr_t)atw | (uintptr_t)p) % 16 == 0;
+  if (quad_ok(v) && aligned) {
+    const dim3 grid = quad_grid(v, batch);
+    hipLaunchKernelGGL(k_pdhg_dual4, grid, dim3(256), 0, st, fbar, p, v, plane, sigma, lam[0],
+                       lam[1], lam[2]);
+    if (hipGetLastError() != hipSuccess) return DTV_ERR_HIP;
+    hipLaunchKernelGGL(k_pdhg_primal4, grid, dim3(256), 0, st, f, fbar, atw, p, v, plane, tau);
+    return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;
+  }
   const dim3 grid = vol3_grid(v, batch);
   hipLaunchKernelGGL(k_pdhg_dual, grid, dim3(256), 0, st, fbar, p, v, plane, sigma, lam[0],
                      lam[1], lam[2]);

@@ -170,12 +170,14 @@ def test_asd_pocs_without_tv_steps_is_landweber_to_the_solution():
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["neumann", "reference"])
 @pytest.mark.parametrize("shp", [(2, 1, 13, 11, 17), (1, 1, 20, 16, 16), (1, 1, 35, 70, 70),
-                                 (2, 1, 3, 9, 130)])
+                                 (2, 1, 3, 9, 130), (1, 1, 19, 64, 64), (1, 1, 35, 204, 204),
+                                 (2, 1, 3, 20, 512)])
 def test_fused_fista_tv_step(cuda, mode, shp):
     """The fused step vs the torch formulation (whose D and D^T meet the
     sparse matrices); the shapes cover several x-y tiles and z chunks of the
-    z-marching kernel (Neumann, and reference mode with X == Y) and the
-    per-voxel kernel (reference mode with X != Y)."""
+    z-marching kernels, one voxel per lane and four (X = 204, 512), in
+    Neumann and reference mode (X == Y), and the per-voxel kernel (reference
+    mode with X != Y)."""
     print(f"[case] fused_fista_tv_step {mode} {shp}", flush=True)
     y = phantom.gaussian(shp, 1)
     g = phantom.gaussian(shp, 2)
@@ -194,7 +196,8 @@ def test_fused_fista_tv_step(cuda, mode, shp):
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["reference", "neumann"])
 @pytest.mark.parametrize("shp", [(2, 1, 13, 11, 17), (1, 1, 1, 1, 5), (1, 1, 9, 1, 1),
-                                 (3, 1, 4, 6, 5), (1, 1, 20, 16, 16), (2, 1, 35, 70, 70)])
+                                 (3, 1, 4, 6, 5), (1, 1, 20, 16, 16), (2, 1, 35, 70, 70),
+                                 (2, 1, 35, 68, 68), (1, 1, 4, 204, 204)])
 def test_fused_tv_descent_matches_oracle(cuda, mode, shp):
     print(f"[case] tv_descent {mode} {shp}", flush=True)
     f = phantom.gaussian(shp, 21)
@@ -210,7 +213,8 @@ def test_fused_tv_descent_matches_oracle(cuda, mode, shp):
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["neumann", "reference"])
 @pytest.mark.parametrize("shp", [(2, 1, 13, 11, 17), (1, 1, 1, 1, 5), (1, 1, 9, 1, 1),
-                                 (3, 1, 4, 6, 5)])
+                                 (3, 1, 4, 6, 5), (2, 1, 5, 12, 16), (1, 1, 3, 8, 260),
+                                 (1, 1, 2, 3, 4)])
 def test_fused_pdhg_dtv_step(cuda, mode, shp):
     """The two fused launches vs the torch step in FP64 (whose D and D^T are
     checked against the sparse matrices above); the clamp and the relu both
