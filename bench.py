@@ -254,6 +254,68 @@ def pdhg_512(iters, log):
                 geometry="512x512x384 @0.4mm, 120 views, 768x576 @0.6mm, SOD 700, SDD 1200")
 
 
+def volume_steps(reps, log):
+    """The fused volume-update kernels on their own, event-timed over ``reps``
+    launches after one warm-up: the FISTA-TV step (20 B of algorithmic HBM
+    traffic per voxel: y, g, xprev read, x and y' written), the ASD-POCS TV
+    descent step (8 B: f read, f' written) in both boundary modes, and the
+    PDHG/DTV dual + primal pair (56 B: the dual reads f-bar and p and writes
+    p; the primal reads f, A^T w and p and writes f and f-bar) at 256^3 and
+    512x512x384 on uniform [0, 1) inputs; then one FISTA-TV step over the 16
+    uneven instances of the batched unroll (one launch)."""
+    import torch
+    from dtvproj import batched, configs, tv
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    gen = torch.Generator(device=dev).manual_seed(40)
+
+    def timed(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def rec(name, shape, mode, ms, per_voxel, voxels):
+        tbs = per_voxel * voxels / ms / 1e9
+        log(f"{name} {shape} {mode}: {ms:.3f} ms, {tbs:.2f} TB/s ({per_voxel} B/voxel)")
+        return dict(kernel=name, shape=list(shape), mode=mode, ms=ms, bytes_per_voxel=per_voxel,
+                    voxels=voxels, achieved_tbs=tbs, frac=tbs * 1e3 / 8000.0)
+
+    out = []
+    for shape in ((256, 256, 256), (384, 512, 512)):
+        full = (1, 1) + shape
+        n = shape[0] * shape[1] * shape[2]
+        u = lambda: torch.rand(full, device=dev, generator=gen)  # noqa: E731
+        y, g, xp = u(), u(), u()
+        xn, yn, fo = torch.empty_like(y), torch.empty_like(y), torch.empty_like(y)
+        for mode in ("neumann", "reference"):
+            t = timed(lambda: tv.fista_tv_step(y, g, xp, 0.1, 0.01, 0.01, 0.5, True, mode,
+                                               xnew=xn, ynew=yn))
+            out.append(rec("fista_tv_step", shape, mode, t, 20, n))
+            t = timed(lambda: tv.tv_descent(y, 0.1, 1.0, mode, out=fo))
+            out.append(rec("tv_descent", shape, mode, t, 8, n))
+        del y, g, xp, xn, yn, fo
+        f, fb, atw = u(), u(), u()
+        p = torch.rand((3,) + full, device=dev, generator=gen)
+        for mode in ("neumann", "reference"):
+            t = timed(lambda: tv.pdhg_dtv_step(f, fb, atw, p, 0.1, 0.1, (0.3, 0.3, 0.3), mode))
+            out.append(rec("pdhg_dtv_step", shape, mode, t, 56, n))
+        del f, fb, atw, p
+    geoms = configs.uneven_16()
+    vox = sum(g.volume_size[0] * g.volume_size[1] * g.volume_size[2] for g in geoms)
+    sinos = [torch.zeros(g.sino_shape(1), device=dev) for g in geoms]
+    bf = batched.BatchedFistaTV(geoms, sinos, iters=2, steps=[0.1] * len(geoms))
+    for t_ in bf.y[0] + bf.g + bf.x:
+        t_.uniform_(generator=gen)
+    t = timed(lambda: bf.tv_step(0))
+    out.append(rec("fista_tv_step_multi", ("uneven_16",), "neumann", t, 20, vox))
+    return out
+
+
 PMC_FILE = os.path.join(ROOT, "profiles", "pmc_sparse_256.json")
 
 
@@ -314,7 +376,7 @@ def roofline_record(dom, dom_t, dom_b, acct, t_fwd, t_adj, med_fwd, med_adj, log
                 pmc_source="profiles/pmc_sparse_256.json")
 
 
-LEGS = ("beijing", "batched", "pdhg512", "dense")
+LEGS = ("beijing", "batched", "pdhg512", "dense", "volume_steps")
 
 
 def main():
@@ -364,7 +426,8 @@ def main():
         res = {"beijing": lambda: beijing_loop(max(1, args.steps), log),
                "batched": lambda: batched_unroll(log),
                "pdhg512": lambda: pdhg_512(max(1, args.steps), log),
-               "dense": dense_leg}[args.leg]()
+               "dense": dense_leg,
+               "volume_steps": lambda: volume_steps(max(1, args.steps), log)}[args.leg]()
         print(json.dumps({"leg": args.leg, "result": res}), flush=True)
         return
     _capi.lib()

@@ -202,18 +202,22 @@ class BatchedFistaTV:
         for t in self.x + self.y[0] + self.y[1]:
             t.zero_()
 
+    def tv_step(self, k):
+        """Iteration ``k``'s fused TV/FISTA update over every instance (one
+        launch)."""
+        host, dev_table = self.tv_tables[k]
+        rc = _capi.lib().dtv_tv_fista_step_multi(ctypes.addressof(host), dev_table.data_ptr(),
+                                                 len(self.geoms), self.mode,
+                                                 ops._stream(self.mp.device))
+        _capi.check(rc, "dtv_tv_fista_step_multi")
+
     def run(self):
         """Eager unroll (also what gets captured)."""
-        L = _capi.lib()
-        stream = ops._stream(self.mp.device)
         for k in range(self.iters):
             p = k % 2
             self.mp.run(self.fwd[p], adjoint=False)
             self.mp.run(self.adj, adjoint=True)
-            host, dev_table = self.tv_tables[k]
-            rc = L.dtv_tv_fista_step_multi(ctypes.addressof(host), dev_table.data_ptr(),
-                                           len(self.geoms), self.mode, stream)
-            _capi.check(rc, "dtv_tv_fista_step_multi")
+            self.tv_step(k)
         return self.x
 
     def capture(self):
