@@ -799,7 +799,15 @@ const char* dtv_strerror(int code) {
   }
 }
 
-const char* dtv_version(void) { return "dtvproj 0.1.0 gfx950"; }
+// The build passes the digest of the sources it compiled (the loader compares
+// it with the sources beside the library, so a stale build is refused).
+#ifndef DTV_SOURCE_DIGEST
+#define DTV_SOURCE_DIGEST unknown
+#endif
+#define DTV_STR2(x) #x
+#define DTV_STR(x) DTV_STR2(x)
+
+const char* dtv_version(void) { return "dtvproj 0.2.0 gfx950 src:" DTV_STR(DTV_SOURCE_DIGEST); }
 
 }  // extern "C"
 

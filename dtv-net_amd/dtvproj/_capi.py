@@ -1,7 +1,8 @@
 """ctypes binding of ``libdtvproj.so`` (declared in ``include/dtvproj.h``).
 
 The library is loaded lazily on first use.  There is no fallback: if the
-library is missing or a GPU entry point fails, a RuntimeError is raised.
+library is missing, was built from other sources than the ones beside it, or
+a GPU entry point fails, a RuntimeError is raised.
 """
 from __future__ import annotations
 
@@ -9,7 +10,7 @@ import ctypes
 import os
 import threading
 
-from .build import LIB_PATH
+from .build import LIB_PATH, source_digest
 
 _lock = threading.Lock()
 _lib = None
@@ -135,8 +136,22 @@ def lib():
         L.dtv_strerror.restype = ctypes.c_char_p
         L.dtv_version.argtypes = []
         L.dtv_version.restype = ctypes.c_char_p
+        check_fresh(L.dtv_version().decode())
         _lib = L
         return _lib
+
+
+def check_fresh(version_string: str) -> None:
+    """Raise if the library's compiled-in source digest differs from the
+    digest of the sources beside it (an edit since the last build)."""
+    want = source_digest()
+    if want is None:  # installed without its sources: nothing to compare
+        return
+    have = version_string.rpartition("src:")[2] if "src:" in version_string else None
+    if have != want:
+        raise RuntimeError(
+            f"{LIB_PATH} was built from other sources (digest {have}, sources {want}); "
+            "rebuild it with `python -m dtvproj.build`")
 
 
 def check(rc: int, what: str) -> None:

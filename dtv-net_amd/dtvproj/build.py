@@ -6,6 +6,8 @@ with the source tree.
 """
 from __future__ import annotations
 
+import glob
+import hashlib
 import os
 import shutil
 import subprocess
@@ -45,10 +47,36 @@ def sources():
     return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
+def digest_inputs():
+    """Every file the library is compiled from: the HIP sources, the private
+    headers and the public C header."""
+    files = glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.h"))
+    files += glob.glob(os.path.join(INCLUDE, "*.h"))
+    return sorted(files)
+
+
+def source_digest() -> str | None:
+    """First 16 hex digits of the SHA-256 over the library's inputs (name and
+    contents of each file), or None when the sources are not present.  The
+    build compiles it into ``dtv_version()``, and the loader refuses a library
+    whose digest differs from that of the sources beside it."""
+    files = digest_inputs()
+    if not files:
+        return None
+    h = hashlib.sha256()
+    for f in files:
+        h.update(os.path.basename(f).encode() + b"\0")
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+        h.update(b"\0")
+    return h.hexdigest()[:16]
+
+
 def build(verbose: bool = False) -> str:
     """Compile every HIP source into ``libdtvproj.so``; returns its path."""
     out_tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc_path(), *HIPCC_FLAGS, f"-I{CSRC}", f"-I{INCLUDE}", *sources(), "-o", out_tmp]
+    cmd = [hipcc_path(), *HIPCC_FLAGS, f"-DDTV_SOURCE_DIGEST={source_digest()}", f"-I{CSRC}",
+           f"-I{INCLUDE}", *sources(), "-o", out_tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

@@ -341,6 +341,9 @@ int dtv_views_beijing(const float* project_vector, int32_t n_views,
                       double* views_out, dtv_scheme* scheme_out);
 
 const char* dtv_strerror(int code);
+/* "dtvproj <version> gfx950 src:<16 hex digits>": the digits are the first of
+ * the SHA-256 over the sources the library was compiled from (its build
+ * script computes them), so a loader can refuse a stale build. */
 const char* dtv_version(void);
 
 #ifdef __cplusplus

@@ -68,6 +68,24 @@ def test_library_loads_and_exports_header_symbols():
     assert "gfx950" in _capi.version()
 
 
+def test_library_is_built_from_the_sources_beside_it():
+    """The loaded library carries the digest of the sources it was compiled
+    from, and the loader refuses one whose digest differs (a stale build)."""
+    from dtvproj import build
+    want = build.source_digest()
+    assert want is not None and len(want) == 16
+    assert _capi.version().endswith("src:" + want)
+    _capi.check_fresh(_capi.version())
+    flipped = want[:-1] + ("0" if want[-1] != "0" else "1")
+    for stale in ("dtvproj 0.2.0 gfx950 src:" + flipped, "dtvproj 0.1.0 gfx950",
+                  "dtvproj 0.2.0 gfx950 src:unknown"):
+        with pytest.raises(RuntimeError, match="rebuild"):
+            _capi.check_fresh(stale)
+    # the digest covers every source and header the library is compiled from
+    names = {os.path.basename(f) for f in build.digest_inputs()}
+    assert set(build.SOURCES) <= names and "dtvproj.h" in names and "dtv_geom.h" in names
+
+
 def test_host_view_builders_match_python_geometry():
     vs, ds = (37, 29, 21), (45, 27)
     ang = np.linspace(0, 360, 13, False) + 0.3
