@@ -399,7 +399,7 @@ int dtv_l1ball(const dtv_l1_item* items, int32_t n_items, int32_t mode, double t
 int dtv_shrink_step(const float* y, const float* g, const float* xprev, float* xnew,
                     float* ynew, int64_t n, float step, float thr, float beta, void* stream) {
   if (!y || !g || !xprev || !xnew || !ynew) return DTV_ERR_NULL_POINTER;
-  if (n < 1 || (const void*)ynew == xprev) return DTV_ERR_INVALID_ARGUMENT;
+  if (n < 1 || (const void*)ynew == xprev || ynew == xnew) return DTV_ERR_INVALID_ARGUMENT;
   hipLaunchKernelGGL(k_shrink_step, dim3(flat_blocks(n)), dim3(256), 0, (hipStream_t)stream,
                      y, g, xprev, xnew, ynew, n, step, thr, beta);
   return hipGetLastError() == hipSuccess ? DTV_OK : DTV_ERR_HIP;

@@ -754,6 +754,9 @@ int dtv_tv_fista_step(const float* y, const float* g, const float* xprev, float*
                       float lam, float eps, float beta, int32_t nonneg, int32_t mode,
                       void* stream) {
   if (!y || !g || !xprev || !xnew || !ynew) return DTV_ERR_NULL_POINTER;
+  // y is read by the neighbours' stencils: neither output may overwrite it
+  if ((const float*)xnew == y || (const float*)ynew == y || xnew == ynew)
+    return DTV_ERR_INVALID_ARGUMENT;
   Vol3 v;
   int rc = vol_args(volume_size, batch, mode, v);
   if (rc) return rc;
@@ -785,6 +788,8 @@ int dtv_tv_fista_step_multi(const dtv_tv_item* items_host, const dtv_tv_item* it
     const dtv_tv_item& it = items_host[k];
     if (!it.y || !it.g || !it.xprev || !it.xnew || !it.ynew) return DTV_ERR_NULL_POINTER;
     if (it.nx < 1 || it.ny < 1 || it.nz < 1) return DTV_ERR_INVALID_ARGUMENT;
+    if ((const float*)it.xnew == it.y || (const float*)it.ynew == it.y || it.xnew == it.ynew)
+      return DTV_ERR_INVALID_ARGUMENT;
     march = march && march_ok(make_vol3(it.nx, it.ny, it.nz, mode));
   }
   int64_t total = 0;

@@ -149,6 +149,7 @@ def tv_descent(f, step, eps=1.0, mode="reference", out=None):
         raise RuntimeError("out must be float32 with the shape of f")
     if out.data_ptr() == f.data_ptr():
         raise RuntimeError("out must not share storage with f (stencil reads)")
+    _same_device(f, out)
     Z, Y, X = f.shape[-3:]
     B = f.numel() // (X * Y * Z)
     with torch.cuda.device(f.device):
@@ -186,6 +187,7 @@ def pdhg_dtv_step(f, fbar, atw, p, sigma, tau, lam, mode="neumann"):
         raise RuntimeError("fbar and atw must have the shape of f, p the shape [3, *f.shape]")
     if len({t.untyped_storage().data_ptr() for t in ts.values()}) != 4:
         raise RuntimeError("f, fbar, atw and p must not share storage")
+    _same_device(f, fbar, atw, p)
     Z, Y, X = f.shape[-3:]
     B = f.numel() // (X * Y * Z)
     lam3 = (ctypes.c_float * 3)(*[float(v) for v in lam])
@@ -282,6 +284,21 @@ def l1ball_torch(v, b, tol=1e-4, max_iter=100):
     return torch.sign(v) * torch.clamp(a - lam.to(a.dtype), min=0)
 
 
+def _check_outputs(like, *outs):
+    """Each (tensor, name) in ``outs``: a contiguous FP32 CUDA tensor of the
+    shape of ``like`` (the kernels write ``like.numel()`` elements)."""
+    for o, n in outs:
+        check_input(o, n)
+        if o.dtype != torch.float32 or o.shape != like.shape:
+            raise RuntimeError(f"{n} must be float32 with the shape {tuple(like.shape)}")
+
+
+def _same_device(*ts):
+    dev = ts[0].device
+    if any(t.device != dev for t in ts):
+        raise RuntimeError("all tensors must be on one device")
+
+
 def shrink(v, thr):
     """Soft threshold sign(v) max(|v| - thr, 0)."""
     return torch.sign(v) * torch.clamp(v.abs() - thr, min=0)
@@ -297,8 +314,12 @@ def shrink_step(y, g, xprev, step, thr, beta, xnew=None, ynew=None):
             raise RuntimeError(f"{n} must be float32 with the shape of y")
     xnew = torch.empty_like(y) if xnew is None else xnew
     ynew = torch.empty_like(y) if ynew is None else ynew
+    _check_outputs(y, (xnew, "xnew"), (ynew, "ynew"))
+    _same_device(y, g, xprev, xnew, ynew)
     if ynew.data_ptr() == xprev.data_ptr():
         raise RuntimeError("ynew must not share storage with xprev")
+    if ynew.data_ptr() == xnew.data_ptr():
+        raise RuntimeError("xnew and ynew must be distinct tensors")
     with torch.cuda.device(y.device):
         rc = _capi.lib().dtv_shrink_step(y.data_ptr(), g.data_ptr(), xprev.data_ptr(),
                                          xnew.data_ptr(), ynew.data_ptr(), y.numel(),
@@ -324,6 +345,8 @@ def fdpg_step(f, g, sigma, v, t, beta, mode="reference", out=None):
         if x.dtype != torch.float32 or x.shape != f.shape:
             raise RuntimeError(f"{n} must be float32 with the shape of f")
     out = torch.empty_like(f) if out is None else out
+    _check_outputs(f, (out, "out"))
+    _same_device(f, g, out)
     if out.data_ptr() in (f.data_ptr(), g.data_ptr()):
         raise RuntimeError("out must not share storage with f or g")
     Z, Y, X = f.shape[-3:]
@@ -386,6 +409,9 @@ def fista_tv_step(y, g, xprev, step, lam, eps, beta, nonneg=True, mode="neumann"
             raise RuntimeError(f"{n} must be float32 with the shape of y")
         if o.data_ptr() == y.data_ptr() or o.data_ptr() == g.data_ptr():
             raise RuntimeError(f"{n} must not share storage with y or g (stencil reads)")
+    if xnew.data_ptr() == ynew.data_ptr():
+        raise RuntimeError("xnew and ynew must be distinct tensors")
+    _same_device(y, g, xprev, xnew, ynew)
     Z, Y, X = y.shape[-3:]
     B = y.numel() // (X * Y * Z)
     with torch.cuda.device(y.device):

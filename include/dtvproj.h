@@ -203,7 +203,9 @@ typedef struct dtv_tv_item {
 
 /* Fused step for ``batch`` equal-size volumes.  mode 0: Neumann forward
  * differences; mode 1: the reference's difference matrices
- * (model/PDHG/Utils/Gradients.py:8-47, incl. its D_y stride of Y).  The
+ * (model/PDHG/Utils/Gradients.py:8-47, incl. its D_y stride of Y).  xnew
+ * may be xprev (in place); neither output may be y (the stencil reads y's
+ * neighbours) and xnew != ynew, else DTV_ERR_INVALID_ARGUMENT.  The
  * reference has no fused counterpart: its unrolls apply torch.sparse
  * matrices per batch item (e.g. model/PDHG/ASDPOCS.py:38-43,
  * model/FISTA/DTVFISTA.py:29-36). */
@@ -293,7 +295,7 @@ int dtv_l1ball_noalloc(const dtv_l1_item* items, int32_t n_items, int32_t mode, 
 
 /* Soft threshold with FISTA momentum over n values (model/FISTA/FISTA.py:21-25):
  *   d = y - step * g;  xnew = sign(d) relu(|d| - thr);  ynew = xnew + beta (xnew - xprev)
- * xnew may be xprev; ynew must differ from xprev. */
+ * xnew may be xprev; ynew must differ from xprev and from xnew. */
 int dtv_shrink_step(const float* y, const float* g, const float* xprev, float* xnew,
                     float* ynew, int64_t n, float step, float thr, float beta, void* stream);
 

@@ -159,6 +159,16 @@ def test_capi_rejects_bad_arguments_without_gpu():
     assert lib.dtv_row_filter(None, q, 4, 8, None, 0, p, 5, None) == 2
     assert lib.dtv_row_filter(p, q, 4, 20000, None, 0, ctypes.c_void_p(48), 5001,
                               None) == 3                                       # LDS budget
+    # FISTA-TV step: outputs may not overwrite y (stencil input) or each other
+    one = f32(1.0)
+    assert lib.dtv_tv_fista_step(p, q, r, p, s, 1, v3, one, one, one, one, 1, 0, None) == 1
+    assert lib.dtv_tv_fista_step(p, q, r, s, p, 1, v3, one, one, one, one, 1, 0, None) == 1
+    assert lib.dtv_tv_fista_step(p, q, r, s, s, 1, v3, one, one, one, one, 1, 0, None) == 1
+    # shrink step: ynew may be neither xprev nor xnew; at least one value
+    assert lib.dtv_shrink_step(p, q, r, s, r, 4, one, one, one, None) == 1
+    assert lib.dtv_shrink_step(p, q, r, s, s, 4, one, one, one, None) == 1
+    assert lib.dtv_shrink_step(p, q, r, s, None, 4, one, one, one, None) == 2
+    assert lib.dtv_shrink_step(p, q, r, s, ctypes.c_void_p(80), 0, one, one, one, None) == 1
 
 
 def test_ops_require_device_tensors():

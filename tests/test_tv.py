@@ -322,6 +322,16 @@ def test_fused_shrink_step_closed_form(cuda):
     xpd = xp.to(cuda)
     gx2, gy2 = tv.shrink_step(y.to(cuda), g.to(cuda), xpd, step, thr, beta, xnew=xpd)
     assert rel_l2(gx2, x) <= 1e-6 and rel_l2(gy2, yn) <= 1e-6
+    # outputs are checked before the launch: size, dtype, aliasing
+    yd, gd = y.to(cuda), g.to(cuda)
+    small = torch.empty(5, device=cuda)
+    with pytest.raises(RuntimeError, match="shape"):
+        tv.shrink_step(yd, gd, xpd, step, thr, beta, xnew=small)
+    with pytest.raises(RuntimeError, match="shape"):
+        tv.shrink_step(yd, gd, xpd, step, thr, beta, ynew=torch.empty_like(yd).double())
+    both = torch.empty_like(yd)
+    with pytest.raises(RuntimeError, match="distinct"):
+        tv.shrink_step(yd, gd, xpd, step, thr, beta, xnew=both, ynew=both)
 
 
 def _fdpg_fp64(f, g, sigma, v, t, beta, mode):
@@ -352,6 +362,9 @@ def test_fused_fdpg_step_matches_sparse_matrices(cuda, mode, shp):
     assert rel_l2(got, torch.from_numpy(want)) <= 1e-6
     got_t = tv.fdpg_step_torch(f.double(), g.double(), sigma, v, t, beta, mode)
     assert rel_l2(got_t, torch.from_numpy(want)) <= 1e-12
+    with pytest.raises(RuntimeError, match="shape"):
+        tv.fdpg_step(f.to(cuda), g.to(cuda), sigma, v, t, beta, mode,
+                     out=torch.empty(3, device=cuda))
 
 
 def test_asd_pocs_step_schedule(monkeypatch):
