@@ -333,6 +333,18 @@ __device__ __forceinline__ void sched_sort(RaySched& rs, const unsigned (&slot)[
   __syncthreads();
 }
 
+// Pass 2 hands the sorted list (longest class first) to the waves in
+// 64-ray blocks, one block per wave per round of blockDim.x rays, in snake
+// order: wave w takes block w of even rounds and block nw - 1 - w of odd
+// ones, so the wave with the longest rays of one round gets the shortest of
+// the next and the waves reach the next barrier together (on the 256^3 /
+// 60-view pair: A 2.76 -> 2.61 ms, A^T 2.97 -> 2.87 ms against the plain
+// round-robin order).  Every list index below the hit count is visited once.
+__device__ __forceinline__ int pass2_index(int p0, int odd) {
+  const int w = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
+  return p0 + ((odd ? nw - 1 - w : w) << 6) + (int)(threadIdx.x & 63);
+}
+
 // Candidate index -> pixel.  Forward: row-major over the rectangle.
 // Adjoint: the rectangle split into its nine stride-3 sub-lattices, so rays
 // that end up side by side in a wave are mostly three pixels apart and
@@ -502,7 +514,9 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
         const int nhit = uni((int)rs.n);
         View v;
         uni_view(st.sv[cur], v);
-        for (int p = threadIdx.x; p < nhit; p += blockDim.x) {
+        for (int p0 = 0, odd = 0; p0 < nhit; p0 += blockDim.x, odd ^= 1) {
+          const int p = pass2_index(p0, odd);
+          if (p >= nhit) continue;
           int iu, iv;
           rm.pixel(seg + rs.list[p], false, iu, iv);
           int ka, kb;
@@ -828,7 +842,9 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
           const int nhit = uni((int)rs.n);
           View v;
           uni_view(st.sv[cur], v);
-          for (int p = threadIdx.x; p < nhit; p += blockDim.x) {
+          for (int p0 = 0, odd = 0; p0 < nhit; p0 += blockDim.x, odd ^= 1) {
+            const int p = pass2_index(p0, odd);
+            if (p >= nhit) continue;
             int iu, iv;
             rm.pixel(seg + rs.list[p], true, iu, iv);
             const float val = srow[(size_t)iv * g.nu + iu] * vs;
