@@ -316,6 +316,55 @@ def volume_steps(reps, log):
     return out
 
 
+def workloads(reps, log):
+    """A and matched A^T with the default kernels on the other workload sizes
+    (seeded ellipsoid phantoms; A^T on the phantom's own projection), medians
+    of ``reps`` event-timed calls: 64^3 / 72 views, 4 x 128^3 / 90 views,
+    256^3 / 60 views, 512x512x384 / 120 views, and the reference's Standard
+    options geometry (256x256x64, 48 views of 288x28, SID 1000 / SDD 2000
+    pixel units)."""
+    import torch
+    from dtvproj import configs, geometry as G, ops, phantom, roofline
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    cases = [("64^3 @1mm, 72 views, 128^2", configs.fista_64(), 1, 1.0, 10),
+             ("4 x 128^3 @1mm, 90 views, 192^2", configs.unroll_128(), 4, 1.0, 10),
+             ("256^3 @0.5mm, 60 views, 384^2", configs.sparse_256(), 1, 0.5, 20),
+             ("512x512x384 @0.4mm, 120 views, 768x576", configs.large_512(), 1, 0.4, 40),
+             ("Standard options 256x256x64, 48 views, 288x28",
+              G.standard(G.uniform_angles(48), (256, 256, 64), (288, 28), 1000.0, 2000.0), 1,
+              1.0, 20)]
+
+    def med(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return statistics.median(ts)
+
+    out = []
+    for name, g, batch, vx, nell in cases:
+        vol = torch.cat([phantom.phantom(g.volume_size, vx, nell, seed=2 + b)
+                         for b in range(batch)]).to(dev)
+        sino = ops.forward(vol, g)
+        ta = med(lambda: ops.forward(vol, g))
+        tt = med(lambda: ops.adjoint(sino, g))
+        acct = roofline.pair_bytes(g, batch=batch)
+        rec = dict(workload=name, batch=batch, forward_ms=ta, adjoint_ms=tt, pair_ms=ta + tt,
+                   grays_per_s=acct["rays"] / ((ta + tt) * 1e-3) / 1e9,
+                   tap_gbs=acct["pair"] / ((ta + tt) * 1e-3) / 1e9)
+        log(f"{name} (batch {batch}): A {ta:.2f} ms, A^T {tt:.2f} ms, "
+            f"{rec['grays_per_s']:.2f} Grays/s")
+        out.append(rec)
+        del vol, sino
+    return out
+
+
 PMC_FILE = os.path.join(ROOT, "profiles", "pmc_sparse_256.json")
 
 
@@ -376,7 +425,7 @@ def roofline_record(dom, dom_t, dom_b, acct, t_fwd, t_adj, med_fwd, med_adj, log
                 pmc_source="profiles/pmc_sparse_256.json")
 
 
-LEGS = ("beijing", "batched", "pdhg512", "dense", "volume_steps")
+LEGS = ("beijing", "batched", "pdhg512", "dense", "volume_steps", "workloads")
 
 
 def main():
@@ -427,7 +476,8 @@ def main():
                "batched": lambda: batched_unroll(log),
                "pdhg512": lambda: pdhg_512(max(1, args.steps), log),
                "dense": dense_leg,
-               "volume_steps": lambda: volume_steps(max(1, args.steps), log)}[args.leg]()
+               "volume_steps": lambda: volume_steps(max(1, args.steps), log),
+               "workloads": lambda: workloads(max(1, args.steps), log)}[args.leg]()
         print(json.dumps({"leg": args.leg, "result": res}), flush=True)
         return
     _capi.lib()
