@@ -180,16 +180,22 @@ __device__ __forceinline__ void qvec_vals(const Vol3& v, bool hx, bool hy, bool 
 
 // Marches block b of volume f (march_blocks(v) blocks per volume).  Per
 // voxel the lane produces, out.load(i) fetches what the update reads besides
-// f (issued before the slice's barrier) and out.store(i, fc, gtv, pre)
-// writes it; gtv = sum_a D_a^T q_a and fc = f(i).  The slice's f values at
-// z + 1 carry to the next slice, so each slice loads three values per lane;
-// offsets are 32-bit (volumes below 2^31 voxels) from the volume's base, and
-// the q's of two consecutive slices use alternate LDS buffers, so one
-// barrier per slice orders them.
+// f (issued one slice ahead) and out.store(i, fc, gtv, pre) writes it; gtv =
+// sum_a D_a^T q_a and fc = f(i).  Each lane holds f at its point of the
+// current slice and of the next one (loaded a slice ahead): the x neighbour
+// is the next lane's value (a wave is one row of the region), the y
+// neighbour the next row's value, which every lane copies into LDS a slice
+// ahead, so per slice a lane loads one value of f (lane 63 and the last row
+// load their neighbour beyond the region, and in the reference mode the
+// last row of the volume its neighbour (x, 0, z + 1)).  Offsets are 32-bit
+// (volumes below 2^31 voxels) from the volume's base; q and f of two
+// consecutive slices use alternate LDS buffers, so one barrier per slice
+// orders them.
 template <bool kAsd, typename Out>
 __device__ __forceinline__ void tv_march(const Vol3& v, const float* __restrict__ f, float eps,
                                          int b, Out& out) {
   __shared__ float sq[2][2][kMY][kMX];
+  __shared__ float sf[2][kMY][kMX];
   const int txy = march_tiles_x(v) * march_tiles_y(v);
   const int chunk = b / txy, t = b - chunk * txy;
   const int ty = t / march_tiles_x(v), tx = t - ty * march_tiles_x(v);
@@ -204,6 +210,9 @@ __device__ __forceinline__ void tv_march(const Vol3& v, const float* __restrict_
   const bool ref = v.mode != 0;
   // has_next on D_y: Neumann y != Y - 1, reference i < n - s_1
   auto hy_at = [&](unsigned i) { return ref ? i < n - s1 : y != v.ny - 1; };
+  // where the y neighbour comes from: the next row's LDS copy, or memory
+  const bool fy_lds = ly < kMY - 1 && y + 1 < v.ny;
+  const bool fx_mem = lx == kMX - 1 && hx;
   float qzp = 0.f, fc = 0.f;
   if (in_xy) {
     const unsigned i0 = (unsigned)z0 * s2 + row;
@@ -216,29 +225,35 @@ __device__ __forceinline__ void tv_march(const Vol3& v, const float* __restrict_
       qzp = q[2];
     }
   }
-  // the values of slice z are loaded during slice z - 1 (software pipelining:
-  // their latency overlaps the previous slice's barrier and update)
-  float fx = 0.f, fy = 0.f, fz = 0.f;
+  sf[z0 & 1][ly][lx] = fc;
+  // what slice z needs from memory is loaded during slice z - 1 (software
+  // pipelining): f one slice further up, the neighbours beyond the region
+  float fxm = 0.f, fym = 0.f, fz = 0.f;
   typename Out::Pre pre{};
   auto fetch = [&](int z, float& gx, float& gy, float& gz, typename Out::Pre& gp) {
     const unsigned i = (unsigned)z * s2 + row;
-    if (hx) gx = f[i + 1];
-    if (hy_at(i)) gy = f[i + s1];
+    if (fx_mem) gx = f[i + 1];
+    if (!fy_lds && hy_at(i)) gy = f[i + s1];
     if (z != v.nz - 1) gz = f[i + s2];
     if (out_lane) gp = out.load(i);
   };
-  if (in_xy && z0 < z1) fetch(z0, fx, fy, fz, pre);
+  if (in_xy && z0 < z1) fetch(z0, fxm, fym, fz, pre);
+  __syncthreads();  // the first slice's f in sf
   for (int z = z0; z < z1; ++z) {
     const unsigned i = (unsigned)z * s2 + row;
     const bool hy = hy_at(i), hz = z != v.nz - 1;
     float nfx = 0.f, nfy = 0.f, nfz = 0.f;
     typename Out::Pre npre{};
     if (in_xy && z + 1 < z1) fetch(z + 1, nfx, nfy, nfz, npre);
+    const int buf = z & 1;
+    const float right = __shfl_down(fc, 1, kMX);
+    const float fx = fx_mem ? fxm : right;
+    const float fy = fy_lds ? sf[buf][ly + 1][lx] : fym;
     float q[3] = {0.f, 0.f, 0.f};
     if (in_xy) qvec_vals<kAsd>(v, hx, hy, hz, fc, fx, fy, fz, eps, q);
-    const int buf = z & 1;
     sq[buf][0][ly][lx] = q[0];
     sq[buf][1][ly][lx] = q[1];
+    sf[buf ^ 1][ly][lx] = fz;  // read by the row below during slice z + 1
     __syncthreads();
     if (out_lane) {
       float gtv = -(q[0] + q[1] + q[2]);
@@ -259,8 +274,8 @@ __device__ __forceinline__ void tv_march(const Vol3& v, const float* __restrict_
     }
     qzp = q[2];
     fc = fz;
-    fx = nfx;
-    fy = nfy;
+    fxm = nfx;
+    fym = nfy;
     fz = nfz;
     pre = npre;
   }
