@@ -176,12 +176,15 @@ __global__ __launch_bounds__(kBpTileX) void k_bp_standard(
   const double cx = (ix - g.nx / 2.0) * s;
   const double cy = (iy - g.ny / 2.0) * s;
   const float wpi = (float)(kPiRef / g.n_views);
+  // a view's byte offsets fit 32 bits and its rows 24-bit multiplies
+  const bool offsets32 = (int64_t)g.nu * g.nv < (int64_t)1 << 29 && g.nu < (1 << 21);
   float acc[kBpZ];
 #pragma unroll
   for (int k = 0; k < kBpZ; ++k) acc[k] = 0.f;
   const float* sb = sino + (size_t)b * g.n_views * g.nu * g.nv;
   for (int a0 = 0; a0 < g.n_views; a0 += kBpViewChunk) {
-    const int na = min(kBpViewChunk, g.n_views - a0);
+    // wave-uniform (a scalar register), so the view loop below is uniform
+    const int na = __builtin_amdgcn_readfirstlane(min(kBpViewChunk, g.n_views - a0));
     __syncthreads();
     if (threadIdx.x < na) {
       const double th = g.angles[a0 + threadIdx.x] / 180.0 * kPiRef - kPiRef / 2.0;
@@ -189,7 +192,9 @@ __global__ __launch_bounds__(kBpTileX) void k_bp_standard(
       cs_[threadIdx.x][1] = sin(th);
     }
     __syncthreads();
-    if (ix < g.nx) {
+    // lanes past the volume edge run too (their taps are bounds-checked and
+    // their sums never written), so the view loop stays wave-uniform
+    {
       for (int j = 0; j < na; ++j) {
         const double c = cs_[j][0], sn = cs_[j][1];
         const double L = g.sid + (cx * c + cy * sn);
@@ -202,11 +207,42 @@ __global__ __launch_bounds__(kBpTileX) void k_bp_standard(
         const float w = (float)(m * m) * wpi;
         const float* img = sb + (size_t)(a0 + j) * g.nu * g.nv;
         double vv = (z0 - g.nz / 2.0) * dz + g.nv / 2.0;
+        // Fast path: the lane's kBpZ detector rows in 32.32 fixed point (row k
+        // at vv + k dz, formed only where that cannot overflow; dz > 0, so
+        // the rows increase with k and the first and last bound them all) and
+        // every tap on the detector, read through 32-bit byte offsets from the
+        // view's base.  Otherwise the bounds-checked gather of every tap.
+        bool fast = false;
+        int64_t V = 0, DV = 0;
+        if (offsets32 && (unsigned)iu < (unsigned)(g.nu - 1) && dz > 0.0 && dz < 1048576.0 &&
+            fabs(vv) < 536870912.0) {
+          V = to_fix(vv);
+          DV = to_fix(dz);
+          fast = (V >> 32) >= 0 && ((V + (kBpZ - 1) * DV) >> 32) < g.nv - 1;
+        }
+        if (fast) {
+          const char* base = reinterpret_cast<const char*>(img);
+          const unsigned row = 4u * (unsigned)g.nu;
 #pragma unroll
-        for (int k = 0; k < kBpZ; ++k) {
-          const double fv = floor(vv);
-          acc[k] += w * bil_checked(img, g.nu, g.nv, iu, fx, (int)fv, (float)(vv - fv));
-          vv += dz;
+          for (int k = 0; k < kBpZ; ++k) {
+            const unsigned o = __umul24((unsigned)(V >> 32), row) + 4u * (unsigned)iu;
+            const float fy = (float)(uint32_t)V * kInvFix;
+            const float t00 = *reinterpret_cast<const float*>(base + o);
+            const float t01 = *reinterpret_cast<const float*>(base + o + 4u);
+            const float t10 = *reinterpret_cast<const float*>(base + o + row);
+            const float t11 = *reinterpret_cast<const float*>(base + o + row + 4u);
+            const float a0 = t00 + fx * (t01 - t00);
+            const float a1 = t10 + fx * (t11 - t10);
+            acc[k] += w * (a0 + fy * (a1 - a0));
+            V += DV;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < kBpZ; ++k) {
+            const double fv = floor(vv);
+            acc[k] += w * bil_checked(img, g.nu, g.nv, iu, fx, (int)fv, (float)(vv - fv));
+            vv += dz;
+          }
         }
       }
     }
