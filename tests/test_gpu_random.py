@@ -1,0 +1,134 @@
+"""GPU parity over seeded random geometries: every kernel family against the
+FP64 oracle on small circular and Standard-scheme problems whose sizes,
+magnifications, detector shapes and view angles are drawn from fixed seeds
+(odd and non-multiple-of-tile sizes, short and wide detectors, irregular
+angles, sources close to the volume).  Bar: relative L2 <= 1e-5 for A, the
+matched A^T and the reference backprojector against the oracle, and an adjoint
+dot-product mismatch <= 1e-6 of ||Ax|| ||y|| (the inputs are N(0,1), whose
+dot products nearly cancel).
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+
+from oracle import cone
+from dtvproj import geometry as G, ops, phantom
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+LAUNCHES = [None, dict(algorithm=1), dict(algorithm=2, brick=(6, 5, 7), block=128),
+            dict(algorithm=2, brick=(16, 16, 8), block=256)]
+
+
+def _circ_case(seed):
+    rng = np.random.default_rng(seed)
+    X, Y, Z = (int(v) for v in rng.integers(5, 41, size=3))
+    vx = float(rng.uniform(0.5, 1.5))
+    U, V = (int(v) for v in rng.integers(8, 49, size=2))
+    half_diag = 0.5 * vx * float(np.sqrt(X * X + Y * Y + Z * Z))
+    # every fourth case puts the source just outside the volume's sphere
+    sod = half_diag * (1.1 if seed % 4 == 0 else float(rng.uniform(1.5, 6.0)))
+    sdd = sod * float(rng.uniform(1.2, 3.0))
+    # the detector covers the volume's shadow, more or less
+    cover = float(rng.uniform(0.6, 1.4))
+    pitch = (2.0 * half_diag * sdd / sod * cover / U, 2.0 * half_diag * sdd / sod * cover / V)
+    n = int(rng.integers(3, 17))
+    ang = np.sort(rng.uniform(0.0, 360.0, size=n))
+    return dict(vol=(X, Y, Z), vx=vx, det=(U, V), pitch=pitch, sod=sod, sdd=sdd, ang=ang)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_circular_geometry_vs_oracle(cuda, seed):
+    c = _circ_case(seed)
+    X, Y, Z = c["vol"]
+    U, V = c["det"]
+    print(f"[case] random circular seed={seed} vol={c['vol']} det={c['det']} "
+          f"views={len(c['ang'])} sod={c['sod']:.1f} sdd={c['sdd']:.1f}", flush=True)
+    g = G.circular(c["ang"], c["vol"], c["vx"], c["det"], c["pitch"], c["sod"], c["sdd"])
+    args = (c["ang"], c["vol"], c["vx"], c["det"], c["pitch"], c["sod"], c["sdd"])
+    x = phantom.gaussian((2, 1, Z, Y, X), 300 + seed)
+    y = phantom.gaussian((2, 1, len(c["ang"]), V, U), 400 + seed)
+    want_a = cone.circ_forward(x, *args)
+    want_t = cone.circ_adjoint(y, *args)
+    xd, yd = x.to(cuda), y.to(cuda)
+    for launch in LAUNCHES:
+        a = ops.forward(xd, g, launch)
+        t = ops.adjoint(yd, g, launch)
+        assert rel_l2(a, want_a) <= TOL, (launch, rel_l2(a, want_a))
+        assert rel_l2(t, want_t) <= TOL, (launch, rel_l2(t, want_t))
+        # N(0,1) inputs make <Ax, y> a near-cancelling sum, so the mismatch is
+        # measured against the Cauchy-Schwarz scale ||Ax|| ||y||
+        lhs = float((a.double().cpu() * y.double()).sum())
+        rhs = float((x.double() * t.double().cpu()).sum())
+        scale = float(a.double().norm()) * float(y.double().norm())
+        assert abs(lhs - rhs) <= 1e-6 * scale, launch
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_standard_reference_backprojector_vs_oracle(cuda, seed):
+    """The Standard reference backprojector over random sizes, distances and
+    short detectors, so that the 16-voxel z runs of its lanes start, end and
+    straddle the detector's top and bottom rows and the volume is narrower
+    than a wave."""
+    rng = np.random.default_rng(1000 + seed)
+    X, Y, Z = (int(v) for v in rng.integers(6, 70, size=3))
+    U = int(rng.integers(16, 96))
+    V = int(rng.integers(6, 40))
+    sid = float(rng.uniform(1.2, 4.0)) * 0.5 * float(np.hypot(X, Y))
+    sdd = sid * float(rng.uniform(1.3, 2.5))
+    ang = np.sort(rng.uniform(0.0, 360.0, size=int(rng.integers(3, 13))))
+    print(f"[case] random standard bp_ref seed={seed} vol={(X, Y, Z)} det={(U, V)} "
+          f"views={len(ang)} sid={sid:.1f} sdd={sdd:.1f}", flush=True)
+    g = G.standard(ang, (X, Y, Z), (U, V), sid, sdd)
+    y = phantom.gaussian((1, 1, len(ang), V, U), 500 + seed)
+    got = ops.bp_ref(y.to(cuda), g)
+    want = cone.std_bp_ref(y, ang, (X, Y, Z), (U, V), sid, sdd)
+    assert rel_l2(got, want) <= TOL
+
+
+def _rot(axis, deg):
+    c, s = np.cos(np.radians(deg)), np.sin(np.radians(deg))
+    i, j = [(1, 2), (0, 2), (0, 1)][axis]
+    m = np.eye(3)
+    m[i, i], m[i, j], m[j, i], m[j, j] = c, -s, s, c
+    return m
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_beijing_scan_vs_oracle(cuda, seed):
+    """Beijing view vectors (21 sub-detectors per source angle) with random
+    volume and strip sizes, distances and pitches, each view's detector
+    tilted by up to 3 degrees and shifted by up to a voxel, so no two views
+    share an orientation: A and the matched A^T on every kernel family, and
+    the reference backprojector."""
+    rng = np.random.default_rng(2000 + seed)
+    X, Y, Z = (int(v) for v in rng.integers(6, 41, size=3))
+    U, V = int(rng.integers(3, 13)), int(rng.integers(6, 33))
+    sid = float(rng.uniform(0.8, 4.0)) * 0.5 * float(np.hypot(X, Y))
+    sdd = sid + float(rng.uniform(10.0, 60.0))
+    pitch = (float(rng.uniform(0.4, 1.2)), float(rng.uniform(0.5, 2.0)))
+    P = np.asarray(G.beijing_default_vectors(int(rng.integers(1, 3)), 21, (U, V), sid, sdd, pitch),
+                   dtype=np.float64)
+    for r in P:
+        m = _rot(int(rng.integers(0, 3)), float(rng.uniform(-3.0, 3.0)))
+        r[3:6] += rng.uniform(-1.0, 1.0, size=3)
+        r[6:9] = m @ r[6:9]
+        r[9:12] = m @ r[9:12]
+    P = P.astype(np.float32)
+    print(f"[case] random beijing seed={seed} vol={(X, Y, Z)} det={(U, V)} views={len(P)} "
+          f"sid={sid:.1f} sdd={sdd:.1f}", flush=True)
+    vs, ds = (X, Y, Z), (U, V)
+    g = G.beijing(P, vs, ds)
+    x = phantom.gaussian((1, 1, Z, Y, X), 600 + seed)
+    y = phantom.gaussian(g.sino_shape(1), 700 + seed)
+    want_a = cone.bj_forward(x, P, vs, ds)
+    want_t = cone.bj_adjoint(y, P, vs, ds)
+    for launch in LAUNCHES:
+        a = ops.forward(x.to(cuda), g, launch)
+        t = ops.adjoint(y.to(cuda), g, launch)
+        assert rel_l2(a, want_a) <= TOL, (launch, rel_l2(a, want_a))
+        assert rel_l2(t, want_t) <= TOL, (launch, rel_l2(t, want_t))
+    assert rel_l2(ops.bp_ref(y.to(cuda), g), cone.bj_bp_ref(y, P, vs, ds)) <= TOL
