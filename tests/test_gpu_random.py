@@ -132,3 +132,49 @@ def test_random_beijing_scan_vs_oracle(cuda, seed):
         assert rel_l2(a, want_a) <= TOL, (launch, rel_l2(a, want_a))
         assert rel_l2(t, want_t) <= TOL, (launch, rel_l2(t, want_t))
     assert rel_l2(ops.bp_ref(y.to(cuda), g), cone.bj_bp_ref(y, P, vs, ds)) <= TOL
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_multi_instance_launch_vs_oracle(cuda, seed):
+    """dtv_cone_forward_multi / dtv_cone_adjoint_multi over 3-5 random
+    instances in one launch (random circular geometries, at times a random
+    Beijing scan among them), one shared brick shape per launch drawn from
+    compiled and run-time shapes, grid and persistent workgroups alternating
+    with the seed, each instance's output against the FP64 oracle."""
+    from dtvproj import batched
+    rng = np.random.default_rng(3000 + seed)
+    n = int(rng.integers(3, 6))
+    geoms, oracles = [], []
+    for i in range(n):
+        if i == n - 1 and seed % 2 == 0:
+            X, Y, Z = (int(v) for v in rng.integers(8, 33, size=3))
+            U, V = int(rng.integers(4, 12)), int(rng.integers(8, 25))
+            sid = float(rng.uniform(1.5, 3.0)) * 0.5 * float(np.hypot(X, Y))
+            P = G.beijing_default_vectors(int(rng.integers(1, 3)), 21, (U, V), sid,
+                                          sid + float(rng.uniform(20.0, 50.0)),
+                                          (float(rng.uniform(0.5, 1.0)), 1.0))
+            geoms.append(G.beijing(P, (X, Y, Z), (U, V)))
+            oracles.append((lambda P, vs, ds: (lambda x: cone.bj_forward(x, P, vs, ds),
+                                               lambda y: cone.bj_adjoint(y, P, vs, ds)))(
+                P, (X, Y, Z), (U, V)))
+        else:
+            c = _circ_case(3100 + 10 * seed + i)
+            geoms.append(G.circular(c["ang"], c["vol"], c["vx"], c["det"], c["pitch"], c["sod"],
+                                    c["sdd"]))
+            args = (c["ang"], c["vol"], c["vx"], c["det"], c["pitch"], c["sod"], c["sdd"])
+            oracles.append((lambda args: (lambda x: cone.circ_forward(x, *args),
+                                          lambda y: cone.circ_adjoint(y, *args)))(args))
+    shapes = [((24, 24, 14), (32, 16, 16)), ((16, 16, 8), (16, 16, 8)), ((6, 5, 7), (9, 7, 5))]
+    fb, ab = shapes[seed % len(shapes)]
+    persistent = 0 if seed % 2 else 23
+    print(f"[case] random multi seed={seed} instances={n} bricks={fb},{ab} "
+          f"persistent={persistent}", flush=True)
+    mp = batched.MultiProjector(geoms, bricks=[(fb, ab)] * n, block=256,
+                                persistent_blocks=persistent)
+    vols = [phantom.gaussian(g.volume_shape(1), 800 + 10 * seed + i) for i, g in enumerate(geoms)]
+    ys = [phantom.gaussian(g.sino_shape(1), 900 + 10 * seed + i) for i, g in enumerate(geoms)]
+    sinos = mp.forward([v.to(cuda) for v in vols])
+    outs = mp.adjoint([y.to(cuda) for y in ys])
+    for i, (fwd, adj) in enumerate(oracles):
+        assert rel_l2(sinos[i], fwd(vols[i].double())) <= TOL, f"forward {i}"
+        assert rel_l2(outs[i], adj(ys[i].double())) <= TOL, f"adjoint {i}"
