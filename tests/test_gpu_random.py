@@ -178,3 +178,44 @@ def test_random_multi_instance_launch_vs_oracle(cuda, seed):
     for i, (fwd, adj) in enumerate(oracles):
         assert rel_l2(sinos[i], fwd(vols[i].double())) <= TOL, f"forward {i}"
         assert rel_l2(outs[i], adj(ys[i].double())) <= TOL, f"adjoint {i}"
+
+
+def _rand_shape(rng, square=False):
+    B = int(rng.integers(1, 3))
+    Z = int(rng.integers(1, 40))
+    X = int(rng.choice([int(rng.integers(1, 90)), 4 * int(rng.integers(1, 80))]))
+    Y = X if square else int(rng.integers(1, 90))
+    return (B, 1, Z, Y, X)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_fused_volume_steps(cuda, seed):
+    """The fused FISTA-TV step, the ASD-POCS descent step and the PDHG/DTV
+    pair on random shapes (1 to 39 slices, widths with and without the
+    four-voxel march, square and not, so reference mode takes both the march
+    and the per-voxel kernels), both boundary modes, against their FP64
+    formulations whose difference operators meet the sparse matrices."""
+    from dtvproj import tv
+    from oracle import tv as otv
+    rng = np.random.default_rng(4000 + seed)
+    mode = ("neumann", "reference")[seed % 2]
+    shp = _rand_shape(rng, square=bool(seed % 3 == 0))
+    Z, Y, X = shp[-3:]
+    print(f"[case] random volume steps seed={seed} {mode} {shp}", flush=True)
+    y, g, xp = (phantom.gaussian(shp, 40 * seed + k) for k in range(3))
+    args = dict(step=0.3, lam=0.7, eps=0.05, beta=0.4, nonneg=True, mode=mode)
+    rx, ry = tv.fista_tv_step_torch(y.double(), g.double(), xp.double(), **args)
+    x, yn = tv.fista_tv_step(y.to(cuda), g.to(cuda), xp.to(cuda), **args)
+    assert rel_l2(x, rx) <= 1e-6 and rel_l2(yn, ry) <= 1e-6
+    want = y.double().numpy() - 0.25 * otv.asd_tv_grad(y.numpy(), 1.0, (X, Y, Z), mode)
+    assert rel_l2(tv.tv_descent(y.to(cuda), 0.25, 1.0, mode), torch.from_numpy(want)) <= 1e-6
+    f, fbar, atw = y - 0.3, g, xp
+    p = torch.stack([phantom.gaussian(shp, 40 * seed + 10 + a) - 0.4 for a in range(3)])
+    lam = (0.3, 0.5, 0.2)
+    rf, rfb, rp = tv.pdhg_dtv_step_torch(f.double(), fbar.double(), atw.double(), p.double(),
+                                         0.7, 0.4, lam, mode)
+    fd, fbd, pd = f.to(cuda), fbar.to(cuda), p.to(cuda)
+    tv.pdhg_dtv_step(fd, fbd, atw.to(cuda), pd, 0.7, 0.4, lam, mode)
+    assert rel_l2(fd, rf) <= 1e-6 and rel_l2(fbd, rfb) <= 1e-6
+    for a in range(3):
+        assert rel_l2(pd[a], rp[a]) <= 1e-6
