@@ -153,6 +153,12 @@ __device__ __forceinline__ int chunk_view(const Geo& g, const ChunkStage& st, in
   return 32 * w + pos;
 }
 
+// A brick's views may be split over nparts workgroups (grid z): part p takes
+// the chunked views [end(p - 1), end(p)), end(p) = n (p + 1) / nparts.
+__device__ __forceinline__ int view_part_end(int n, int part, int nparts) {
+  return (int)((long long)n * (part + 1) / nparts);
+}
+
 // Stages chunked views [c0, c0+n) of the box [lo, hi) (voxel frame) into st
 // (their view indices into st.vidx).
 __device__ void stage_chunk(const Geo& g, int c0, int n, const double lo[3], const double hi[3],
@@ -444,8 +450,8 @@ template <int CBX, int CBY, int CBZ>
 __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
                                                float* __restrict__ sbase, const Geo& g,
                                                int bx_, int by_, int bz_, int nbx, int nby,
-                                               int brick, float* tile, ChunkStage& st,
-                                               RaySched& rs) {
+                                               int brick, int part, int nparts, float* tile,
+                                               ChunkStage& st, RaySched& rs) {
   const int bx = CBX ? CBX : bx_, by = CBY ? CBY : by_, bz = CBZ ? CBZ : bz_;
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   // bricks tile floor(p) over [-1, N-1]; the tile holds voxels [o, o+b]
@@ -478,8 +484,9 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
     nlive = st.n_live;
   }
   int vslot = 0;  // st.sv slot of the next processed view
-  for (int c0 = 0; c0 < nlive; c0 += kChunk) {
-    const int nc = min(kChunk, nlive - c0);
+  const int vend = view_part_end(nlive, part, nparts);
+  for (int c0 = view_part_end(nlive, part - 1, nparts); c0 < vend; c0 += kChunk) {
+    const int nc = min(kChunk, vend - c0);
     __syncthreads();  // the previous chunk's readers are done with st
     stage_chunk(g, c0, nc, lo, hi, false, st);
     __syncthreads();
@@ -564,7 +571,7 @@ __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol
   const int b = blockIdx.y;
   fwd_brick_body<CBX, CBY, CBZ>(vol + (size_t)b * g.nx * g.ny * g.nz,
                  sino + (size_t)b * g.n_views * g.nv * g.nu, g, bx, by, bz, nbx, nby,
-                 blockIdx.x, tile, st, rs);
+                 blockIdx.x, blockIdx.z, gridDim.z, tile, st, rs);
 }
 
 constexpr int kAdjPerThread = 32;  // owned voxels per lane (bx*by*bz <= 32*block)
@@ -708,7 +715,8 @@ template <int CBX, int CBY, int CBZ>
 __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
                                                float* __restrict__ vb, const Geo& g, int bx_,
                                                int by_, int bz_, int nbx, int nby, int brick,
-                                               int* iacc, ChunkStage& st, RaySched& rs) {
+                                               int part, int nparts, int* iacc, ChunkStage& st,
+                                               RaySched& rs) {
   const int bx = CBX ? CBX : bx_, by = CBY ? CBY : by_, bz = CBZ ? CBZ : bz_;
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   const int x0 = jx * bx, y0 = jy * by, z0 = jz * bz;
@@ -736,8 +744,9 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
     nlive = st.n_live;
   }
   int vslot = 0;  // st.sv slot of the next processed view
-  for (int c0 = 0; c0 < nlive; c0 += kChunk) {
-    const int nc = min(kChunk, nlive - c0);
+  const int vend = view_part_end(nlive, part, nparts);
+  for (int c0 = view_part_end(nlive, part - 1, nparts); c0 < vend; c0 += kChunk) {
+    const int nc = min(kChunk, vend - c0);
     __syncthreads();  // the previous chunk's readers are done with st
     stage_chunk(g, c0, nc, lo, hi, true, st);
     __syncthreads();
@@ -876,8 +885,13 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
     const int i = threadIdx.x + j * blockDim.x;
     if (i < nown) {
       const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
-      if (lx < ex && ly < ey && lz < ez)
-        vb[((size_t)(z0 + lz) * g.ny + (y0 + ly)) * g.nx + (x0 + lx)] = vacc[j];
+      if (lx < ex && ly < ey && lz < ez) {
+        float* o = vb + ((size_t)(z0 + lz) * g.ny + (y0 + ly)) * g.nx + (x0 + lx);
+        if (nparts == 1)
+          *o = vacc[j];
+        else
+          unsafeAtomicAdd(o, vacc[j]);  // the output was zero-filled
+      }
     }
   }
 }
@@ -894,7 +908,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_brick(const float* __restrict__ 
   const int b = blockIdx.y;
   adj_brick_body<CBX, CBY, CBZ>(sino + (size_t)b * g.n_views * g.nv * g.nu,
                  vol + (size_t)b * g.nx * g.ny * g.nz, g, bx, by, bz, nbx, nby, blockIdx.x,
-                 iacc, st, rs);
+                 blockIdx.z, gridDim.z, iacc, st, rs);
 }
 
 // ---- uneven instances in one launch ----------------------------------------
@@ -944,7 +958,7 @@ __global__ __launch_bounds__(256) void k_fwd_brick_multi(const dtv_instance* __r
     const dtv_work_item it = work[w];
     const dtv_instance in = inst[it.instance];
     const Geo g = instance_geo(in);
-    fwd_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, tile, st, rs);
+    fwd_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, 0, 1, tile, st, rs);
   }
 }
 
@@ -963,7 +977,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_brick_multi(const dtv_instance* 
     const dtv_work_item it = work[w];
     const dtv_instance in = inst[it.instance];
     const Geo g = instance_geo(in);
-    adj_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, iacc, st, rs);
+    adj_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, 0, 1, iacc, st, rs);
   }
 }
 
@@ -983,7 +997,7 @@ __global__ void k_zero_int(int* p) { *p = 0; }
 namespace dtv {
 
 int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int bx, int by,
-                     int bz, int block, hipStream_t st) {
+                     int bz, int block, int parts, hipStream_t st) {
   const int nbx = (g.nx + 1 + bx - 1) / bx, nby = (g.ny + 1 + by - 1) / by,
             nbz = (g.nz + 1 + bz - 1) / bz;
   const size_t lds = fwd_tile_bytes(bx, by, bz);
@@ -992,13 +1006,13 @@ int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int
   if (bx == a && by == b && bz == c) kern = &k_fwd_brick<a, b, c>;
   DTV_BRICK_SHAPES(DTV_PICK)
 #undef DTV_PICK
-  hipLaunchKernelGGL(kern, dim3(nbx * nby * nbz, batch), dim3(block), lds, st, vol, sino, g, bx,
-                     by, bz, nbx, nby);
+  hipLaunchKernelGGL(kern, dim3(nbx * nby * nbz, batch, parts), dim3(block), lds, st, vol, sino,
+                     g, bx, by, bz, nbx, nby);
   return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
 int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int bx, int by,
-                     int bz, int block, hipStream_t st) {
+                     int bz, int block, int parts, hipStream_t st) {
   const int nbx = (g.nx + bx - 1) / bx, nby = (g.ny + by - 1) / by, nbz = (g.nz + bz - 1) / bz;
   if (bx * by * bz > kAdjPerThread * block) return 1;
   const size_t lds = adj_tile_bytes(bx, by, bz);
@@ -1007,8 +1021,8 @@ int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int
   if (bx == a && by == b && bz == c) kern = &k_adj_brick<a, b, c>;
   DTV_BRICK_SHAPES(DTV_PICK)
 #undef DTV_PICK
-  hipLaunchKernelGGL(kern, dim3(nbx * nby * nbz, batch), dim3(block), lds, st, sino, vol, g, bx,
-                     by, bz, nbx, nby);
+  hipLaunchKernelGGL(kern, dim3(nbx * nby * nbz, batch, parts), dim3(block), lds, st, sino, vol,
+                     g, bx, by, bz, nbx, nby);
   return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 

@@ -30,9 +30,13 @@ constexpr size_t kMaxTileBytes = 56 * 1024;
 // managed, host or fine-grained memory and pointers HIP does not know.
 int check_atomic_target(const void* p);
 
+// parts: workgroups per brick, each taking an equal share of the brick's views
+// (grid z).  A's parts add into the sinogram like whole bricks; A^T's parts add
+// into an output the caller has zero-filled (with two parts the sum does not
+// depend on their order, since FP32 addition is commutative).
 int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int bx, int by,
-                     int bz, int block, hipStream_t st);
+                     int bz, int block, int parts, hipStream_t st);
 int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int bx, int by,
-                     int bz, int block, hipStream_t st);
+                     int bz, int block, int parts, hipStream_t st);
 
 }  // namespace dtv

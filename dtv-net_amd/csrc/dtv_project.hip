@@ -468,6 +468,8 @@ struct Launch {
   int algo = DTV_ALGO_BRICK;
   int bx = 24, by = 24, bz = 14;
   int block = 256;
+  int parts = 1;  // workgroups sharing one brick's views
+  bool parts_requested = false;  // set by the caller (else a default)
 };
 
 // Bricks of shape (bx, by, bz) covering a volume: A's bricks tile floor(p)
@@ -478,27 +480,52 @@ long long brick_count(const Geo& g, int bx, int by, int bz, bool adjoint) {
          ((g.nz + e + bz - 1) / bz);
 }
 
-// Default kernels, by problem size (measured on MI355X, 256 CUs):
-//  * at least one workgroup per compute unit (>= 256 with the batch): A on
-//    24x24x14 bricks, A^T on 32x16x16 (fastest at 256^3 / 60 views, 4 x 128^3
-//    / 90 views and the 256x256x64 / 3024-view Beijing scan; 160^3 / 60
-//    views: A 1.03 vs 1.47 ms, A^T 1.24 vs 1.44 ms on 16x16x8);
-//  * fewer: 16x16x8 bricks (96^3 / 72 views: A 0.64 vs 0.98 ms, A^T 0.85 vs
-//    1.39 ms; still so at 720 views), and for A the one-lane-per-ray kernel
-//    when even those give under 256 workgroups (64^3 / 72 views: 0.58 vs
-//    0.79 ms; 32^3: 0.03 vs 0.22 ms).
+// Default kernels, by problem size (measured on MI355X, 256 CUs, dense inputs):
+//  * A: 24x24x14 bricks with their views split over parts = ceil(4608 /
+//    (bricks x batch)) workgroups per brick, at most 16 (about six rounds of the
+//    768 workgroups the chip holds at once): 256^3 / 60 views 2.62 -> 2.56 ms
+//    (3 parts), 128^3 / 90 views 1.79 ms (16x16x16, 1 part) -> 0.96 ms (13),
+//    64^3 / 72 views 0.61 ms (ray kernel) -> 0.21 ms (16), the 3024-view
+//    Beijing scan 7.75 -> 6.59 ms (8).  Only when even 16 parts would give
+//    under 256 workgroups, the one-lane-per-ray kernel (32^3 / 24 views:
+//    0.042 ms against 0.050 on bricks).
+//  * A^T: 32x16x16 bricks when they number at least 256 with the batch, else
+//    16x16x8 (96^3 / 72 views: 0.85 vs 1.39 ms before the split), and the views
+//    split over two workgroups per brick that add into a zero-filled output
+//    (FP32 addition is commutative, so two parts stay bitwise reproducible):
+//    256^3 / 60 views on the phantom's projection 2.88 -> 2.74 ms, 128^3 / 90
+//    views 3.60 -> 2.63 ms, 64^3 / 72 views 1.12 -> 0.60 ms.  Outputs the
+//    hardware atomics cannot address (managed or host memory) keep one part.
+constexpr long long kFwdTargetGroups = 4608;
+constexpr int kFwdMaxParts = 16;
+
+// Parts per brick for a brick shape when the caller does not fix them.
+int default_parts(bool adjoint, const Geo& g, int batch, int bx, int by, int bz) {
+  if (adjoint) return 2;
+  const long long nb = brick_count(g, bx, by, bz, false) * batch;
+  return (int)std::max<long long>(
+      1, std::min<long long>(kFwdMaxParts, (kFwdTargetGroups + nb - 1) / nb));
+}
+
 Launch default_launch(bool adjoint, const Geo& g, int batch) {
   Launch L;
   if (adjoint) {
     L.bx = 32;
     L.by = 16;
     L.bz = 16;
+    if (brick_count(g, L.bx, L.by, L.bz, true) * batch < 256) {
+      L.bx = 16;
+      L.by = 16;
+      L.bz = 8;
+    }
+    L.parts = 2;
+    return L;
   }
-  if (brick_count(g, L.bx, L.by, L.bz, adjoint) * batch >= 256) return L;
-  L.bx = 16;
-  L.by = 16;
-  L.bz = 8;
-  if (!adjoint && brick_count(g, 16, 16, 8, false) * batch < 256) L.algo = DTV_ALGO_RAY;
+  if (brick_count(g, L.bx, L.by, L.bz, false) * batch * kFwdMaxParts < 256) {
+    L.algo = DTV_ALGO_RAY;
+    return L;
+  }
+  L.parts = default_parts(false, g, batch, L.bx, L.by, L.bz);
   return L;
 }
 
@@ -525,6 +552,14 @@ int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint, const Geo& g,
     if (l->block < 64 || l->block > 256 || l->block % 64) return DTV_ERR_INVALID_ARGUMENT;
     out.block = l->block;
   }
+  if (l->view_parts) {
+    if (l->view_parts < 1 || l->view_parts > (adjoint ? 2 : kFwdMaxParts))
+      return DTV_ERR_INVALID_ARGUMENT;
+    out.parts = l->view_parts;
+    out.parts_requested = true;
+  } else {
+    out.parts = default_parts(adjoint, g, batch, out.bx, out.by, out.bz);
+  }
   return DTV_OK;
 }
 
@@ -544,7 +579,7 @@ int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipS
     if (rc) return rc;
     rc = run_zero(sino, (size_t)batch * g.n_views * g.nu * g.nv, st);
     if (rc) return rc;
-    return launch_fwd_brick(vol, sino, batch, g, L.bx, L.by, L.bz, L.block, st);
+    return launch_fwd_brick(vol, sino, batch, g, L.bx, L.by, L.bz, L.block, L.parts, st);
   }
   const dim3 block(kRayTileU, kRayTileV);
   const dim3 grid((g.nu + kRayTileU - 1) / kRayTileU, (g.nv + kRayTileV - 1) / kRayTileV,
@@ -559,8 +594,24 @@ int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipS
 
 int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st,
                 const Launch& L) {
-  if (L.algo == DTV_ALGO_BRICK)
-    return launch_adj_brick(sino, vol, batch, g, L.bx, L.by, L.bz, L.block, st);
+  if (L.algo == DTV_ALGO_BRICK) {
+    int parts = L.parts;
+    if (parts > 1) {
+      // the parts add into the output with FP32 global atomics
+      int rc = check_atomic_target(vol);
+      if (rc) {
+        if (!L.parts_requested) {
+          parts = 1;
+        } else {
+          return rc;
+        }
+      } else {
+        rc = run_zero(vol, (size_t)batch * g.nx * g.ny * g.nz, st);
+        if (rc) return rc;
+      }
+    }
+    return launch_adj_brick(sino, vol, batch, g, L.bx, L.by, L.bz, L.block, parts, st);
+  }
   int rc = check_atomic_target(vol);
   if (rc) return rc;
   rc = run_zero(vol, (size_t)batch * g.nx * g.ny * g.nz, st);

@@ -39,19 +39,21 @@ class DtvLaunch(ctypes.Structure):
         ("algorithm", c_int32),
         ("brick", c_int32 * 3),
         ("block", c_int32),
-        ("reserved", c_int32 * 3),
+        ("view_parts", c_int32),
+        ("reserved", c_int32 * 2),
     ]
 
 
 ALGO_DEFAULT, ALGO_RAY, ALGO_BRICK = 0, 1, 2
 
 
-def launch_struct(algorithm=0, brick=(0, 0, 0), block=0) -> DtvLaunch:
+def launch_struct(algorithm=0, brick=(0, 0, 0), block=0, view_parts=0) -> DtvLaunch:
     l = DtvLaunch()
     l.algorithm = int(algorithm)
     for i in range(3):
         l.brick[i] = int(brick[i])
     l.block = int(block)
+    l.view_parts = int(view_parts)
     return l
 
 

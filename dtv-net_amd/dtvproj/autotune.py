@@ -1,5 +1,5 @@
-"""Autotuner for the brick kernels' tile shape (brick edge lengths) and ray
-bundle size (lanes per workgroup).
+"""Autotuner for the brick kernels' tile shape (brick edge lengths), ray
+bundle size (lanes per workgroup) and view split (workgroups per brick).
 
 For a geometry it times every candidate launch for A and for the matched
 A^T on the device (median of ``reps`` timed calls after one warm-up, HIP
@@ -36,8 +36,19 @@ def signature(geom: ConeGeometry) -> str:
                        sc.axis_mode, round(float(sc.margin), 6)])
 
 
+# view splits tried per shape: A the library's default for the shape (0) and
+# none; A^T one part and two
+FORWARD_PARTS = (0, 1)
+ADJOINT_PARTS = (1, 2)
+
+
 def launch_of(c):
-    return dict(algorithm=2, brick=tuple(c[:3]), block=c[3])
+    """(bx, by, bz, lanes[, view_parts]) -> launch dict (view_parts 0 or
+    absent: the library's default split for the shape)."""
+    L = dict(algorithm=2, brick=tuple(c[:3]), block=c[3])
+    if len(c) > 4 and c[4]:
+        L["view_parts"] = int(c[4])
+    return L
 
 
 def _time(fn, reps):
@@ -82,14 +93,22 @@ def tune(geom: ConeGeometry, device=None, reps: int = 3, candidates=CANDIDATES,
     y = torch.rand(geom.sino_shape(1), generator=gen).to(device)
     times = {"forward": {}, "adjoint": {}}
     for c in candidates:
-        L = launch_of(c)
         if c[0] * c[1] * c[2] > 32 * c[3]:
             continue
-        times["forward"][str(c)] = _time(lambda: ops.forward(x, geom, L), reps)
-        times["adjoint"][str(c)] = _time(lambda: ops.adjoint(y, geom, L), reps)
+        for p in FORWARD_PARTS:
+            cf = tuple(c[:4]) + (p,)
+            L = launch_of(cf)
+            times["forward"][str(cf)] = _time(lambda: ops.forward(x, geom, L), reps)
+        for p in ADJOINT_PARTS:
+            ca = tuple(c[:4]) + (p,)
+            L = launch_of(ca)
+            times["adjoint"][str(ca)] = _time(lambda: ops.adjoint(y, geom, L), reps)
         if verbose:
-            print(f"tune {c}: fwd {times['forward'][str(c)]:.3f} ms  "
-                  f"adj {times['adjoint'][str(c)]:.3f} ms", flush=True)
+            print(f"tune {c}: fwd " + " ".join(
+                f"{times['forward'][str(tuple(c[:4]) + (p,))]:.3f}" for p in FORWARD_PARTS) +
+                " ms  adj " + " ".join(
+                f"{times['adjoint'][str(tuple(c[:4]) + (p,))]:.3f}" for p in ADJOINT_PARTS) +
+                " ms", flush=True)
     best_f = min(times["forward"], key=times["forward"].get)
     best_a = min(times["adjoint"], key=times["adjoint"].get)
     bf = tuple(json.loads(best_f.replace("(", "[").replace(")", "]")))

@@ -47,7 +47,10 @@ typedef struct dtv_launch {
   int32_t algorithm;   /* 0: default, 1: ray-driven (gather / atomic scatter), 2: brick (LDS) */
   int32_t brick[3];    /* brick edge in voxels (x, y, z) for algorithm 2 */
   int32_t block;       /* threads per workgroup for algorithm 2 (64..256) */
-  int32_t reserved[3];
+  int32_t view_parts;  /* algorithm 2: workgroups sharing one brick's views (0 = default;
+                          A: 1..16; A^T: 1 or 2, with the output zero-filled and added to,
+                          so it must be coarse-grained device memory) */
+  int32_t reserved[2];
 } dtv_launch;
 
 #define DTV_ALGO_DEFAULT 0

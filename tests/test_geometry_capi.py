@@ -130,6 +130,14 @@ def test_capi_rejects_bad_arguments_without_gpu():
                                 _capi.int3((2048, 2048, 1024)), _capi.int2((64, 64)),
                                 views.ctypes.data_as(_capi.c_double_p), 4, ctypes.byref(sc),
                                 None) == 3
+    # view splits beyond what the kernels take (A: 16 parts, A^T: 2)
+    views4 = np.zeros((4, 12))
+    vp = views4.ctypes.data_as(_capi.c_double_p)
+    for fn, parts in ((lib.dtv_cone_forward_ex, 17), (lib.dtv_cone_forward_ex, -1),
+                      (lib.dtv_cone_adjoint_ex, 3)):
+        L = _capi.launch_struct(algorithm=2, brick=(16, 16, 8), block=256, view_parts=parts)
+        assert fn(ctypes.c_void_p(16), ctypes.c_void_p(16), 1, _capi.int3((8, 8, 8)),
+                  _capi.int2((8, 8)), vp, 4, ctypes.byref(sc), ctypes.byref(L), None) == 1
     # TV steps and the row filter: null pointers, aliasing, bad sizes and modes
     p = ctypes.c_void_p(16)
     q = ctypes.c_void_p(32)

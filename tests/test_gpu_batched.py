@@ -40,7 +40,9 @@ def test_multi_matches_single_instance_calls(cuda, persistent):
     ys = [phantom.gaussian(g.sino_shape(1), 60 + i).to(cuda) for i, g in enumerate(geoms)]
     outs = mp.adjoint(ys)
     for i, g in enumerate(geoms):
-        ref = ops.adjoint(ys[i], g, dict(algorithm=2, brick=BRICKS[i][1], block=256))
+        # one workgroup per brick, as in the multi-instance launch
+        ref = ops.adjoint(ys[i], g, dict(algorithm=2, brick=BRICKS[i][1], block=256,
+                                         view_parts=1))
         assert torch.equal(outs[i], ref)   # integer accumulation: bitwise equal
 
 
@@ -65,7 +67,8 @@ def test_multi_with_a_culled_many_view_instance(cuda, persistent):
     ys = [phantom.gaussian(g.sino_shape(1), 80 + i).to(cuda) for i, g in enumerate(geoms)]
     outs = mp.adjoint(ys)
     for i, g in enumerate(geoms):
-        ref = ops.adjoint(ys[i], g, dict(algorithm=2, brick=bricks[i][1], block=256))
+        ref = ops.adjoint(ys[i], g, dict(algorithm=2, brick=bricks[i][1], block=256,
+                                         view_parts=1))
         assert torch.equal(outs[i], ref)
 
 
