@@ -958,7 +958,10 @@ __global__ __launch_bounds__(256) void k_fwd_brick_multi(const dtv_instance* __r
     const dtv_work_item it = work[w];
     const dtv_instance in = inst[it.instance];
     const Geo g = instance_geo(in);
-    fwd_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, 0, 1, tile, st, rs);
+    const int nb = in.nbx * in.nby * in.nbz;
+    fwd_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby,
+                                  it.brick % nb, it.brick / nb, max(1, in.view_parts), tile, st,
+                                  rs);
   }
 }
 
@@ -977,8 +980,22 @@ __global__ __launch_bounds__(256, 3) void k_adj_brick_multi(const dtv_instance* 
     const dtv_work_item it = work[w];
     const dtv_instance in = inst[it.instance];
     const Geo g = instance_geo(in);
-    adj_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick, 0, 1, iacc, st, rs);
+    const int nb = in.nbx * in.nby * in.nbz;
+    adj_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby,
+                                  it.brick % nb, it.brick / nb, max(1, in.view_parts), iacc, st,
+                                  rs);
   }
+}
+
+// Adjoint instances whose views are split: out = 0 (the parts add into it),
+// blockIdx.y = instance.
+__global__ void k_zero_split_multi(const dtv_instance* __restrict__ inst) {
+  const dtv_instance in = inst[blockIdx.y];
+  if (in.view_parts <= 1) return;
+  const size_t n = (size_t)in.nx * in.ny * in.nz;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    in.out[i] = 0.f;
 }
 
 // out[i] = -init[i] (or 0) for every instance's output, blockIdx.y = instance
@@ -1059,6 +1076,7 @@ int validate_multi(const dtv_instance* ih, int32_t n_inst, const dtv_work_item* 
       l = fwd_tile_bytes(in.bx, in.by, in.bz);
     }
     if (l > kMaxTileBytes) return DTV_ERR_INVALID_ARGUMENT;
+    if (in.view_parts < 0 || in.view_parts > (adjoint ? 2 : 16)) return DTV_ERR_INVALID_ARGUMENT;
     lds = l > lds ? l : lds;
     const size_t no = (size_t)in.n_views * in.nu * in.nv;
     max_out = no > max_out ? no : max_out;
@@ -1066,7 +1084,8 @@ int validate_multi(const dtv_instance* ih, int32_t n_inst, const dtv_work_item* 
   for (int w = 0; w < n_work; ++w) {
     if (wh[w].instance < 0 || wh[w].instance >= n_inst) return DTV_ERR_INVALID_ARGUMENT;
     const dtv_instance& in = ih[wh[w].instance];
-    if (wh[w].brick < 0 || wh[w].brick >= in.nbx * in.nby * in.nbz) return DTV_ERR_INVALID_ARGUMENT;
+    const long long units = (long long)in.nbx * in.nby * in.nbz * std::max(1, in.view_parts);
+    if (wh[w].brick < 0 || wh[w].brick >= units) return DTV_ERR_INVALID_ARGUMENT;
   }
   (void)max_out;
   return DTV_OK;
@@ -1108,6 +1127,20 @@ int launch_multi(bool adjoint, const dtv_instance* ih, const dtv_instance* id, i
     const unsigned gx = (unsigned)std::min<size_t>((max_out + 255) / 256, 4096);
     hipLaunchKernelGGL(k_init_multi, dim3(gx, n_inst), dim3(256), 0, st, id);
     if (hipGetLastError() != hipSuccess) return DTV_ERR_HIP;
+  } else {
+    size_t max_split = 0;
+    for (int i = 0; i < n_inst; ++i) {
+      if (ih[i].view_parts <= 1) continue;
+      rc = check_atomic_target(ih[i].out);
+      if (rc) return rc;
+      const size_t nv = (size_t)ih[i].nx * ih[i].ny * ih[i].nz;
+      max_split = nv > max_split ? nv : max_split;
+    }
+    if (max_split) {
+      const unsigned gx = (unsigned)std::min<size_t>((max_split + 255) / 256, 4096);
+      hipLaunchKernelGGL(k_zero_split_multi, dim3(gx, n_inst), dim3(256), 0, st, id);
+      if (hipGetLastError() != hipSuccess) return DTV_ERR_HIP;
+    }
   }
   const bool persistent = counter && persistent_blocks > 0;
   if (persistent) {

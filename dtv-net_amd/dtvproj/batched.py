@@ -33,7 +33,7 @@ class Instance(ctypes.Structure):
         ("axis_mode", ctypes.c_int32),
         ("bx", ctypes.c_int32), ("by", ctypes.c_int32), ("bz", ctypes.c_int32),
         ("nbx", ctypes.c_int32), ("nby", ctypes.c_int32), ("nbz", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("view_parts", ctypes.c_int32),
         ("box_lo", ctypes.c_double * 3), ("box_hi", ctypes.c_double * 3),
         ("margin", ctypes.c_double), ("scale", ctypes.c_double),
     ]
@@ -66,9 +66,15 @@ class MultiProjector:
     (instance, brick) items from a device counter until the list is empty;
     ``"auto"`` (default) = as many as are resident at once for each kernel
     (the library caps the grid by the kernel's LDS and registers: three per
-    compute unit for the default bricks); 0 = one workgroup per item."""
+    compute unit for the default bricks); 0 = one workgroup per item.
+    ``view_parts``: per-instance (forward, adjoint) splits of each brick's views
+    over work items (forward at most 16, adjoint at most 2), a single int for
+    every instance, or ``"auto"`` (default): bricks whose views would cost more
+    than a sixth of a workgroup slot's share of the whole launch are split, so
+    the launch has enough similar-sized items to balance (:func:`auto_parts`)."""
 
-    def __init__(self, geoms, bricks=None, block=256, persistent_blocks="auto", device=None):
+    def __init__(self, geoms, bricks=None, block=256, persistent_blocks="auto", device=None,
+                 view_parts="auto"):
         self.geoms = list(geoms)
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.block = block
@@ -88,6 +94,12 @@ class MultiProjector:
             self.tuned = autotune.tune_multi(self.geoms, self.device)
             bricks = [(self.tuned["forward"], self.tuned["adjoint"])] * len(self.geoms)
         self.bricks = bricks or [((24, 24, 14), (32, 16, 16))] * len(self.geoms)
+        if view_parts == "auto":
+            cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+            view_parts = auto_parts(self.geoms, self.bricks, 3 * cus)
+        elif isinstance(view_parts, int):
+            view_parts = [(view_parts, view_parts)] * len(self.geoms)
+        self.view_parts = [(int(f), int(a)) for f, a in view_parts]
         self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.views = [g.device_views(self.device) for g in self.geoms]
 
@@ -116,9 +128,11 @@ class MultiProjector:
                 e.box_hi[c] = g.scheme.box_hi[c]
             e.margin = g.scheme.margin
             e.scale = g.scheme.scale
+            parts = self.view_parts[i][1 if adjoint else 0]
+            e.view_parts = parts
             nb = e.nbx * e.nby * e.nbz
-            cost = float(g.n_views * b[0] * b[1] * b[2])
-            work += [(cost, i, k) for k in range(nb)]
+            cost = float(g.n_views * b[0] * b[1] * b[2]) / parts
+            work += [(cost, i, k) for k in range(nb * parts)]
         work.sort(key=lambda t: (-t[0], t[1], t[2]))
         wh = (WorkItem * len(work))()
         for j, (_, i, k) in enumerate(work):
@@ -155,13 +169,40 @@ class MultiProjector:
         return vols
 
 
+def auto_parts(geoms, bricks, slots):
+    """Per-instance (forward, adjoint) view splits for a multi launch: with
+    per-brick cost c_i = views x brick voxels and a target item cost of the
+    launch's total over 6 x ``slots`` (about six items per resident
+    workgroup), A splits a brick into ceil(c_i / target) parts (at most 16)
+    and A^T into two when c_i exceeds twice the target."""
+    def count(g, b, adjoint):
+        e = 0 if adjoint else 1
+        X, Y, Z = g.volume_size
+        return ((X + e + b[0] - 1) // b[0]) * ((Y + e + b[1] - 1) // b[1]) * \
+            ((Z + e + b[2] - 1) // b[2])
+
+    out = []
+    costs = []
+    for adjoint in (False, True):
+        c = [g.n_views * b[int(adjoint)][0] * b[int(adjoint)][1] * b[int(adjoint)][2]
+             for g, b in zip(geoms, bricks)]
+        total = sum(ci * count(g, b[int(adjoint)], adjoint) for ci, g, b in zip(c, geoms, bricks))
+        costs.append((c, total / (6.0 * slots)))
+    (cf, tf), (ca, ta) = costs
+    for i in range(len(geoms)):
+        pf = max(1, min(16, -(-cf[i] // max(tf, 1e-30))))
+        pa = 2 if ca[i] > 2.0 * ta else 1
+        out.append((int(pf), int(pa)))
+    return out
+
+
 class BatchedFistaTV:
     """``iters`` FISTA-TV iterations over uneven instances (3 launches per
     iteration, all tables static, graph-capturable)."""
 
     def __init__(self, geoms, sinos, iters=20, lam=1e-3, eps=1e-3, steps=None, bricks=None,
-                 block=256, persistent_blocks="auto", mode="neumann"):
-        self.mp = MultiProjector(geoms, bricks, block, persistent_blocks)
+                 block=256, persistent_blocks="auto", mode="neumann", view_parts="auto"):
+        self.mp = MultiProjector(geoms, bricks, block, persistent_blocks, view_parts=view_parts)
         dev = self.mp.device
         self.geoms = self.mp.geoms
         self.iters = iters

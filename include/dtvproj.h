@@ -151,7 +151,11 @@ int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,
  * 0 when init is NULL, then accumulated: out = A in - init).  Adjoint: in =
  * sinogram, out = volume (fully overwritten).  bx..bz is this instance's
  * brick shape, nbx..nbz its brick grid (forward bricks tile [-1, N-1]:
- * ceil((N+1)/b); adjoint bricks tile [0, N-1]: ceil(N/b)). */
+ * ceil((N+1)/b); adjoint bricks tile [0, N-1]: ceil(N/b)).  view_parts
+ * (0 or 1: none) splits each brick's views over that many work items
+ * (forward at most 16, adjoint at most 2; an adjoint instance with two parts
+ * has its volume zero-filled by the call and added into, so it must be
+ * coarse-grained device memory). */
 typedef struct dtv_instance {
   const float* in;
   float* out;
@@ -161,12 +165,13 @@ typedef struct dtv_instance {
   int32_t axis_mode;
   int32_t bx, by, bz;
   int32_t nbx, nby, nbz;
-  int32_t reserved;
+  int32_t view_parts;
   double box_lo[3], box_hi[3];
   double margin, scale;
 } dtv_instance;
 
-/* One brick of one instance. */
+/* One brick of one instance, or one part of its views: brick = b + nb * part
+ * for brick b < nb = nbx * nby * nbz and part < the instance's view_parts. */
 typedef struct dtv_work_item {
   int32_t instance;
   int32_t brick;

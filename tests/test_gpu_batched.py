@@ -30,7 +30,8 @@ BRICKS = [((32, 16, 16), (32, 16, 16)), ((16, 16, 8), (8, 8, 8)), ((7, 5, 3), (1
 def test_multi_matches_single_instance_calls(cuda, persistent):
     geoms = _small_uneven()
     _case("multi_vs_single", persistent=persistent)
-    mp = batched.MultiProjector(geoms, bricks=BRICKS, block=256, persistent_blocks=persistent)
+    mp = batched.MultiProjector(geoms, bricks=BRICKS, block=256, persistent_blocks=persistent,
+                                view_parts=1)
     vols = [phantom.gaussian(g.volume_shape(1), 40 + i).to(cuda) for i, g in enumerate(geoms)]
     inits = [phantom.gaussian(g.sino_shape(1), 50 + i).to(cuda) for i, g in enumerate(geoms)]
     sinos = mp.forward(vols, inits)
@@ -58,7 +59,8 @@ def test_multi_with_a_culled_many_view_instance(cuda, persistent):
              G.circular(G.uniform_angles(11), (40, 40, 24), 1.0, (60, 30), 1.6, 200.0, 400.0)]
     bricks = [((16, 16, 8), (16, 16, 8)), ((24, 24, 14), (32, 16, 16))]
     _case("multi_culled", persistent=persistent, views=[g.n_views for g in geoms])
-    mp = batched.MultiProjector(geoms, bricks=bricks, block=256, persistent_blocks=persistent)
+    mp = batched.MultiProjector(geoms, bricks=bricks, block=256, persistent_blocks=persistent,
+                                view_parts=1)
     vols = [phantom.gaussian(g.volume_shape(1), 70 + i).to(cuda) for i, g in enumerate(geoms)]
     sinos = mp.forward(vols)
     for i, g in enumerate(geoms):
@@ -97,7 +99,9 @@ def test_multi_instance_kernels_vs_fp64_oracle(cuda, persistent):
     bricks = [((24, 24, 14), (32, 16, 16)), ((16, 16, 8), (8, 8, 8)), ((7, 5, 3), (16, 16, 16)),
               ((16, 16, 8), (16, 16, 8))]
     _case("multi vs oracle", persistent=persistent, views=[g.n_views for g in geoms])
-    mp = batched.MultiProjector(geoms, bricks=bricks, block=256, persistent_blocks=persistent)
+    # bricks split over several items (A up to 5 parts, A^T 2) beside whole ones
+    mp = batched.MultiProjector(geoms, bricks=bricks, block=256, persistent_blocks=persistent,
+                                view_parts=[(3, 2), (1, 1), (5, 2), (4, 2)])
     vols = [phantom.gaussian(g.volume_shape(1), 140 + i) for i, g in enumerate(geoms)]
     inits = [phantom.gaussian(g.sino_shape(1), 150 + i) for i, g in enumerate(geoms)]
     ys = [phantom.gaussian(g.sino_shape(1), 160 + i) for i, g in enumerate(geoms)]

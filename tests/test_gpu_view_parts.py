@@ -3,15 +3,13 @@
 whole bricks, A^T's two parts add into a zero-filled volume.  Every split
 meets the FP64 oracle at the projector bar, agrees with the unsplit launch to
 FP32 rounding, and the two-part A^T is bitwise repeatable."""
-import ctypes
-
 import pytest
 import torch
 
 from conftest import rel_l2
 
 from oracle import cone
-from dtvproj import _capi, geometry as G, ops, phantom
+from dtvproj import geometry as G, ops, phantom
 
 pytestmark = pytest.mark.gpu
 
@@ -113,3 +111,44 @@ def test_view_parts_out_of_range_rejected(cuda):
         ops.forward(x, g, dict(algorithm=2, brick=(16, 16, 8), block=256, view_parts=17))
     with pytest.raises(RuntimeError):
         ops.adjoint(y, g, dict(algorithm=2, brick=(16, 16, 8), block=256, view_parts=3))
+
+
+@pytest.mark.parametrize("persistent", [0, 29])
+def test_multi_instance_parts_match_single_calls(cuda, persistent):
+    """Split bricks inside a multi-instance launch (one item per brick part)
+    give what the single-instance calls give with the same split: A to FP32
+    rounding, the two-part A^T bitwise (same views per part, integer groups,
+    two commutative adds)."""
+    from dtvproj import batched
+    geoms = [G.circular(G.uniform_angles(13), (33, 29, 21), 1.0, (50, 36), 1.3, 120.0, 260.0),
+             G.circular(G.uniform_angles(9) + 5.0, (24, 24, 30), 1.1, (40, 44), 1.4, 100.0, 230.0)]
+    parts = [(4, 2), (7, 2)]
+    bricks = [((16, 16, 8), (16, 16, 8)), ((24, 24, 14), (8, 8, 8))]
+    _case("multi_parts_vs_single", persistent=persistent, parts=parts)
+    mp = batched.MultiProjector(geoms, bricks=bricks, block=256, persistent_blocks=persistent,
+                                view_parts=parts)
+    vols = [phantom.gaussian(g.volume_shape(1), 47 + i).to(cuda) for i, g in enumerate(geoms)]
+    ys = [phantom.gaussian(g.sino_shape(1), 49 + i).to(cuda) for i, g in enumerate(geoms)]
+    sinos = mp.forward(vols)
+    outs = mp.adjoint(ys)
+    for i, g in enumerate(geoms):
+        f = ops.forward(vols[i], g, dict(algorithm=2, brick=bricks[i][0], block=256,
+                                         view_parts=parts[i][0]))
+        assert rel_l2(sinos[i], f) <= 1e-6
+        a = ops.adjoint(ys[i], g, dict(algorithm=2, brick=bricks[i][1], block=256,
+                                       view_parts=parts[i][1]))
+        assert torch.equal(outs[i], a)
+
+
+def test_auto_parts_split_the_costly_bricks():
+    """The multi launch's automatic split: instances whose bricks cost more
+    than the launch's target item get more parts, A^T at most two."""
+    from dtvproj import batched, configs
+    geoms = configs.uneven_16()
+    bricks = [((24, 24, 14), (32, 16, 16))] * len(geoms)
+    parts = batched.auto_parts(geoms, bricks, 768)
+    assert len(parts) == len(geoms)
+    assert all(1 <= f <= 16 and a in (1, 2) for f, a in parts)
+    cost = [g.n_views for g in geoms]
+    hi, lo = max(range(len(geoms)), key=cost.__getitem__), min(range(len(geoms)), key=cost.__getitem__)
+    assert parts[hi][0] >= parts[lo][0]
