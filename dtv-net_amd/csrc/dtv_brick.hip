@@ -458,7 +458,7 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
   const int ox = -1 + jx * bx, oy = -1 + jy * by, oz = -1 + jz * bz;
   const int ex = min(bx, g.nx - ox), ey = min(by, g.ny - oy), ez = min(bz, g.nz - oz);
   const int tx = bx + 1, ty = by + 1, tz = bz + 1;
-  const int sty = tx, stz = fwd_tile_stz(bx, by);
+  const int sty = fwd_tile_sty(bx), stz = fwd_tile_stz(bx, by);
   __syncthreads();  // the previous item of a persistent workgroup is done with LDS
   for (int i = threadIdx.x; i < tx * ty * tz; i += blockDim.x) {
     const int lx = i % tx, ly = (i / tx) % ty, lz = i / (tx * ty);
@@ -660,7 +660,7 @@ __device__ __forceinline__ void flush_owned(int* iacc, float (&vacc)[kAdjPerThre
   if (kFast && blockDim.x == 256) {
     // voxel j of lane tid: row q + R (j % P) of plane j / P, q = tid / bx
     constexpr int R = kFast ? 256 / CBX : 1, P = kFast ? CBY / R : 1;
-    constexpr int SY = CBX + 3, SZ = (CBX + 3) * (CBY + 2) + 1;
+    constexpr int SY = adj_tile_sty(CBX), SZ = adj_tile_stz(CBX, CBY);
     const int base = SZ + (tid / (kFast ? CBX : 1) + 1) * SY + tid % (kFast ? CBX : 1) + 1;
     if (fixed) {
 #pragma unroll
@@ -721,7 +721,7 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   const int x0 = jx * bx, y0 = jy * by, z0 = jz * bz;
   const int ex = min(bx, g.nx - x0), ey = min(by, g.ny - y0), ez = min(bz, g.nz - z0);
-  const int sty = bx + 3, stz = (bx + 3) * (by + 2) + 1;
+  const int sty = adj_tile_sty(bx), stz = adj_tile_stz(bx, by);
   const int nown = bx * by * bz;
   __syncthreads();  // the previous item of a persistent workgroup is done with LDS
   for (int i = threadIdx.x; i < stz * (bz + 2); i += blockDim.x) iacc[i] = 0;

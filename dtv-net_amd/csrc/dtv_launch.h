@@ -13,15 +13,36 @@ namespace dtv {
 // scheduling state (~5 KiB), a workgroup fits in 64 KiB of LDS.
 // A's z stride is padded to 17 (mod 32) so that rays one detector row apart,
 // which sit one z plane apart in the tile, fall on far-apart banks.
-__host__ __device__ inline int fwd_tile_stz(int bx, int by) {
-  const int p = (bx + 1) * (by + 1);
-  return p + ((17 - p) & 31);
+#ifndef DTV_FWD_STY_PAD
+#define DTV_FWD_STY_PAD 0
+#endif
+#ifndef DTV_FWD_STZ_RES
+#define DTV_FWD_STZ_RES 17
+#endif
+#ifndef DTV_FWD_STZ_MOD
+#define DTV_FWD_STZ_MOD 32
+#endif
+#ifndef DTV_ADJ_STY_PAD
+#define DTV_ADJ_STY_PAD 1
+#endif
+#ifndef DTV_ADJ_STZ_PAD
+#define DTV_ADJ_STZ_PAD 1
+#endif
+__host__ __device__ constexpr int fwd_tile_sty(int bx) { return bx + 1 + DTV_FWD_STY_PAD; }
+__host__ __device__ constexpr int fwd_tile_stz(int bx, int by) {
+  return fwd_tile_sty(bx) * (by + 1) +
+         ((DTV_FWD_STZ_RES - fwd_tile_sty(bx) * (by + 1)) & (DTV_FWD_STZ_MOD - 1));
+}
+// A^T rows hold the brick's bx voxels, the halo on both sides and padding.
+__host__ __device__ constexpr int adj_tile_sty(int bx) { return bx + 2 + DTV_ADJ_STY_PAD; }
+__host__ __device__ constexpr int adj_tile_stz(int bx, int by) {
+  return adj_tile_sty(bx) * (by + 2) + DTV_ADJ_STZ_PAD;
 }
 inline size_t fwd_tile_bytes(int bx, int by, int bz) {
   return (size_t)fwd_tile_stz(bx, by) * (bz + 1) * 4;
 }
 inline size_t adj_tile_bytes(int bx, int by, int bz) {
-  return (size_t)((bx + 3) * (by + 2) + 1) * (bz + 2) * 4;
+  return (size_t)adj_tile_stz(bx, by) * (bz + 2) * 4;
 }
 constexpr size_t kMaxTileBytes = 56 * 1024;
 
