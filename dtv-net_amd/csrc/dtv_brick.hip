@@ -32,8 +32,8 @@ using namespace dtv;
 // Brick shapes compiled with constant dimensions (the autotuner's candidates);
 // any other shape runs the run-time-shape kernels.
 #define DTV_BRICK_SHAPES(X) \
-  X(32, 16, 16) X(24, 24, 14) X(32, 32, 8) X(16, 16, 32) X(16, 16, 16) X(32, 16, 8) X(16, 16, 8) \
-  X(8, 8, 8)
+  X(32, 16, 16) X(26, 26, 14) X(24, 24, 14) X(32, 32, 8) X(16, 16, 32) X(16, 16, 16) X(32, 16, 8) \
+  X(16, 16, 8) X(8, 8, 8)
 
 
 namespace {

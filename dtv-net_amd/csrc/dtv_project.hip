@@ -466,7 +466,7 @@ int run_zero(float* p, size_t n, hipStream_t st) {
 // Resolved kernel choice.
 struct Launch {
   int algo = DTV_ALGO_BRICK;
-  int bx = 24, by = 24, bz = 14;
+  int bx = 26, by = 26, bz = 14;
   int block = 256;
   int parts = 1;  // workgroups sharing one brick's views
   bool parts_requested = false;  // set by the caller (else a default)
@@ -481,14 +481,19 @@ long long brick_count(const Geo& g, int bx, int by, int bz, bool adjoint) {
 }
 
 // Default kernels, by problem size (measured on MI355X, 256 CUs, dense inputs):
-//  * A: 24x24x14 bricks with their views split over parts = ceil(4608 /
-//    (bricks x batch)) workgroups per brick, at most 16 (about six rounds of the
-//    768 workgroups the chip holds at once): 256^3 / 60 views 2.62 -> 2.56 ms
-//    (3 parts), 128^3 / 90 views 1.79 ms (16x16x16, 1 part) -> 0.96 ms (13),
-//    64^3 / 72 views 0.61 ms (ray kernel) -> 0.21 ms (16), the 3024-view
-//    Beijing scan 7.75 -> 6.59 ms (8).  Only when even 16 parts would give
-//    under 256 workgroups, the one-lane-per-ray kernel (32^3 / 24 views:
-//    0.042 ms against 0.050 on bricks).
+//  * A: 26x26x14 bricks (45 KB of tile: three workgroups per CU, the most a
+//    forward tile can hold at that occupancy; with the views split it beat
+//    24x24x14 at 256^3 / 60 views 2.47 vs 2.60 ms, 128^3 / 90 views 0.92 vs
+//    1.01 ms, 512x512x384 / 120 views 27.1 vs 28.8 ms and the 3024-view
+//    Beijing scan 6.40 vs 6.64 ms; odd row strides only: 25x25x14 lost to
+//    both) with their views split over parts = ceil(4608 / (bricks x batch))
+//    workgroups per brick, at most 16 (about six rounds of the 768 workgroups
+//    the chip holds at once): 256^3 / 60 views 2.62 -> 2.56 ms (3 parts, on
+//    24x24x14), 128^3 / 90 views 1.79 ms (16x16x16, 1 part) -> 0.96 ms (13),
+//    64^3 / 72 views 0.61 ms (ray kernel) -> 0.21 ms (16), the Beijing scan
+//    7.75 -> 6.59 ms (8).  Only when even 16 parts would give under 256
+//    workgroups, the one-lane-per-ray kernel (32^3 / 24 views: 0.042 ms
+//    against 0.050 on bricks).
 //  * A^T: 32x16x16 bricks when they number at least 256 with the batch, else
 //    16x16x8 (96^3 / 72 views: 0.85 vs 1.39 ms before the split), and the views
 //    split over two workgroups per brick that add into a zero-filled output

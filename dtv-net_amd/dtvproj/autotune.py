@@ -21,7 +21,7 @@ from .geometry import ConeGeometry
 
 # (bx, by, bz, lanes); the adjoint needs bx*by*bz <= 32 * lanes
 CANDIDATES = [
-    (32, 16, 16, 256), (24, 24, 14, 256), (32, 32, 8, 256), (16, 16, 32, 256),
+    (32, 16, 16, 256), (26, 26, 14, 256), (24, 24, 14, 256), (32, 32, 8, 256), (16, 16, 32, 256),
     (16, 16, 16, 256),
     (32, 16, 8, 256), (16, 16, 16, 128), (16, 16, 8, 128), (8, 8, 8, 64),
 ]
@@ -93,22 +93,20 @@ def tune(geom: ConeGeometry, device=None, reps: int = 3, candidates=CANDIDATES,
     y = torch.rand(geom.sino_shape(1), generator=gen).to(device)
     times = {"forward": {}, "adjoint": {}}
     for c in candidates:
-        if c[0] * c[1] * c[2] > 32 * c[3]:
-            continue
         for p in FORWARD_PARTS:
             cf = tuple(c[:4]) + (p,)
             L = launch_of(cf)
             times["forward"][str(cf)] = _time(lambda: ops.forward(x, geom, L), reps)
-        for p in ADJOINT_PARTS:
-            ca = tuple(c[:4]) + (p,)
-            L = launch_of(ca)
-            times["adjoint"][str(ca)] = _time(lambda: ops.adjoint(y, geom, L), reps)
+        # the adjoint's lanes own at most 32 voxels each
+        if c[0] * c[1] * c[2] <= 32 * c[3]:
+            for p in ADJOINT_PARTS:
+                ca = tuple(c[:4]) + (p,)
+                L = launch_of(ca)
+                times["adjoint"][str(ca)] = _time(lambda: ops.adjoint(y, geom, L), reps)
         if verbose:
-            print(f"tune {c}: fwd " + " ".join(
-                f"{times['forward'][str(tuple(c[:4]) + (p,))]:.3f}" for p in FORWARD_PARTS) +
-                " ms  adj " + " ".join(
-                f"{times['adjoint'][str(tuple(c[:4]) + (p,))]:.3f}" for p in ADJOINT_PARTS) +
-                " ms", flush=True)
+            print(f"tune {c}: " + " ".join(f"{k} {v:.3f}" for d in times.values()
+                                           for k, v in d.items() if k.startswith(str(c[:4])[:-1])),
+                  flush=True)
     best_f = min(times["forward"], key=times["forward"].get)
     best_a = min(times["adjoint"], key=times["adjoint"].get)
     bf = tuple(json.loads(best_f.replace("(", "[").replace(")", "]")))
@@ -141,17 +139,20 @@ def tune_multi(geoms, device=None, reps: int = 3, candidates=CANDIDATES,
     outs_v = [torch.empty_like(x) for x in xs]
     times = {"forward": {}, "adjoint": {}}
     for c in candidates:
-        if c[3] != 256 or c[0] * c[1] * c[2] > 32 * c[3]:
+        if c[3] != 256:
             continue
         shape = tuple(c[:3])
-        mp = batched.MultiProjector(geoms, [(shape, shape)] * len(geoms), device=device)
+        adj_ok = c[0] * c[1] * c[2] <= 32 * c[3]   # at most 32 owned voxels per lane
+        adj_shape = shape if adj_ok else (32, 16, 16)
+        mp = batched.MultiProjector(geoms, [(shape, adj_shape)] * len(geoms), device=device)
         bf = mp.bind_forward(xs, outs_s)
-        ba = mp.bind_adjoint(ys, outs_v)
         times["forward"][str(shape)] = _time(lambda: mp.run(bf, False), reps)
-        times["adjoint"][str(shape)] = _time(lambda: mp.run(ba, True), reps)
+        if adj_ok:
+            ba = mp.bind_adjoint(ys, outs_v)
+            times["adjoint"][str(shape)] = _time(lambda: mp.run(ba, True), reps)
         if verbose:
             print(f"tune_multi {shape}: fwd {times['forward'][str(shape)]:.3f} ms  "
-                  f"adj {times['adjoint'][str(shape)]:.3f} ms", flush=True)
+                  f"adj {times['adjoint'].get(str(shape), float('nan')):.3f} ms", flush=True)
     parse = lambda k: tuple(json.loads(k.replace("(", "[").replace(")", "]")))
     best_f = parse(min(times["forward"], key=times["forward"].get))
     best_a = parse(min(times["adjoint"], key=times["adjoint"].get))

@@ -59,7 +59,7 @@ class MultiProjector:
     """A / matched A^T over uneven instances, one kernel launch per call.
 
     ``bricks``: per-instance (forward, adjoint) brick shapes; default
-    24x24x14 / 32x16x16 (the single-call defaults), ``"tune"`` to run the
+    26x26x14 / 32x16x16 (the single-call defaults), ``"tune"`` to run the
     autotuner per instance, or ``"tune_shared"`` for the one A shape and one
     A^T shape that time fastest on this multi launch (one compiled shape
     keeps the launch on its constant-shape kernel).  ``persistent_blocks``: workgroups that pull
@@ -93,7 +93,7 @@ class MultiProjector:
         elif bricks == "tune_shared":
             self.tuned = autotune.tune_multi(self.geoms, self.device)
             bricks = [(self.tuned["forward"], self.tuned["adjoint"])] * len(self.geoms)
-        self.bricks = bricks or [((24, 24, 14), (32, 16, 16))] * len(self.geoms)
+        self.bricks = bricks or [((26, 26, 14), (32, 16, 16))] * len(self.geoms)
         if view_parts == "auto":
             cus = torch.cuda.get_device_properties(self.device).multi_processor_count
             view_parts = auto_parts(self.geoms, self.bricks, 3 * cus)

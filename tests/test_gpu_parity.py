@@ -105,8 +105,10 @@ def test_beijing_forward_adjoint_bp_jit(cuda):
     assert rel_l2(bp, cone.bj_bp_ref(y, P, vs, ds)) <= TOL
 
 
-@pytest.mark.parametrize("brick", [(16, 16, 8), (24, 24, 14), (32, 16, 16)])
-def test_beijing_many_narrow_views_culled_per_brick(cuda, brick):
+@pytest.mark.parametrize("brick,adj_brick", [((16, 16, 8), (16, 16, 8)), ((24, 24, 14), (24, 24, 14)),
+                                             ((26, 26, 14), (32, 16, 16)),
+                                             ((32, 16, 16), (32, 16, 16))])
+def test_beijing_many_narrow_views_culled_per_brick(cuda, brick, adj_brick):
     """147 views (7 angles x 21 sub-detectors 8 pixels wide) is more than four
     chunks, so each brick first drops the views whose detector its box cannot
     reach; most sub-detectors see only a few bricks.  A and A^T still meet the
@@ -125,7 +127,7 @@ def test_beijing_many_narrow_views_culled_per_brick(cuda, brick):
     assert float((ref == 0).double().mean()) > 0.3
     assert bool((out[ref == 0] == 0).all())
     y = phantom.gaussian(g.sino_shape(1), 32)
-    adj = ops.adjoint(y.to(cuda), g, L)
+    adj = ops.adjoint(y.to(cuda), g, dict(L, brick=adj_brick))
     assert rel_l2(adj, cone.bj_adjoint(y, P, vs, ds)) <= TOL
 
 
@@ -216,9 +218,10 @@ BRICKS = [dict(algorithm=2, brick=(16, 16, 16), block=256),
           dict(algorithm=2, brick=(8, 8, 8), block=64),
           dict(algorithm=2, brick=(32, 16, 8), block=128),
           dict(algorithm=2, brick=(7, 5, 3), block=64),
-          # the large-volume defaults (A, A^T)
-          dict(algorithm=2, brick=(24, 24, 14), block=256),
-          dict(algorithm=2, brick=(32, 16, 16), block=256)]
+          # the large-volume defaults (A, A^T) and the former A default
+          dict(algorithm=2, brick=(26, 26, 14), block=256),
+          dict(algorithm=2, brick=(32, 16, 16), block=256),
+          dict(algorithm=2, brick=(24, 24, 14), block=256)]
 
 
 @pytest.mark.parametrize("launch", BRICKS, ids=lambda l: "x".join(map(str, l["brick"])))
@@ -234,6 +237,12 @@ def test_brick_vs_ray_and_oracle(cuda, launch):
     fr = ops.forward(x.to(cuda), g, RAY)
     fb = ops.forward(x.to(cuda), g, launch)
     ar = ops.adjoint(y.to(cuda), g, RAY)
+    bx, by, bz = launch["brick"]
+    if bx * by * bz > 32 * launch["block"]:
+        # more voxels than the adjoint's lanes can own (32 each): refused
+        with pytest.raises(RuntimeError, match="invalid argument"):
+            ops.adjoint(y.to(cuda), g, launch)
+        launch = dict(launch, brick=(32, 16, 16))
     ab = ops.adjoint(y.to(cuda), g, launch)
     assert rel_l2(fb, fr) <= 1e-6
     # the brick adjoint accumulates each group of views in 32-bit fixed point

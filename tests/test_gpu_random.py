@@ -164,7 +164,8 @@ def test_random_multi_instance_launch_vs_oracle(cuda, seed):
             args = (c["ang"], c["vol"], c["vx"], c["det"], c["pitch"], c["sod"], c["sdd"])
             oracles.append((lambda args: (lambda x: cone.circ_forward(x, *args),
                                           lambda y: cone.circ_adjoint(y, *args)))(args))
-    shapes = [((24, 24, 14), (32, 16, 16)), ((16, 16, 8), (16, 16, 8)), ((6, 5, 7), (9, 7, 5))]
+    shapes = [((26, 26, 14), (32, 16, 16)), ((16, 16, 8), (16, 16, 8)), ((6, 5, 7), (9, 7, 5)),
+              ((24, 24, 14), (32, 16, 16))]
     fb, ab = shapes[seed % len(shapes)]
     persistent = 0 if seed % 2 else 23
     print(f"[case] random multi seed={seed} instances={n} bricks={fb},{ab} "

@@ -31,7 +31,7 @@ def _circ():
     return g, args
 
 
-@pytest.mark.parametrize("brick", [(16, 16, 8), (24, 24, 14), (8, 8, 8)])
+@pytest.mark.parametrize("brick", [(16, 16, 8), (26, 26, 14), (8, 8, 8)])
 def test_forward_parts_meet_oracle(cuda, brick):
     """1 to 16 parts, including more parts than the 17 views split evenly
     (parts with one view or none)."""

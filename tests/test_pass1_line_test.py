@@ -90,8 +90,8 @@ def test_pass1_line_test_keeps_every_ray_with_a_sample(name):
     iv = iv.astype(np.float32)
     N = g.volume_size
     missed_at_negative = 0
-    for trial in range(6):
-        b = [(24, 24, 14), (32, 16, 16), (16, 16, 8)][trial % 3]
+    for trial in range(8):
+        b = [(24, 24, 14), (32, 16, 16), (16, 16, 8), (26, 26, 14)][trial % 4]
         # the forward's box: floor(p) in [o, o+b-1], o in [-1, N-1]
         o = [int(rng.integers(-1, max(0, N[c] - b[c] // 2))) for c in range(3)]
         flo = o
