@@ -494,13 +494,16 @@ long long brick_count(const Geo& g, int bx, int by, int bz, bool adjoint) {
 //    7.75 -> 6.59 ms (8).  Only when even 16 parts would give under 256
 //    workgroups, the one-lane-per-ray kernel (32^3 / 24 views: 0.042 ms
 //    against 0.050 on bricks).
-//  * A^T: 32x16x16 bricks when they number at least 256 with the batch, else
-//    16x16x8 (96^3 / 72 views: 0.85 vs 1.39 ms before the split), and the views
-//    split over two workgroups per brick that add into a zero-filled output
-//    (FP32 addition is commutative, so two parts stay bitwise reproducible):
-//    256^3 / 60 views on the phantom's projection 2.88 -> 2.74 ms, 128^3 / 90
-//    views 3.60 -> 2.63 ms, 64^3 / 72 views 1.12 -> 0.60 ms.  Outputs the
-//    hardware atomics cannot address (managed or host memory) keep one part.
+//  * A^T: the views split over two workgroups per brick that add into a
+//    zero-filled output (FP32 addition is commutative, so two parts stay
+//    bitwise reproducible): 256^3 / 60 views on the phantom's projection 2.88
+//    -> 2.74 ms, 64^3 / 72 views 1.12 -> 0.60 ms.  Bricks of 32x16x16 when
+//    they number at least 512 with the batch (256^3, 512x512x384, the Beijing
+//    scan), else 16x16x16 when those number at least 200 (128^3 / 90 views
+//    2.33 vs 2.61 ms, 160^3 / 60 views 2.74 vs 2.83, 96^3 / 72 views 1.36 vs
+//    1.48 on 16x16x8), else 16x16x8 (64^3: 0.59 vs 0.91 ms on 16x16x16).
+//    Outputs the hardware atomics cannot address (managed or host memory)
+//    keep one part.
 constexpr long long kFwdTargetGroups = 4608;
 constexpr int kFwdMaxParts = 16;
 
@@ -518,10 +521,10 @@ Launch default_launch(bool adjoint, const Geo& g, int batch) {
     L.bx = 32;
     L.by = 16;
     L.bz = 16;
-    if (brick_count(g, L.bx, L.by, L.bz, true) * batch < 256) {
+    if (brick_count(g, 32, 16, 16, true) * batch < 512) {
       L.bx = 16;
       L.by = 16;
-      L.bz = 8;
+      L.bz = brick_count(g, 16, 16, 16, true) * batch >= 200 ? 16 : 8;
     }
     L.parts = 2;
     return L;
