@@ -715,8 +715,8 @@ template <int CBX, int CBY, int CBZ>
 __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
                                                float* __restrict__ vb, const Geo& g, int bx_,
                                                int by_, int bz_, int nbx, int nby, int brick,
-                                               int part, int nparts, int* iacc, ChunkStage& st,
-                                               RaySched& rs) {
+                                               int part, int nparts, bool add, int* iacc,
+                                               ChunkStage& st, RaySched& rs) {
   const int bx = CBX ? CBX : bx_, by = CBY ? CBY : by_, bz = CBZ ? CBZ : bz_;
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   const int x0 = jx * bx, y0 = jy * by, z0 = jz * bz;
@@ -887,7 +887,7 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
       const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
       if (lx < ex && ly < ey && lz < ez) {
         float* o = vb + ((size_t)(z0 + lz) * g.ny + (y0 + ly)) * g.nx + (x0 + lx);
-        if (nparts == 1)
+        if (!add)
           *o = vacc[j];
         else
           unsafeAtomicAdd(o, vacc[j]);  // the output was zero-filled
@@ -896,19 +896,37 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
   }
 }
 
-// at least 3 waves per SIMD (<= 168 VGPRs)
+// At least 3 waves per SIMD (<= 168 VGPRs).  ws == nullptr: the parts (grid
+// z) add into vol (zero-filled when there is more than one); else part p
+// stores its partial volumes into ws[p][batch][voxels], which k_sum_parts
+// then adds up in part order.
 template <int CBX, int CBY, int CBZ>
 __global__ __launch_bounds__(256, 3) void k_adj_brick(const float* __restrict__ sino,
-                                                   float* __restrict__ vol, Geo g, int bx,
+                                                   float* __restrict__ vol,
+                                                   float* __restrict__ ws, Geo g, int bx,
                                                    int by, int bz, int nbx, int nby) {
   extern __shared__ int iacc[];
   __shared__ ChunkStage st;
   __shared__ RaySched rs;
   if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
   const int b = blockIdx.y;
-  adj_brick_body<CBX, CBY, CBZ>(sino + (size_t)b * g.n_views * g.nv * g.nu,
-                 vol + (size_t)b * g.nx * g.ny * g.nz, g, bx, by, bz, nbx, nby, blockIdx.x,
-                 blockIdx.z, gridDim.z, iacc, st, rs);
+  const size_t nvox = (size_t)g.nx * g.ny * g.nz;
+  float* out = ws ? ws + ((size_t)blockIdx.z * gridDim.y + b) * nvox : vol + (size_t)b * nvox;
+  adj_brick_body<CBX, CBY, CBZ>(sino + (size_t)b * g.n_views * g.nv * g.nu, out, g, bx, by, bz,
+                                nbx, nby, blockIdx.x, blockIdx.z, gridDim.z,
+                                !ws && gridDim.z > 1, iacc, st, rs);
+}
+
+// out[i] = ws[0][i] + ws[1][i] + ... in part order (FP32, one fixed order: the
+// split adjoint is bitwise reproducible for any number of parts)
+__global__ void k_sum_parts(const float* __restrict__ ws, float* __restrict__ out, size_t n,
+                            int parts) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    float s = ws[i];
+    for (int p = 1; p < parts; ++p) s += ws[(size_t)p * n + i];
+    out[i] = s;
+  }
 }
 
 // ---- uneven instances in one launch ----------------------------------------
@@ -982,8 +1000,8 @@ __global__ __launch_bounds__(256, 3) void k_adj_brick_multi(const dtv_instance* 
     const Geo g = instance_geo(in);
     const int nb = in.nbx * in.nby * in.nbz;
     adj_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby,
-                                  it.brick % nb, it.brick / nb, max(1, in.view_parts), iacc, st,
-                                  rs);
+                                  it.brick % nb, it.brick / nb, max(1, in.view_parts),
+                                  in.view_parts > 1, iacc, st, rs);
   }
 }
 
@@ -1028,8 +1046,8 @@ int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int
   return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
-int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int bx, int by,
-                     int bz, int block, int parts, hipStream_t st) {
+int launch_adj_brick(const float* sino, float* vol, float* ws, int batch, const Geo& g, int bx,
+                     int by, int bz, int block, int parts, hipStream_t st) {
   const int nbx = (g.nx + bx - 1) / bx, nby = (g.ny + by - 1) / by, nbz = (g.nz + bz - 1) / bz;
   if (bx * by * bz > kAdjPerThread * block) return 1;
   const size_t lds = adj_tile_bytes(bx, by, bz);
@@ -1039,7 +1057,13 @@ int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int
   DTV_BRICK_SHAPES(DTV_PICK)
 #undef DTV_PICK
   hipLaunchKernelGGL(kern, dim3(nbx * nby * nbz, batch, parts), dim3(block), lds, st, sino, vol,
-                     g, bx, by, bz, nbx, nby);
+                     ws, g, bx, by, bz, nbx, nby);
+  if (hipGetLastError() != hipSuccess) return 4;
+  if (ws && parts > 1) {
+    const size_t n = (size_t)batch * g.nx * g.ny * g.nz;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_sum_parts, dim3(blocks), dim3(256), 0, st, ws, vol, n, parts);
+  }
   return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 

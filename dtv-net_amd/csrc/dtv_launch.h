@@ -52,12 +52,14 @@ constexpr size_t kMaxTileBytes = 56 * 1024;
 int check_atomic_target(const void* p);
 
 // parts: workgroups per brick, each taking an equal share of the brick's views
-// (grid z).  A's parts add into the sinogram like whole bricks; A^T's parts add
-// into an output the caller has zero-filled (with two parts the sum does not
-// depend on their order, since FP32 addition is commutative).
+// (grid z).  A's parts add into the sinogram like whole bricks.  A^T's parts
+// either add into an output the caller has zero-filled (ws == nullptr; with
+// two parts the sum does not depend on their order, since FP32 addition is
+// commutative) or store their partial volumes into ws (parts x batch x voxels
+// floats), which a second kernel adds up in part order into vol.
 int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int bx, int by,
                      int bz, int block, int parts, hipStream_t st);
-int launch_adj_brick(const float* sino, float* vol, int batch, const Geo& g, int bx, int by,
-                     int bz, int block, int parts, hipStream_t st);
+int launch_adj_brick(const float* sino, float* vol, float* ws, int batch, const Geo& g, int bx,
+                     int by, int bz, int block, int parts, hipStream_t st);
 
 }  // namespace dtv

@@ -494,39 +494,60 @@ long long brick_count(const Geo& g, int bx, int by, int bz, bool adjoint) {
 //    7.75 -> 6.59 ms (8).  Only when even 16 parts would give under 256
 //    workgroups, the one-lane-per-ray kernel (32^3 / 24 views: 0.042 ms
 //    against 0.050 on bricks).
-//  * A^T: the views split over two workgroups per brick that add into a
-//    zero-filled output (FP32 addition is commutative, so two parts stay
-//    bitwise reproducible): 256^3 / 60 views on the phantom's projection 2.88
-//    -> 2.74 ms, 64^3 / 72 views 1.12 -> 0.60 ms.  Bricks of 32x16x16 when
-//    they number at least 512 with the batch (256^3, 512x512x384, the Beijing
-//    scan), else 16x16x16 when those number at least 200 (128^3 / 90 views
-//    2.33 vs 2.61 ms, 160^3 / 60 views 2.74 vs 2.83, 96^3 / 72 views 1.36 vs
-//    1.48 on 16x16x8), else 16x16x8 (64^3: 0.59 vs 0.91 ms on 16x16x16).
-//    Outputs the hardware atomics cannot address (managed or host memory)
-//    keep one part.
+//  * A^T, through the workspace entry point (each part stores its partial
+//    volumes, a second kernel adds them in part order: bitwise reproducible
+//    for any split): 32x16x16 bricks (16x16x16 when fewer than 128 of those,
+//    16x16x8 below 32 of 16^3), parts = ceil(4096 / (bricks x batch)), at most
+//    16 (N(0,1) sinograms: 256^3 / 60 views 3.92 ms with 2 parts against 4.01
+//    unsplit, 160^3 / 60 views 2.20 ms (9 parts) against 3.51, 128^3 / 90
+//    views 1.87 ms (16) against 3.58, 64^3 / 72 views 0.37 ms (16, 16^3)
+//    against 1.12 on 16x16x8, the Beijing scan 15.6 ms (8) against 17.4; at
+//    512x512x384 / 120 views any split was slower, 46.0 unsplit against 50.2
+//    ms).  Without a workspace (the reference-shaped entry points) at most two
+//    parts can add into a zero-filled output (FP32 addition is commutative,
+//    so the sum stays bitwise reproducible): two below 4096 bricks, on
+//    32x16x16 bricks when they number at least 512 with the batch, else
+//    16x16x16 when those number at least 200, else 16x16x8 (128^3: 2.33 vs
+//    2.61 ms, 64^3: 0.59 vs 0.91 ms).  Outputs the hardware atomics cannot
+//    address (managed or host memory) keep one part.
+#ifndef DTV_ADJ_TARGET_GROUPS
+#define DTV_ADJ_TARGET_GROUPS 4096
+#endif
 constexpr long long kFwdTargetGroups = 4608;
 constexpr int kFwdMaxParts = 16;
 
-// Parts per brick for a brick shape when the caller does not fix them.
-int default_parts(bool adjoint, const Geo& g, int batch, int bx, int by, int bz) {
-  if (adjoint) return 2;
-  const long long nb = brick_count(g, bx, by, bz, false) * batch;
+constexpr long long kAdjTargetGroups = DTV_ADJ_TARGET_GROUPS;
+constexpr int kAdjMaxParts = 16;      // with a workspace
+constexpr int kAdjMaxAtomicParts = 2;  // adding into the output
+
+// Parts per brick for a brick shape when the caller does not fix them.  A^T
+// without a workspace takes two (bitwise reproducible atomic adds); with one,
+// enough parts for about kAdjTargetGroups workgroups (at least two).
+int default_parts(bool adjoint, const Geo& g, int batch, int bx, int by, int bz,
+                  bool ws = false) {
+  const long long nb = brick_count(g, bx, by, bz, adjoint) * batch;
+  if (adjoint && !ws) return nb < kAdjTargetGroups ? kAdjMaxAtomicParts : 1;
+  if (adjoint)
+    return (int)std::max<long long>(
+        1, std::min<long long>(kAdjMaxParts, (kAdjTargetGroups + nb - 1) / nb));
   return (int)std::max<long long>(
       1, std::min<long long>(kFwdMaxParts, (kFwdTargetGroups + nb - 1) / nb));
 }
 
-Launch default_launch(bool adjoint, const Geo& g, int batch) {
+Launch default_launch(bool adjoint, const Geo& g, int batch, bool ws = false) {
   Launch L;
   if (adjoint) {
     L.bx = 32;
     L.by = 16;
     L.bz = 16;
-    if (brick_count(g, 32, 16, 16, true) * batch < 512) {
+    const long long n32 = brick_count(g, 32, 16, 16, true) * batch;
+    const long long n16 = brick_count(g, 16, 16, 16, true) * batch;
+    if (ws ? n32 < 128 : n32 < 512) {
       L.bx = 16;
       L.by = 16;
-      L.bz = brick_count(g, 16, 16, 16, true) * batch >= 200 ? 16 : 8;
+      L.bz = (ws ? n16 >= 32 : n16 >= 200) ? 16 : 8;
     }
-    L.parts = 2;
+    L.parts = default_parts(true, g, batch, L.bx, L.by, L.bz, ws);
     return L;
   }
   if (brick_count(g, L.bx, L.by, L.bz, false) * batch * kFwdMaxParts < 256) {
@@ -537,8 +558,9 @@ Launch default_launch(bool adjoint, const Geo& g, int batch) {
   return L;
 }
 
-int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint, const Geo& g, int batch) {
-  out = default_launch(adjoint, g, batch);
+int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint, const Geo& g, int batch,
+                   bool ws = false) {
+  out = default_launch(adjoint, g, batch, ws);
   if (!l) return DTV_OK;
   if (l->algorithm != DTV_ALGO_DEFAULT) {
     if (l->algorithm != DTV_ALGO_RAY && l->algorithm != DTV_ALGO_BRICK)
@@ -561,12 +583,12 @@ int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint, const Geo& g,
     out.block = l->block;
   }
   if (l->view_parts) {
-    if (l->view_parts < 1 || l->view_parts > (adjoint ? 2 : kFwdMaxParts))
-      return DTV_ERR_INVALID_ARGUMENT;
+    const int maxp = adjoint ? (ws ? kAdjMaxParts : kAdjMaxAtomicParts) : kFwdMaxParts;
+    if (l->view_parts < 1 || l->view_parts > maxp) return DTV_ERR_INVALID_ARGUMENT;
     out.parts = l->view_parts;
     out.parts_requested = true;
   } else {
-    out.parts = default_parts(adjoint, g, batch, out.bx, out.by, out.bz);
+    out.parts = default_parts(adjoint, g, batch, out.bx, out.by, out.bz, ws);
   }
   return DTV_OK;
 }
@@ -574,7 +596,7 @@ int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint, const Geo& g,
 int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st,
                 const Launch& L);
 int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st,
-                const Launch& L);
+                const Launch& L, float* ws, size_t ws_bytes);
 
 int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st) {
   return run_forward(vol, sino, batch, g, st, default_launch(false, g, batch));
@@ -597,13 +619,24 @@ int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipS
 }
 
 int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st) {
-  return run_adjoint(sino, vol, batch, g, st, default_launch(true, g, batch));
+  return run_adjoint(sino, vol, batch, g, st, default_launch(true, g, batch), nullptr, 0);
+}
+
+// Bytes of the A^T workspace for `parts` view parts: each part's partial volumes.
+size_t adjoint_ws_bytes(int32_t batch, const Geo& g, int parts) {
+  return parts > 1 ? (size_t)parts * batch * g.nx * g.ny * g.nz * sizeof(float) : 0;
 }
 
 int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st,
-                const Launch& L) {
+                const Launch& L, float* ws, size_t ws_bytes) {
+  if (L.algo == DTV_ALGO_BRICK && ws && L.parts > 1) {
+    // every part stores its partial volumes; a second kernel adds them in order
+    if (ws_bytes < adjoint_ws_bytes(batch, g, L.parts)) return DTV_ERR_INVALID_ARGUMENT;
+    return launch_adj_brick(sino, vol, ws, batch, g, L.bx, L.by, L.bz, L.block, L.parts, st);
+  }
   if (L.algo == DTV_ALGO_BRICK) {
     int parts = L.parts;
+    if (parts > kAdjMaxAtomicParts) return DTV_ERR_INVALID_ARGUMENT;  // needs a workspace
     if (parts > 1) {
       // the parts add into the output with FP32 global atomics
       int rc = check_atomic_target(vol);
@@ -618,7 +651,7 @@ int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipS
         if (rc) return rc;
       }
     }
-    return launch_adj_brick(sino, vol, batch, g, L.bx, L.by, L.bz, L.block, parts, st);
+    return launch_adj_brick(sino, vol, nullptr, batch, g, L.bx, L.by, L.bz, L.block, parts, st);
   }
   int rc = check_atomic_target(vol);
   if (rc) return rc;
@@ -821,7 +854,49 @@ int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,
   Launch L;
   rc = resolve_launch(launch, L, true, g, batch);
   if (rc) return rc;
-  return run_adjoint(sino, volume, batch, g, (hipStream_t)stream, L);
+  return run_adjoint(sino, volume, batch, g, (hipStream_t)stream, L, nullptr, 0);
+}
+
+int dtv_cone_adjoint_ws(const float* sino, float* volume, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const double* views, int32_t n_views, const dtv_scheme* scheme,
+                        const dtv_launch* launch, float* workspace, size_t workspace_bytes,
+                        void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!volume || !sino) return DTV_ERR_NULL_POINTER;
+  Geo g;
+  rc = fill_table_geo(g, volume_size, detector_size, views, n_views, scheme);
+  if (rc) return rc;
+  Launch L;
+  rc = resolve_launch(launch, L, true, g, batch, true);
+  if (rc) return rc;
+  if (L.algo == DTV_ALGO_BRICK && L.parts > 1) {
+    if (!workspace) return DTV_ERR_NULL_POINTER;
+    if (workspace_bytes < adjoint_ws_bytes(batch, g, L.parts)) return DTV_ERR_INVALID_ARGUMENT;
+  }
+  return run_adjoint(sino, volume, batch, g, (hipStream_t)stream, L, workspace, workspace_bytes);
+}
+
+int dtv_cone_adjoint_plan(int32_t batch, const int32_t volume_size[3],
+                          const int32_t detector_size[2], int32_t n_views,
+                          const dtv_launch* launch, dtv_launch* plan, size_t* workspace_bytes) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!plan || !workspace_bytes) return DTV_ERR_NULL_POINTER;
+  Geo g = make_geo(volume_size, detector_size, n_views);
+  Launch L;
+  rc = resolve_launch(launch, L, true, g, batch, true);
+  if (rc) return rc;
+  *plan = dtv_launch{};
+  plan->algorithm = L.algo;
+  plan->brick[0] = L.bx;
+  plan->brick[1] = L.by;
+  plan->brick[2] = L.bz;
+  plan->block = L.block;
+  plan->view_parts = L.algo == DTV_ALGO_BRICK ? L.parts : 1;
+  *workspace_bytes = L.algo == DTV_ALGO_BRICK ? adjoint_ws_bytes(batch, g, L.parts) : 0;
+  return DTV_OK;
 }
 
 static void export_scheme(const Geo& g, dtv_scheme* out) {

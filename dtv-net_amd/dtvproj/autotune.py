@@ -37,9 +37,10 @@ def signature(geom: ConeGeometry) -> str:
 
 
 # view splits tried per shape: A the library's default for the shape (0) and
-# none; A^T one part and two
+# none; A^T the library's workspace split for the shape (0), one part and two
+# parts adding into the output
 FORWARD_PARTS = (0, 1)
-ADJOINT_PARTS = (1, 2)
+ADJOINT_PARTS = (0, 1, 2)
 
 
 def launch_of(c):

@@ -85,18 +85,53 @@ def forward(volume: torch.Tensor, geom: ConeGeometry, launch=None) -> torch.Tens
     return out
 
 
+def adjoint_plan(geom: ConeGeometry, batch: int = 1, launch=None):
+    """The launch :func:`adjoint` runs for ``geom`` (and an optional fixed
+    ``launch``) with the view split it picks, and the workspace bytes it
+    needs: ``(DtvLaunch, bytes)``.  No GPU work."""
+    plan = _capi.DtvLaunch()
+    nbytes = ctypes.c_size_t(0)
+    rc = _capi.lib().dtv_cone_adjoint_plan(
+        int(batch), _capi.int3(geom.volume_size), _capi.int2(geom.detector_size), geom.n_views,
+        _launch_arg(launch), ctypes.byref(plan), ctypes.byref(nbytes))
+    _capi.check(rc, "dtv_cone_adjoint_plan")
+    return plan, int(nbytes.value)
+
+
 def adjoint(sino: torch.Tensor, geom: ConeGeometry, launch=None) -> torch.Tensor:
-    """Matched A^T: exact transpose of :func:`forward`."""
+    """Matched A^T: exact transpose of :func:`forward`.
+
+    Each brick's views are split over several workgroups whose partial
+    volumes go to a workspace from torch's allocator and are added up in a
+    fixed order (``dtv_cone_adjoint_ws``): bitwise reproducible for any
+    split.  A ``launch`` with ``view_parts`` 1 or 2 and no other wish for a
+    workspace runs ``dtv_cone_adjoint_ex`` instead (two parts add into the
+    output with atomics)."""
     _check_f32(sino, "sino")
     B = _batch_of(sino, _sino_numel(geom), "sino")
     out = torch.empty(geom.volume_shape(B), dtype=torch.float32, device=sino.device)
     views = geom.device_views(sino.device)
     sc = _capi.scheme_struct(geom.scheme)
+    lib = _capi.lib()
+    L = launch
+    if isinstance(L, dict) and L.get("view_parts", 0) in (1, 2):
+        ws_bytes = 0
+    else:
+        plan, ws_bytes = adjoint_plan(geom, B, L)
+        if ws_bytes:
+            L = plan
     with torch.cuda.device(sino.device):
-        rc = _capi.lib().dtv_cone_adjoint_ex(
-            sino.data_ptr(), out.data_ptr(), B, _capi.int3(geom.volume_size),
-            _capi.int2(geom.detector_size), views.data_ptr(), geom.n_views,
-            ctypes.byref(sc), _launch_arg(launch), _stream(sino.device))
+        if ws_bytes:
+            ws = torch.empty(ws_bytes // 4, dtype=torch.float32, device=sino.device)
+            rc = lib.dtv_cone_adjoint_ws(
+                sino.data_ptr(), out.data_ptr(), B, _capi.int3(geom.volume_size),
+                _capi.int2(geom.detector_size), views.data_ptr(), geom.n_views,
+                ctypes.byref(sc), _launch_arg(L), ws.data_ptr(), ws_bytes, _stream(sino.device))
+        else:
+            rc = lib.dtv_cone_adjoint_ex(
+                sino.data_ptr(), out.data_ptr(), B, _capi.int3(geom.volume_size),
+                _capi.int2(geom.detector_size), views.data_ptr(), geom.n_views,
+                ctypes.byref(sc), _launch_arg(L), _stream(sino.device))
     _capi.check(rc, "dtv_cone_adjoint")
     return out
 

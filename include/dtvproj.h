@@ -144,6 +144,29 @@ int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,
                         const double* views, int32_t n_views, const dtv_scheme* scheme,
                         const dtv_launch* launch, void* stream);
 
+/* Matched A^T with each brick's views split over view_parts workgroups that
+ * store their partial volumes in a caller-provided workspace; a second kernel
+ * adds the parts up in part order, so the result is bitwise reproducible for
+ * any number of parts (up to 16), needs no atomics and no zero fill.  The
+ * workspace holds view_parts x batch x voxels floats (device memory; see
+ * dtv_cone_adjoint_plan).  launch == NULL or view_parts == 0 takes the
+ * library's split for the volume size.  No reference counterpart: it serves
+ * callers that can hand over scratch memory (the Python operators pass a
+ * torch allocation); JITStandardGeometry.backward-shaped callers keep
+ * dtv_*_adjoint. */
+int dtv_cone_adjoint_ws(const float* sino, float* volume, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const double* views, int32_t n_views, const dtv_scheme* scheme,
+                        const dtv_launch* launch, float* workspace, size_t workspace_bytes,
+                        void* stream);
+
+/* The launch dtv_cone_adjoint_ws takes for these sizes and `launch` (NULL =
+ * defaults): kernel, brick, lanes and view parts in *plan, and the workspace
+ * bytes it needs (0 for one part).  No GPU work. */
+int dtv_cone_adjoint_plan(int32_t batch, const int32_t volume_size[3],
+                          const int32_t detector_size[2], int32_t n_views,
+                          const dtv_launch* launch, dtv_launch* plan, size_t* workspace_bytes);
+
 /* ---- Uneven instances in one launch ----------------------------------- */
 
 /* One volume/sinogram pair of a multi-instance call (canonical view table

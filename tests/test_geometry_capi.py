@@ -138,6 +138,14 @@ def test_capi_rejects_bad_arguments_without_gpu():
         L = _capi.launch_struct(algorithm=2, brick=(16, 16, 8), block=256, view_parts=parts)
         assert fn(ctypes.c_void_p(16), ctypes.c_void_p(16), 1, _capi.int3((8, 8, 8)),
                   _capi.int2((8, 8)), vp, 4, ctypes.byref(sc), ctypes.byref(L), None) == 1
+    # the split adjoint: a workspace is needed beyond one part, and big enough
+    L = _capi.launch_struct(algorithm=2, brick=(16, 16, 8), block=256, view_parts=3)
+    args = (ctypes.c_void_p(16), ctypes.c_void_p(16), 1, _capi.int3((8, 8, 8)), _capi.int2((8, 8)),
+            vp, 4, ctypes.byref(sc), ctypes.byref(L))
+    assert lib.dtv_cone_adjoint_ws(*args, None, 0, None) == 2
+    assert lib.dtv_cone_adjoint_ws(*args, ctypes.c_void_p(64), 3 * 512 * 4 - 1, None) == 1
+    L.view_parts = 17
+    assert lib.dtv_cone_adjoint_ws(*args, ctypes.c_void_p(64), 1 << 20, None) == 1
     # TV steps and the row filter: null pointers, aliasing, bad sizes and modes
     p = ctypes.c_void_p(16)
     q = ctypes.c_void_p(32)
@@ -219,3 +227,22 @@ def test_pair_bytes_accounting():
     assert b["forward"] == 32 * 1000 + 4 * g.n_rays
     assert b["adjoint"] == 32 * 1000 + 4 * 32 ** 3
     assert b["pair"] == b["forward"] + b["adjoint"]
+
+
+def test_adjoint_plan_splits_by_volume_size():
+    """The workspace split the library picks (no GPU work): two parts at
+    256^3 / 60 views, none at 512x512x384, many on small volumes (at most 16),
+    and the workspace holds parts x batch x voxels floats."""
+    from dtvproj import configs
+    cases = [(configs.sparse_256(), 1, (32, 16, 16), 2), (configs.large_512(), 1, (32, 16, 16), 1),
+             (configs.fista_64(), 1, (16, 16, 16), 16), (configs.unroll_128(), 4, (32, 16, 16), 4)]
+    for g, batch, brick, parts in cases:
+        plan, nbytes = ops.adjoint_plan(g, batch)
+        X, Y, Z = g.volume_size
+        assert plan.algorithm == 2 and tuple(plan.brick) == brick and plan.view_parts == parts
+        assert nbytes == (parts * batch * X * Y * Z * 4 if parts > 1 else 0)
+    # a fixed launch keeps its shape and split
+    plan, nbytes = ops.adjoint_plan(configs.fista_64(), 1,
+                                    dict(algorithm=2, brick=(8, 8, 8), block=64, view_parts=5))
+    assert tuple(plan.brick) == (8, 8, 8) and plan.block == 64 and plan.view_parts == 5
+    assert nbytes == 5 * 64 ** 3 * 4

@@ -104,13 +104,17 @@ def test_default_launch_splits_small_volumes(cuda):
 
 
 def test_view_parts_out_of_range_rejected(cuda):
-    g, _ = _circ()
+    """At most 16 parts; A^T beyond two parts goes through the workspace."""
+    g, args = _circ()
     x = torch.zeros(g.volume_shape(1), device=cuda)
-    y = torch.zeros(g.sino_shape(1), device=cuda)
+    y = phantom.gaussian(g.sino_shape(1), 51)
     with pytest.raises(RuntimeError):
         ops.forward(x, g, dict(algorithm=2, brick=(16, 16, 8), block=256, view_parts=17))
     with pytest.raises(RuntimeError):
-        ops.adjoint(y, g, dict(algorithm=2, brick=(16, 16, 8), block=256, view_parts=3))
+        ops.adjoint(y.to(cuda), g, dict(algorithm=2, brick=(16, 16, 8), block=256, view_parts=17))
+    three = ops.adjoint(y.to(cuda), g, dict(algorithm=2, brick=(16, 16, 8), block=256,
+                                            view_parts=3))
+    assert rel_l2(three, cone.circ_adjoint(y, *args)) <= TOL
 
 
 @pytest.mark.parametrize("persistent", [0, 29])
@@ -152,3 +156,67 @@ def test_auto_parts_split_the_costly_bricks():
     cost = [g.n_views for g in geoms]
     hi, lo = max(range(len(geoms)), key=cost.__getitem__), min(range(len(geoms)), key=cost.__getitem__)
     assert parts[hi][0] >= parts[lo][0]
+
+
+def _adjoint_ws(y, g, cuda, brick, parts):
+    """A^T through dtv_cone_adjoint_ws with an explicit split."""
+    import ctypes
+    import numpy as np
+    from dtvproj import _capi
+    B = y.shape[0]
+    out = torch.empty(g.volume_shape(B), device=cuda)
+    L = _capi.launch_struct(algorithm=2, brick=brick, block=256, view_parts=parts)
+    nb = parts * B * int(np.prod(g.volume_size)) * 4 if parts > 1 else 0
+    ws = torch.empty(max(nb // 4, 1), device=cuda)
+    sc = _capi.scheme_struct(g.scheme)
+    rc = _capi.lib().dtv_cone_adjoint_ws(
+        y.data_ptr(), out.data_ptr(), B, _capi.int3(g.volume_size), _capi.int2(g.detector_size),
+        g.device_views(cuda).data_ptr(), g.n_views, ctypes.byref(sc), ctypes.byref(L),
+        ws.data_ptr(), nb, ops._stream(cuda))
+    _capi.check(rc, "dtv_cone_adjoint_ws")
+    return out
+
+
+@pytest.mark.parametrize("brick", [(16, 16, 8), (32, 16, 16), (8, 8, 8)])
+def test_adjoint_workspace_split_meets_oracle_and_repeats(cuda, brick):
+    """Parts that store partial volumes in a workspace, added up in part
+    order by a second kernel: 1 to 16 parts (more than the 17 views split
+    evenly) meet the oracle, agree with the unsplit launch and repeat
+    bitwise."""
+    g, args = _circ()
+    y = phantom.gaussian(g.sino_shape(2), 48)
+    ref = cone.circ_adjoint(y, *args)
+    yd = y.to(cuda)
+    one = _adjoint_ws(yd, g, cuda, brick, 1)
+    assert rel_l2(one, ref) <= TOL
+    for p in (2, 3, 7, 16):
+        _case("adjoint_ws_parts", brick=brick, parts=p)
+        out = _adjoint_ws(yd, g, cuda, brick, p)
+        assert rel_l2(out, ref) <= TOL
+        assert rel_l2(out, one) <= 2e-6
+        assert torch.equal(out, _adjoint_ws(yd, g, cuda, brick, p))
+
+
+def test_adjoint_workspace_split_culled_beijing(cuda):
+    vs, ds = (40, 36, 16), (8, 24)
+    P = G.beijing_default_vectors(n_angles=7, planes=21, sub_detector=ds, sid=70.0, sdd=114.0)
+    g = G.beijing(P, vs, ds)
+    y = phantom.gaussian(g.sino_shape(1), 49)
+    aref = cone.bj_adjoint(y, P, vs, ds)
+    for p in (4, 16):
+        _case("adjoint_ws_beijing_culled", parts=p)
+        assert rel_l2(_adjoint_ws(y.to(cuda), g, cuda, (16, 16, 8), p), aref) <= TOL
+
+
+def test_default_adjoint_uses_the_planned_split(cuda):
+    """ops.adjoint without a launch runs the library's plan (here many parts
+    through a workspace) and matches an explicit call of that plan bitwise."""
+    ang = G.uniform_angles(40)
+    vs, ds = (48, 48, 40), (72, 56)
+    g = G.circular(ang, vs, 1.0, ds, 1.0, 150.0, 300.0)
+    plan, nbytes = ops.adjoint_plan(g, 1)
+    assert plan.view_parts > 2 and nbytes > 0
+    y = phantom.gaussian(g.sino_shape(1), 50).to(cuda)
+    _case("default_adjoint_plan", parts=plan.view_parts, brick=tuple(plan.brick))
+    assert torch.equal(ops.adjoint(y, g),
+                       _adjoint_ws(y, g, cuda, tuple(plan.brick), plan.view_parts))
