@@ -220,6 +220,28 @@ __device__ void stage_chunk(const Geo& g, int c0, int n, const double lo[3], con
   }
 }
 
+// ---- fixed-point ray sums ------------------------------------------------------
+//
+// The deterministic forward adds each (ray, brick) partial sum into an int64
+// per ray at the power-of-two scale S with S x bound < 2^62, bound = max|vol|
+// x |output scale| x (samples a ray can have inside the volume's support box:
+// one per unit of length along the ray, so its diagonal plus two).  Every
+// partial and every running sum of a ray stays below that bound in magnitude
+// (trilinear weights of a sample sum to at most 1), so nothing overflows, and
+// integer addition makes the result independent of the order of the adds.
+// Returns 0 for a non-finite max (the FP32 path is taken instead); an all-zero
+// volume gets S = 1.
+__device__ __forceinline__ double fix_ray_scale(unsigned absmax_bits, const Geo& g) {
+  if (absmax_bits >= 0x7f800000u) return 0.0;
+  const double dx = g.nx + 2.0, dy = g.ny + 2.0, dz = g.nz + 2.0;
+  const double len = sqrt(dx * dx + dy * dy + dz * dz) + 2.0;
+  const double bound = (double)__uint_as_float(absmax_bits) * fabs(g.scale) * len;
+  if (!(bound > 0.0)) return 1.0;
+  int e;
+  frexp(bound, &e);  // bound < 2^e
+  return ldexp(1.0, 62 - e);
+}
+
 // ---- ray scheduling ----------------------------------------------------------
 //
 // Per (brick, view) the rectangle of candidate rays is taken in segments of
@@ -451,7 +473,9 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
                                                float* __restrict__ sbase, const Geo& g,
                                                int bx_, int by_, int bz_, int nbx, int nby,
                                                int brick, int part, int nparts, float* tile,
-                                               ChunkStage& st, RaySched& rs) {
+                                               ChunkStage& st, RaySched& rs,
+                                               long long* __restrict__ fix = nullptr,
+                                               double fix_scale = 0.0) {
   const int bx = CBX ? CBX : bx_, by = CBY ? CBY : by_, bz = CBZ ? CBZ : bz_;
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   // bricks tile floor(p) over [-1, N-1]; the tile holds voxels [o, o+b]
@@ -553,25 +577,68 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
             X[1] += D[1];
             X[2] += D[2];
           }
-          unsafeAtomicAdd(srow + (size_t)iv * g.nu + iu, acc * scale);
+          if (fix)  // integer accumulation: bitwise reproducible
+            atomicAdd(reinterpret_cast<unsigned long long*>(
+                          fix + (size_t)a * g.nv * g.nu + (size_t)iv * g.nu + iu),
+                      (unsigned long long)(long long)llrint((double)acc * fix_scale));
+          else
+            unsafeAtomicAdd(srow + (size_t)iv * g.nu + iu, acc * scale);
         }
       }
     }
   }
 }
 
+// fix != nullptr: the per-(ray, brick) sums go into int64 fixed point at the
+// scale fix_ray_scale derives from hdr (max |volume|), unless the volume holds
+// a non-finite value, in which case they go into sino as FP32 atomics.
 template <int CBX, int CBY, int CBZ>
 __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol,
-                                                   float* __restrict__ sino, Geo g, int bx,
-                                                   int by, int bz, int nbx, int nby) {
+                                                   float* __restrict__ sino,
+                                                   long long* __restrict__ fix,
+                                                   const unsigned* __restrict__ hdr, Geo g,
+                                                   int bx, int by, int bz, int nbx, int nby) {
   extern __shared__ float tile[];
   __shared__ ChunkStage st;
   __shared__ RaySched rs;
   if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
   const int b = blockIdx.y;
+  double qs = 0.0;
+  if (fix) {
+    const double S = fix_ray_scale(hdr[0], g);
+    if (S > 0.0) {
+      qs = g.scale * S;
+      fix += (size_t)b * g.n_views * g.nv * g.nu;
+    } else {
+      fix = nullptr;
+    }
+  }
   fwd_brick_body<CBX, CBY, CBZ>(vol + (size_t)b * g.nx * g.ny * g.nz,
                  sino + (size_t)b * g.n_views * g.nv * g.nu, g, bx, by, bz, nbx, nby,
-                 blockIdx.x, blockIdx.z, gridDim.z, tile, st, rs);
+                 blockIdx.x, blockIdx.z, gridDim.z, tile, st, rs, fix, qs);
+}
+
+// max |v| over the volume as FP32 bits (non-negative floats order like their
+// bits; NaN sorts above infinity), for the fixed-point scale
+__global__ void k_absmax(const float* __restrict__ v, size_t n, unsigned* __restrict__ hdr) {
+  unsigned m = 0u;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    m = max(m, __float_as_uint(v[i]) & 0x7fffffffu);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(hdr, m);
+}
+
+// sino = fix / S for a finite volume (the FP32 path wrote sino otherwise)
+__global__ void k_fix_to_float(const long long* __restrict__ fix, float* __restrict__ sino,
+                               size_t n, const unsigned* __restrict__ hdr, Geo g) {
+  const double S = fix_ray_scale(hdr[0], g);
+  if (!(S > 0.0)) return;
+  const double inv = 1.0 / S;  // a power of two: exact
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    sino[i] = (float)((double)fix[i] * inv);
 }
 
 constexpr int kAdjPerThread = 32;  // owned voxels per lane (bx*by*bz <= 32*block)
@@ -1032,7 +1099,7 @@ __global__ void k_zero_int(int* p) { *p = 0; }
 namespace dtv {
 
 int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int bx, int by,
-                     int bz, int block, int parts, hipStream_t st) {
+                     int bz, int block, int parts, hipStream_t st, void* fix_ws) {
   const int nbx = (g.nx + 1 + bx - 1) / bx, nby = (g.ny + 1 + by - 1) / by,
             nbz = (g.nz + 1 + bz - 1) / bz;
   const size_t lds = fwd_tile_bytes(bx, by, bz);
@@ -1041,8 +1108,23 @@ int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int
   if (bx == a && by == b && bz == c) kern = &k_fwd_brick<a, b, c>;
   DTV_BRICK_SHAPES(DTV_PICK)
 #undef DTV_PICK
+  // fix_ws: 256-byte header (max |vol| bits), then one int64 per ray; the
+  // caller zero-filled both and the sinogram
+  unsigned* hdr = static_cast<unsigned*>(fix_ws);
+  long long* fix = fix_ws ? reinterpret_cast<long long*>(static_cast<char*>(fix_ws) + 256)
+                          : nullptr;
+  if (fix_ws) {
+    const size_t nv = (size_t)batch * g.nx * g.ny * g.nz;
+    const unsigned blocks = (unsigned)std::min<size_t>((nv + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_absmax, dim3(blocks), dim3(256), 0, st, vol, nv, hdr);
+  }
   hipLaunchKernelGGL(kern, dim3(nbx * nby * nbz, batch, parts), dim3(block), lds, st, vol, sino,
-                     g, bx, by, bz, nbx, nby);
+                     fix, hdr, g, bx, by, bz, nbx, nby);
+  if (fix_ws) {
+    const size_t ns = (size_t)batch * g.n_views * g.nu * g.nv;
+    const unsigned blocks = (unsigned)std::min<size_t>((ns + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_fix_to_float, dim3(blocks), dim3(256), 0, st, fix, sino, ns, hdr, g);
+  }
   return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 

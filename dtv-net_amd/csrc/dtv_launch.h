@@ -57,8 +57,10 @@ int check_atomic_target(const void* p);
 // two parts the sum does not depend on their order, since FP32 addition is
 // commutative) or store their partial volumes into ws (parts x batch x voxels
 // floats), which a second kernel adds up in part order into vol.
+// fix_ws (optional, zero-filled by the caller, 256 + 8 x rays bytes): add the
+// ray sums in int64 fixed point and convert at the end (bitwise reproducible).
 int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int bx, int by,
-                     int bz, int block, int parts, hipStream_t st);
+                     int bz, int block, int parts, hipStream_t st, void* fix_ws = nullptr);
 int launch_adj_brick(const float* sino, float* vol, float* ws, int batch, const Geo& g, int bx,
                      int by, int bz, int block, int parts, hipStream_t st);
 

@@ -594,7 +594,7 @@ int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint, const Geo& g,
 }
 
 int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st,
-                const Launch& L);
+                const Launch& L, void* fix_ws = nullptr);
 int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st,
                 const Launch& L, float* ws, size_t ws_bytes);
 
@@ -602,14 +602,26 @@ int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipS
   return run_forward(vol, sino, batch, g, st, default_launch(false, g, batch));
 }
 
+// Bytes of the deterministic forward's workspace: a 256-byte header and one
+// int64 per ray.
+size_t forward_ws_bytes(int32_t batch, const Geo& g) {
+  return 256 + (size_t)batch * g.n_views * g.nu * g.nv * sizeof(long long);
+}
+
 int run_forward(const float* vol, float* sino, int32_t batch, const Geo& g, hipStream_t st,
-                const Launch& L) {
+                const Launch& L, void* fix_ws) {
   if (L.algo == DTV_ALGO_BRICK) {
     int rc = check_atomic_target(sino);
     if (rc) return rc;
+    if (fix_ws) {
+      rc = check_atomic_target(fix_ws);
+      if (rc) return rc;
+      rc = run_zero(static_cast<float*>(fix_ws), forward_ws_bytes(batch, g) / sizeof(float), st);
+      if (rc) return rc;
+    }
     rc = run_zero(sino, (size_t)batch * g.n_views * g.nu * g.nv, st);
     if (rc) return rc;
-    return launch_fwd_brick(vol, sino, batch, g, L.bx, L.by, L.bz, L.block, L.parts, st);
+    return launch_fwd_brick(vol, sino, batch, g, L.bx, L.by, L.bz, L.block, L.parts, st, fix_ws);
   }
   const dim3 block(kRayTileU, kRayTileV);
   const dim3 grid((g.nu + kRayTileU - 1) / kRayTileU, (g.nv + kRayTileV - 1) / kRayTileV,
@@ -839,6 +851,49 @@ int dtv_cone_forward_ex(const float* volume, float* sino, int32_t batch,
   rc = resolve_launch(launch, L, false, g, batch);
   if (rc) return rc;
   return run_forward(volume, sino, batch, g, (hipStream_t)stream, L);
+}
+
+int dtv_cone_forward_ws(const float* volume, float* sino, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const double* views, int32_t n_views, const dtv_scheme* scheme,
+                        const dtv_launch* launch, void* workspace, size_t workspace_bytes,
+                        void* stream) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!volume || !sino) return DTV_ERR_NULL_POINTER;
+  Geo g;
+  rc = fill_table_geo(g, volume_size, detector_size, views, n_views, scheme);
+  if (rc) return rc;
+  Launch L;
+  rc = resolve_launch(launch, L, false, g, batch);
+  if (rc) return rc;
+  if (L.algo != DTV_ALGO_BRICK)  // the ray-driven gather is deterministic already
+    return run_forward(volume, sino, batch, g, (hipStream_t)stream, L);
+  if (!workspace) return DTV_ERR_NULL_POINTER;
+  if (workspace_bytes < forward_ws_bytes(batch, g) || (uintptr_t)workspace % 256)
+    return DTV_ERR_INVALID_ARGUMENT;
+  return run_forward(volume, sino, batch, g, (hipStream_t)stream, L, workspace);
+}
+
+int dtv_cone_forward_plan(int32_t batch, const int32_t volume_size[3],
+                          const int32_t detector_size[2], int32_t n_views,
+                          const dtv_launch* launch, dtv_launch* plan, size_t* workspace_bytes) {
+  int rc = check_sizes(batch, volume_size, detector_size, n_views);
+  if (rc) return rc;
+  if (!plan || !workspace_bytes) return DTV_ERR_NULL_POINTER;
+  Geo g = make_geo(volume_size, detector_size, n_views);
+  Launch L;
+  rc = resolve_launch(launch, L, false, g, batch);
+  if (rc) return rc;
+  *plan = dtv_launch{};
+  plan->algorithm = L.algo;
+  plan->brick[0] = L.bx;
+  plan->brick[1] = L.by;
+  plan->brick[2] = L.bz;
+  plan->block = L.block;
+  plan->view_parts = L.algo == DTV_ALGO_BRICK ? L.parts : 1;
+  *workspace_bytes = L.algo == DTV_ALGO_BRICK ? forward_ws_bytes(batch, g) : 0;
+  return DTV_OK;
 }
 
 int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,

@@ -87,6 +87,11 @@ _SIGS = {
                           ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                           c_int32, c_int32, c_void_p],
     "dtv_tv_fista_step_multi": [c_void_p, c_void_p, c_int32, c_int32, c_void_p],
+    "dtv_cone_forward_ws": [c_void_p, c_void_p, c_int32, c_int32_p, c_int32_p, c_void_p, c_int32,
+                            ctypes.POINTER(DtvScheme), c_void_p, c_void_p, ctypes.c_size_t,
+                            c_void_p],
+    "dtv_cone_forward_plan": [c_int32, c_int32_p, c_int32_p, c_int32, c_void_p,
+                              ctypes.POINTER(DtvLaunch), ctypes.POINTER(ctypes.c_size_t)],
     "dtv_cone_adjoint_ws": [c_void_p, c_void_p, c_int32, c_int32_p, c_int32_p, c_void_p, c_int32,
                             ctypes.POINTER(DtvScheme), c_void_p, c_void_p, ctypes.c_size_t,
                             c_void_p],

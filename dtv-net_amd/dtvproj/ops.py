@@ -66,21 +66,52 @@ def _launch_arg(launch):
     raise TypeError("launch must be None, a dict or a DtvLaunch")
 
 
-def forward(volume: torch.Tensor, geom: ConeGeometry, launch=None) -> torch.Tensor:
+def forward_plan(geom: ConeGeometry, batch: int = 1, launch=None):
+    """The launch the deterministic :func:`forward` runs for ``geom`` and the
+    workspace bytes it needs: ``(DtvLaunch, bytes)``.  No GPU work."""
+    plan = _capi.DtvLaunch()
+    nbytes = ctypes.c_size_t(0)
+    rc = _capi.lib().dtv_cone_forward_plan(
+        int(batch), _capi.int3(geom.volume_size), _capi.int2(geom.detector_size), geom.n_views,
+        _launch_arg(launch), ctypes.byref(plan), ctypes.byref(nbytes))
+    _capi.check(rc, "dtv_cone_forward_plan")
+    return plan, int(nbytes.value)
+
+
+def forward(volume: torch.Tensor, geom: ConeGeometry, launch=None,
+            deterministic=None) -> torch.Tensor:
     """A: ray sums of ``volume`` [B,1,Z,Y,X] -> sinogram [B,1,A,V,U].
 
     ``launch`` optionally fixes the kernel (``dict(algorithm=, brick=,
-    block=)``); every choice gives the same result to FP32 rounding."""
+    block=, view_parts=)``); every choice gives the same result to FP32
+    rounding.  The brick kernels add per-brick partial sums with FP32
+    atomics, so the last bits vary from run to run; ``deterministic=True``
+    (the default when ``torch.use_deterministic_algorithms(True)`` is in
+    force) adds them in int64 fixed point through a workspace from torch's
+    allocator instead (``dtv_cone_forward_ws``): bitwise reproducible, 2-8%
+    slower on large volumes."""
     _check_f32(volume, "volume")
     B = _batch_of(volume, _vol_numel(geom), "volume")
     out = torch.empty(geom.sino_shape(B), dtype=torch.float32, device=volume.device)
     views = geom.device_views(volume.device)
     sc = _capi.scheme_struct(geom.scheme)
+    if deterministic is None:
+        deterministic = torch.are_deterministic_algorithms_enabled()
+    lib = _capi.lib()
     with torch.cuda.device(volume.device):
-        rc = _capi.lib().dtv_cone_forward_ex(
-            volume.data_ptr(), out.data_ptr(), B, _capi.int3(geom.volume_size),
-            _capi.int2(geom.detector_size), views.data_ptr(), geom.n_views,
-            ctypes.byref(sc), _launch_arg(launch), _stream(volume.device))
+        if deterministic:
+            plan, nbytes = forward_plan(geom, B, launch)
+            ws = torch.empty(max(nbytes // 4, 1), dtype=torch.float32, device=volume.device)
+            rc = lib.dtv_cone_forward_ws(
+                volume.data_ptr(), out.data_ptr(), B, _capi.int3(geom.volume_size),
+                _capi.int2(geom.detector_size), views.data_ptr(), geom.n_views,
+                ctypes.byref(sc), _launch_arg(plan), ws.data_ptr(), nbytes,
+                _stream(volume.device))
+        else:
+            rc = lib.dtv_cone_forward_ex(
+                volume.data_ptr(), out.data_ptr(), B, _capi.int3(geom.volume_size),
+                _capi.int2(geom.detector_size), views.data_ptr(), geom.n_views,
+                ctypes.byref(sc), _launch_arg(launch), _stream(volume.device))
     _capi.check(rc, "dtv_cone_forward")
     return out
 

@@ -139,6 +139,28 @@ int dtv_cone_forward_ex(const float* volume, float* sino, int32_t batch,
                         const int32_t volume_size[3], const int32_t detector_size[2],
                         const double* views, int32_t n_views, const dtv_scheme* scheme,
                         const dtv_launch* launch, void* stream);
+
+/* Deterministic A: as dtv_cone_forward_ex, but each (ray, brick) partial sum
+ * is added into an int64 fixed-point accumulator per ray (at a power-of-two
+ * scale from the volume's largest |value|, so no sum can overflow) and
+ * converted to FP32 at the end, so the result is bitwise reproducible for
+ * any launch.  The workspace (device memory, 256-byte aligned) holds a
+ * 256-byte header and 8 bytes per ray: dtv_cone_forward_plan gives the size.
+ * A volume holding inf or NaN takes the FP32-atomic path (values propagate,
+ * the last bits may vary).  No reference counterpart: the reference's
+ * forward (forwardKernel.cu:115-141) launches per view and is deterministic
+ * by construction; this restores that property for the brick kernels. */
+int dtv_cone_forward_ws(const float* volume, float* sino, int32_t batch,
+                        const int32_t volume_size[3], const int32_t detector_size[2],
+                        const double* views, int32_t n_views, const dtv_scheme* scheme,
+                        const dtv_launch* launch, void* workspace, size_t workspace_bytes,
+                        void* stream);
+
+/* The launch dtv_cone_forward_ws takes (NULL launch = defaults) and its
+ * workspace bytes (0 for the ray-driven kernel).  No GPU work. */
+int dtv_cone_forward_plan(int32_t batch, const int32_t volume_size[3],
+                          const int32_t detector_size[2], int32_t n_views,
+                          const dtv_launch* launch, dtv_launch* plan, size_t* workspace_bytes);
 int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,
                         const int32_t volume_size[3], const int32_t detector_size[2],
                         const double* views, int32_t n_views, const dtv_scheme* scheme,

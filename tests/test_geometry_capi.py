@@ -146,6 +146,14 @@ def test_capi_rejects_bad_arguments_without_gpu():
     assert lib.dtv_cone_adjoint_ws(*args, ctypes.c_void_p(64), 3 * 512 * 4 - 1, None) == 1
     L.view_parts = 17
     assert lib.dtv_cone_adjoint_ws(*args, ctypes.c_void_p(64), 1 << 20, None) == 1
+    # the deterministic forward: a workspace of 256 + 8 bytes per ray, 256-byte aligned
+    L = _capi.launch_struct(algorithm=2, brick=(16, 16, 8), block=256)
+    fargs = (ctypes.c_void_p(16), ctypes.c_void_p(16), 1, _capi.int3((8, 8, 8)), _capi.int2((8, 8)),
+             vp, 4, ctypes.byref(sc), ctypes.byref(L))
+    need = 256 + 8 * 4 * 8 * 8
+    assert lib.dtv_cone_forward_ws(*fargs, None, 0, None) == 2
+    assert lib.dtv_cone_forward_ws(*fargs, ctypes.c_void_p(256), need - 1, None) == 1
+    assert lib.dtv_cone_forward_ws(*fargs, ctypes.c_void_p(128), need, None) == 1
     # TV steps and the row filter: null pointers, aliasing, bad sizes and modes
     p = ctypes.c_void_p(16)
     q = ctypes.c_void_p(32)
@@ -246,3 +254,15 @@ def test_adjoint_plan_splits_by_volume_size():
                                     dict(algorithm=2, brick=(8, 8, 8), block=64, view_parts=5))
     assert tuple(plan.brick) == (8, 8, 8) and plan.block == 64 and plan.view_parts == 5
     assert nbytes == 5 * 64 ** 3 * 4
+
+
+def test_forward_plan_workspace():
+    """The deterministic forward's plan: the default brick launch and 256 +
+    8 bytes per ray of workspace; none for the ray-driven kernel."""
+    from dtvproj import configs
+    g = configs.sparse_256()
+    plan, nbytes = ops.forward_plan(g, 2)
+    assert plan.algorithm == 2 and tuple(plan.brick) == (26, 26, 14) and plan.view_parts >= 1
+    assert nbytes == 256 + 8 * 2 * 60 * 384 * 384
+    plan, nbytes = ops.forward_plan(g, 1, dict(algorithm=1))
+    assert plan.algorithm == 1 and nbytes == 0

@@ -220,3 +220,44 @@ def test_default_adjoint_uses_the_planned_split(cuda):
     _case("default_adjoint_plan", parts=plan.view_parts, brick=tuple(plan.brick))
     assert torch.equal(ops.adjoint(y, g),
                        _adjoint_ws(y, g, cuda, tuple(plan.brick), plan.view_parts))
+
+
+@pytest.mark.parametrize("launch", [None, dict(algorithm=2, brick=(16, 16, 8), block=256,
+                                               view_parts=5),
+                                    dict(algorithm=2, brick=(26, 26, 14), block=256)])
+def test_deterministic_forward_repeats_bitwise_and_meets_oracle(cuda, launch):
+    """The int64 fixed-point forward (dtv_cone_forward_ws): bitwise equal on
+    repeated calls, within FP32 rounding of the atomic version, and at the
+    oracle bar; a batch of two."""
+    g, args = _circ()
+    x = phantom.gaussian(g.volume_shape(2), 52)
+    xd = x.to(cuda)
+    _case("deterministic_forward", launch=launch)
+    a = ops.forward(xd, g, launch, deterministic=True)
+    b = ops.forward(xd, g, launch, deterministic=True)
+    assert torch.equal(a, b)
+    assert rel_l2(a, cone.circ_forward(x, *args)) <= TOL
+    assert rel_l2(a, ops.forward(xd, g, launch, deterministic=False)) <= 1e-6
+
+
+def test_deterministic_forward_follows_torch_flag_and_keeps_nonfinite(cuda):
+    """torch.use_deterministic_algorithms(True) selects the fixed-point path;
+    a volume holding inf or NaN takes the FP32 path, so they reach exactly the
+    rays they reach on it."""
+    g, args = _circ()
+    x = phantom.gaussian(g.volume_shape(1), 53).to(cuda)
+    prev = torch.are_deterministic_algorithms_enabled()
+    try:
+        torch.use_deterministic_algorithms(True)
+        a, b = ops.forward(x, g), ops.forward(x, g)
+    finally:
+        torch.use_deterministic_algorithms(prev)
+    assert torch.equal(a, b)
+    assert torch.equal(a, ops.forward(x, g, deterministic=True))
+    xi = x.clone()
+    xi[0, 0, 9, 13, 15] = float("inf")
+    xi[0, 0, 3, 4, 5] = float("nan")
+    fi = ops.forward(xi, g, deterministic=True)
+    ref = ops.forward(xi, g, dict(algorithm=1))   # ray-driven gather
+    assert torch.equal(torch.isfinite(fi), torch.isfinite(ref))
+    assert torch.equal(torch.isnan(fi), torch.isnan(ref))
