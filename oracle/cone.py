@@ -394,6 +394,19 @@ def circ_adjoint(sino, angles, volume_size, voxel_mm, detector_size, pitch_mm, s
     return out.reshape(sino.shape[0], 1, Z, Y, X)
 
 
+def circ_forward_at(vol, angles, volume_size, voxel_mm, detector_size, pitch_mm, sod, sdd,
+                    rays_idx, dtype=F64):
+    """``circ_forward`` evaluated at the rays ``rays_idx`` (flat indices in
+    sinogram order [view, v, u]) only; returns [B, K].  The same ray
+    construction and sample enumeration, restricted to the chosen rays."""
+    X, Y, Z = volume_size
+    rays = circ_rays(angles, volume_size, voxel_mm, detector_size, pitch_mm, sod, sdd, dtype)
+    idx = torch.as_tensor(np.asarray(rays_idx), dtype=torch.int64).view(-1)
+    sub = Rays(rays.src[idx], rays.d[idx], rays.a_start[idx], rays.a_end[idx], rays.step,
+               rays.div_s, rays.off, rays.scale)
+    return forward(vol.to(dtype), sub, (X, Y, Z))
+
+
 def circ_adjoint_at(sino, angles, volume_size, voxel_mm, detector_size, pitch_mm, sod, sdd,
                     voxels, dtype=F64):
     """``circ_adjoint`` evaluated at the voxels [K, 3] of (ix, iy, iz) only;

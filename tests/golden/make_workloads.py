@@ -19,8 +19,21 @@ full-size workloads, sampled at seeded voxels so the file stays small.
   Stored: the step, the iterate at 32768 seeded voxels, its peak, sum and
   norm.
 
+* ``l512_*`` -- the 512x512x384 workload at 0.4 mm, 120 views, 768x576
+  detector at 0.6 mm, SOD 700 / SDD 1200: A of the i.i.d. N(0,1) volume of
+  seed 20 at 4096 seeded rays (``oracle.cone.circ_forward_at``) and the
+  matched A^T of the N(0,1) sinogram of seed 21 at 4096 seeded voxels (3584
+  uniform, 512 on the faces), with the FP64 sums and sums of squares of both
+  inputs so a test can prove it regenerated them.
+
+* ``u128_*`` -- the 128^3 / 90-view unroll geometry (1 mm voxels, 192x192
+  detector at 1 mm, SOD 500 / SDD 1000): A of the N(0,1) volume of seed 30
+  at 2048 seeded rays and A^T of the N(0,1) sinogram of seed 31 at 2048
+  seeded voxels (1792 uniform, 256 on the faces).
+
 Takes several minutes on 8 cores.  Run from the repository root:
-    python tests/golden/make_workloads.py
+    python tests/golden/make_workloads.py          # everything
+    python tests/golden/make_workloads.py l512 u128   # (re)write only these arrays
 """
 import math
 import os
@@ -47,6 +60,89 @@ U16 = dict(vol=(103, 103, 75), voxel=1.0, det=(153, 91), pitch=2.0, n_views=38, 
 
 def angles(n):
     return np.linspace(0.0, 360.0, n, endpoint=False)
+
+
+L512 = dict(vol=(512, 512, 384), voxel=0.4, det=(768, 576), pitch=0.6, n_views=120,
+            sod=700.0, sdd=1200.0, vol_seed=20, sino_seed=21, n_rays=4096, n_uniform=3584,
+            n_face=512, ray_seed=2023, vox_seed=2024)
+
+
+def face_and_uniform_voxels(vol, n_uniform, n_face, seed):
+    X, Y, Z = vol
+    rng = np.random.default_rng(seed)
+    N = np.array([X, Y, Z])
+    uni = rng.integers(0, N, size=(n_uniform, 3))
+    face = rng.integers(0, N, size=(n_face, 3))
+    ax = rng.integers(0, 3, size=n_face)
+    hi = rng.integers(0, 2, size=n_face)
+    face[np.arange(n_face), ax] = np.where(hi == 1, N[ax] - 1, 0)
+    return np.concatenate([uni, face]).astype(np.int32)
+
+
+def l512_rays():
+    c = L512
+    U, V = c["det"]
+    rng = np.random.default_rng(c["ray_seed"])
+    return np.sort(rng.choice(c["n_views"] * U * V, size=c["n_rays"], replace=False)).astype(np.int64)
+
+
+def l512_voxels():
+    c = L512
+    return face_and_uniform_voxels(c["vol"], c["n_uniform"], c["n_face"], c["vox_seed"])
+
+
+def build_l512(log=print):
+    c = L512
+    X, Y, Z = c["vol"]
+    U, V = c["det"]
+    args = (angles(c["n_views"]), c["vol"], c["voxel"], c["det"], c["pitch"], c["sod"], c["sdd"])
+    x = phantom.gaussian((1, 1, Z, Y, X), c["vol_seed"]).double()
+    rays = l512_rays()
+    fwd = cone.circ_forward_at(x, *args, rays)
+    log(f"l512 forward at {rays.shape[0]} rays")
+    out = dict(l512_rays=rays, l512_forward=fwd[0].numpy(),
+               l512_vol_sum=np.float64(x.sum()), l512_vol_sumsq=np.float64((x * x).sum()))
+    del x
+    y = phantom.gaussian((1, 1, c["n_views"], V, U), c["sino_seed"]).double()
+    vox = l512_voxels()
+    adj = cone.circ_adjoint_at(y, *args, vox)
+    log(f"l512 adjoint at {vox.shape[0]} voxels")
+    out.update(l512_voxels=vox, l512_adjoint=adj[0].numpy(),
+               l512_sino_sum=np.float64(y.sum()), l512_sino_sumsq=np.float64((y * y).sum()))
+    return out
+
+
+U128 = dict(vol=(128, 128, 128), voxel=1.0, det=(192, 192), pitch=1.0, n_views=90, sod=500.0,
+            sdd=1000.0, vol_seed=30, sino_seed=31, n_rays=2048, n_uniform=1792, n_face=256,
+            ray_seed=2025, vox_seed=2026)
+
+
+def u128_rays():
+    c = U128
+    U, V = c["det"]
+    rng = np.random.default_rng(c["ray_seed"])
+    return np.sort(rng.choice(c["n_views"] * U * V, size=c["n_rays"], replace=False)).astype(np.int64)
+
+
+def u128_voxels():
+    c = U128
+    return face_and_uniform_voxels(c["vol"], c["n_uniform"], c["n_face"], c["vox_seed"])
+
+
+def build_u128(log=print):
+    c = U128
+    X, Y, Z = c["vol"]
+    U, V = c["det"]
+    args = (angles(c["n_views"]), c["vol"], c["voxel"], c["det"], c["pitch"], c["sod"], c["sdd"])
+    x = phantom.gaussian((1, 1, Z, Y, X), c["vol_seed"]).double()
+    rays = u128_rays()
+    fwd = cone.circ_forward_at(x, *args, rays)
+    y = phantom.gaussian((1, 1, c["n_views"], V, U), c["sino_seed"]).double()
+    vox = u128_voxels()
+    adj = cone.circ_adjoint_at(y, *args, vox)
+    log(f"u128: {rays.shape[0]} rays, {vox.shape[0]} voxels")
+    return dict(u128_rays=rays, u128_forward=fwd[0].numpy(), u128_vol_sum=np.float64(x.sum()),
+                u128_voxels=vox, u128_adjoint=adj[0].numpy(), u128_sino_sum=np.float64(y.sum()))
 
 
 def s256_voxels():
@@ -121,8 +217,18 @@ def build_u16(log=print):
 
 if __name__ == "__main__":
     torch.set_num_threads(os.cpu_count() or 1)
-    fx = build_s256()
-    print(f"s256: {fx['s256_adjoint'].shape[0]} voxels", flush=True)
-    fx.update(build_u16(lambda s: print(s, flush=True)))
+    log = lambda s: print(s, flush=True)  # noqa: E731
+    parts = dict(l512=build_l512, u128=build_u128)
+    if sys.argv[1:]:
+        with np.load(OUT) as f:
+            fx = {k: f[k] for k in f.files}
+        for name in sys.argv[1:]:
+            fx.update(parts[name](log))
+    else:
+        fx = build_s256()
+        print(f"s256: {fx['s256_adjoint'].shape[0]} voxels", flush=True)
+        fx.update(build_u16(log))
+        fx.update(build_l512(log))
+        fx.update(build_u128(log))
     np.savez_compressed(OUT, **fx)
     print(f"wrote {OUT}: {len(fx)} arrays, {os.path.getsize(OUT)} bytes")

@@ -184,3 +184,32 @@ def test_workload_fixture_uneven16_is_consistent():
     assert abs(step - 1.0 / (lip + c["lam"] * 12.0 / c["eps"])) <= 1e-15 * step
     assert np.isfinite(vals).all() and vals.min() >= 0.0 and vals.max() <= peak
     assert total > 0 and peak > 0
+
+
+@pytest.mark.slow
+def test_workload_fixture_large512_reproduces_on_a_subset():
+    """The oracle still gives the stored 512x512x384 / 120-view A at 32 of the
+    stored rays and A^T at 32 of the stored voxels (inputs regenerated from
+    their seeds, checked by checksum)."""
+    from oracle import cone
+    from dtvproj import phantom
+    m = _wgen()
+    c = m.L512
+    X, Y, Z = c["vol"]
+    U, V = c["det"]
+    args = (m.angles(c["n_views"]), c["vol"], c["voxel"], c["det"], c["pitch"], c["sod"], c["sdd"])
+    with np.load(WFIX) as f:
+        fx = {k: f[k] for k in f.files if k.startswith("l512_")}
+    assert np.array_equal(fx["l512_rays"], m.l512_rays())
+    assert np.array_equal(fx["l512_voxels"], m.l512_voxels())
+    x = phantom.gaussian((1, 1, Z, Y, X), c["vol_seed"]).double()
+    assert float(x.sum()) == float(fx["l512_vol_sum"])
+    pick = np.r_[0:16, len(fx["l512_rays"]) - 16:len(fx["l512_rays"])]
+    got = cone.circ_forward_at(x, *args, fx["l512_rays"][pick])[0].numpy()
+    assert np.abs(got - fx["l512_forward"][pick]).max() <= 1e-12 * np.abs(fx["l512_forward"]).max()
+    del x
+    y = phantom.gaussian((1, 1, c["n_views"], V, U), c["sino_seed"]).double()
+    assert float(y.sum()) == float(fx["l512_sino_sum"])
+    vpick = np.r_[0:16, len(fx["l512_voxels"]) - 16:len(fx["l512_voxels"])]
+    got = cone.circ_adjoint_at(y, *args, fx["l512_voxels"][vpick])[0].numpy()
+    assert np.abs(got - fx["l512_adjoint"][vpick]).max() <= 1e-12 * np.abs(fx["l512_adjoint"]).max()

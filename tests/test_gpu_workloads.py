@@ -4,6 +4,11 @@
 * the matched A^T over all 60 views of the 256^3 sparse-view workload (the
   headline call: two 32-view chunks and many fixed-point view groups per
   brick) at 4096 seeded voxels, rel L2 <= 1e-5;
+* the 512x512x384 / 120-view workload: A at 4096 seeded rays and the matched
+  A^T at 4096 seeded voxels, rel L2 <= 1e-5 (default launches, the
+  deterministic forward and a split A^T);
+* the 4 x 128^3 / 90-view unroll geometry: A at 2048 rays and A^T at 2048
+  voxels of the seeded item of a batch of four;
 * the 16-instance uneven FISTA-TV workload: 20 iterations batched in one
   launch per operator, eager and hipGraph replay, against the plain
   per-volume loop (rel L2 <= 1e-6), and its smallest-work instance against
@@ -52,6 +57,69 @@ def test_sparse256_full_adjoint_vs_oracle_fixture(cuda, wfx, launch):
     assert rel_l2(got, ref) <= 1e-5
     # the 512 face voxels on their own (brick edges and the volume border)
     assert rel_l2(got[-512:], ref[-512:]) <= 1e-5
+
+
+def test_large512_forward_and_adjoint_vs_oracle_fixture(cuda, wfx):
+    g = configs.large_512()
+    X, Y, Z = g.volume_size
+    _case("large_512 A at rays, A^T at voxels vs oracle")
+    x = phantom.gaussian(g.volume_shape(1), 20)
+    x64 = x.double()
+    assert abs(float(x64.sum()) - float(wfx["l512_vol_sum"])) <= 1e-9 * float(wfx["l512_vol_sumsq"])
+    assert abs(float((x64 * x64).sum()) / float(wfx["l512_vol_sumsq"]) - 1.0) <= 1e-12
+    del x64
+    rays = torch.from_numpy(wfx["l512_rays"]).to(cuda)
+    ref = torch.from_numpy(wfx["l512_forward"])
+    xd = x.to(cuda)
+    del x
+    for det in (False, True):
+        sino = ops.forward(xd, g, deterministic=det)
+        assert rel_l2(sino.reshape(-1)[rays], ref) <= 1e-5, f"deterministic={det}"
+        del sino
+    del xd
+    y = phantom.gaussian(g.sino_shape(1), 21)
+    y64 = y.double()
+    assert abs(float(y64.sum()) - float(wfx["l512_sino_sum"])) <= 1e-9 * float(wfx["l512_sino_sumsq"])
+    assert abs(float((y64 * y64).sum()) / float(wfx["l512_sino_sumsq"]) - 1.0) <= 1e-12
+    del y64
+    vox = torch.from_numpy(wfx["l512_voxels"].astype(np.int64))
+    flat = ((vox[:, 2] * Y + vox[:, 1]) * X + vox[:, 0]).to(cuda)
+    aref = torch.from_numpy(wfx["l512_adjoint"])
+    yd = y.to(cuda)
+    for launch in (None, dict(algorithm=2, brick=(32, 16, 16), block=256, view_parts=3)):
+        vol = ops.adjoint(yd, g, launch)
+        got = vol.reshape(-1)[flat]
+        assert rel_l2(got, aref) <= 1e-5, f"launch={launch}"
+        assert rel_l2(got[-512:], aref[-512:]) <= 1e-5
+        del vol
+
+
+def test_unroll128_batch4_vs_oracle_fixture(cuda, wfx):
+    """The 4 x 128^3 / 90-view unroll geometry at full size, the seeded
+    volume (and sinogram) as item 2 of a batch of four: A at 2048 seeded rays
+    and the matched A^T at 2048 seeded voxels."""
+    g = configs.unroll_128()
+    X, Y, Z = g.volume_size
+    _case("unroll_128 batch 4 vs oracle rays / voxels")
+    x = phantom.gaussian(g.volume_shape(1), 30)
+    assert abs(float(x.double().sum()) - float(wfx["u128_vol_sum"])) <= 1e-6
+    xb = torch.cat([phantom.gaussian(g.volume_shape(1), 40 + k) for k in range(2)] + [x] +
+                   [phantom.gaussian(g.volume_shape(1), 43)]).to(cuda)
+    rays = torch.from_numpy(wfx["u128_rays"]).to(cuda)
+    ref = torch.from_numpy(wfx["u128_forward"])
+    for det in (False, True):
+        sino = ops.forward(xb, g, deterministic=det)
+        assert rel_l2(sino[2].reshape(-1)[rays], ref) <= 1e-5, f"deterministic={det}"
+    y = phantom.gaussian(g.sino_shape(1), 31)
+    assert abs(float(y.double().sum()) - float(wfx["u128_sino_sum"])) <= 1e-6
+    yb = torch.cat([phantom.gaussian(g.sino_shape(1), 50 + k) for k in range(2)] + [y] +
+                   [phantom.gaussian(g.sino_shape(1), 53)]).to(cuda)
+    vox = torch.from_numpy(wfx["u128_voxels"].astype(np.int64))
+    flat = ((vox[:, 2] * Y + vox[:, 1]) * X + vox[:, 0]).to(cuda)
+    aref = torch.from_numpy(wfx["u128_adjoint"])
+    for launch in (None, dict(algorithm=2, brick=(16, 16, 16), block=256, view_parts=2)):
+        vol = ops.adjoint(yb, g, launch)
+        assert rel_l2(vol[2].reshape(-1)[flat], aref) <= 1e-5, f"launch={launch}"
 
 
 def test_uneven16_batched_fista_tv(cuda, wfx):

@@ -273,3 +273,22 @@ def test_matched_adjoint_at_selected_voxels(case):
     sub = cone.circ_adjoint_at(y, ang, vs, vx, ds, pitch, sod, sdd, vox)
     ref = full[:, (vox[:, 2] * Y + vox[:, 1]) * X + vox[:, 0]]
     assert float((sub - ref).abs().max()) <= 1e-13 * float(ref.abs().max())
+
+
+@pytest.mark.parametrize("case", [
+    ((8, 8, 8), 1.0, (12, 10), 1.2, 6, 40.0, 80.0),
+    ((9, 7, 5), 1.1, (13, 9), 1.0, 5, 30.0, 120.0),
+])
+def test_forward_at_selected_rays(case):
+    """``circ_forward_at`` (the per-ray evaluation used for the full-size
+    fixtures) equals the full forward at the chosen rays, bitwise (the same
+    samples in the same order per ray)."""
+    vs, vx, ds, pitch, n, sod, sdd = case
+    X, Y, Z = vs
+    U, V = ds
+    ang = G.uniform_angles(n)
+    x = phantom.gaussian((2, 1, Z, Y, X), 6, dtype=torch.float64)
+    full = cone.circ_forward(x, ang, vs, vx, ds, pitch, sod, sdd).reshape(2, -1)
+    idx = np.random.default_rng(3).choice(n * U * V, size=40, replace=False)
+    sub = cone.circ_forward_at(x, ang, vs, vx, ds, pitch, sod, sdd, idx)
+    assert torch.equal(sub, full[:, torch.from_numpy(idx)])
