@@ -9,6 +9,9 @@ full-size workloads, sampled at seeded voxels so the file stays small.
   on the faces of the volume).  The sinogram itself (35 MB) is not stored:
   its FP64 sum and sum of squares are, so a test can prove it regenerated
   the same input.
+* ``s256f_*`` -- A over all 60 views of the same workload applied to the
+  i.i.d. N(0,1) volume of seed 20, at 4096 seeded rays
+  (``oracle.cone.circ_forward_at``), with the volume's FP64 sum.
 * ``u16_*`` -- 20 FISTA-TV iterations (smoothed TV, Neumann differences,
   lam = eps = 1e-3, non-negativity, FISTA momentum of FISTA.py:23-25) on the
   smallest-work instance of the 16-instance uneven workload (103x103x75 at
@@ -33,7 +36,7 @@ full-size workloads, sampled at seeded voxels so the file stays small.
 
 Takes several minutes on 8 cores.  Run from the repository root:
     python tests/golden/make_workloads.py          # everything
-    python tests/golden/make_workloads.py l512 u128   # (re)write only these arrays
+    python tests/golden/make_workloads.py l512 u128 s256f   # (re)write only these arrays
 """
 import math
 import os
@@ -129,6 +132,24 @@ def u128_voxels():
     return face_and_uniform_voxels(c["vol"], c["n_uniform"], c["n_face"], c["vox_seed"])
 
 
+def s256_rays():
+    c = S256
+    U, V = c["det"]
+    rng = np.random.default_rng(2027)
+    return np.sort(rng.choice(c["n_views"] * U * V, size=4096, replace=False)).astype(np.int64)
+
+
+def build_s256f(log=print):
+    c = S256
+    X, Y, Z = c["vol"]
+    args = (angles(c["n_views"]), c["vol"], c["voxel"], c["det"], c["pitch"], c["sod"], c["sdd"])
+    x = phantom.gaussian((1, 1, Z, Y, X), 20).double()
+    rays = s256_rays()
+    fwd = cone.circ_forward_at(x, *args, rays)
+    log(f"s256f: {rays.shape[0]} rays")
+    return dict(s256f_rays=rays, s256f_forward=fwd[0].numpy(), s256f_vol_sum=np.float64(x.sum()))
+
+
 def build_u128(log=print):
     c = U128
     X, Y, Z = c["vol"]
@@ -218,7 +239,7 @@ def build_u16(log=print):
 if __name__ == "__main__":
     torch.set_num_threads(os.cpu_count() or 1)
     log = lambda s: print(s, flush=True)  # noqa: E731
-    parts = dict(l512=build_l512, u128=build_u128)
+    parts = dict(l512=build_l512, u128=build_u128, s256f=build_s256f)
     if sys.argv[1:]:
         with np.load(OUT) as f:
             fx = {k: f[k] for k in f.files}
@@ -230,5 +251,6 @@ if __name__ == "__main__":
         fx.update(build_u16(log))
         fx.update(build_l512(log))
         fx.update(build_u128(log))
+        fx.update(build_s256f(log))
     np.savez_compressed(OUT, **fx)
     print(f"wrote {OUT}: {len(fx)} arrays, {os.path.getsize(OUT)} bytes")

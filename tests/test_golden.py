@@ -213,3 +213,21 @@ def test_workload_fixture_large512_reproduces_on_a_subset():
     vpick = np.r_[0:16, len(fx["l512_voxels"]) - 16:len(fx["l512_voxels"])]
     got = cone.circ_adjoint_at(y, *args, fx["l512_voxels"][vpick])[0].numpy()
     assert np.abs(got - fx["l512_adjoint"][vpick]).max() <= 1e-12 * np.abs(fx["l512_adjoint"]).max()
+
+
+@pytest.mark.slow
+def test_workload_fixture_sparse256_forward_reproduces_on_a_subset():
+    from oracle import cone
+    from dtvproj import phantom
+    m = _wgen()
+    c = m.S256
+    X, Y, Z = c["vol"]
+    with np.load(WFIX) as f:
+        rays, ref, vsum = f["s256f_rays"], f["s256f_forward"], float(f["s256f_vol_sum"])
+    assert np.array_equal(rays, m.s256_rays())
+    x = phantom.gaussian((1, 1, Z, Y, X), 20).double()
+    assert float(x.sum()) == vsum
+    pick = np.r_[0:16, len(rays) - 16:len(rays)]
+    got = cone.circ_forward_at(x, m.angles(c["n_views"]), c["vol"], c["voxel"], c["det"],
+                               c["pitch"], c["sod"], c["sdd"], rays[pick])[0].numpy()
+    assert np.abs(got - ref[pick]).max() <= 1e-12 * np.abs(ref).max()

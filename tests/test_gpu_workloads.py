@@ -4,6 +4,7 @@
 * the matched A^T over all 60 views of the 256^3 sparse-view workload (the
   headline call: two 32-view chunks and many fixed-point view groups per
   brick) at 4096 seeded voxels, rel L2 <= 1e-5;
+* A over all 60 views of the 256^3 workload at 4096 seeded rays;
 * the 512x512x384 / 120-view workload: A at 4096 seeded rays and the matched
   A^T at 4096 seeded voxels, rel L2 <= 1e-5 (default launches, the
   deterministic forward and a split A^T);
@@ -57,6 +58,24 @@ def test_sparse256_full_adjoint_vs_oracle_fixture(cuda, wfx, launch):
     assert rel_l2(got, ref) <= 1e-5
     # the 512 face voxels on their own (brick edges and the volume border)
     assert rel_l2(got[-512:], ref[-512:]) <= 1e-5
+
+
+@pytest.mark.parametrize("launch", [None, dict(algorithm=2, brick=(24, 24, 14), block=256,
+                                               view_parts=1)])
+def test_sparse256_full_forward_vs_oracle_fixture(cuda, wfx, launch):
+    """The headline A over all 60 views (default: 26x26x14 bricks, views split
+    over three workgroups per brick) at 4096 seeded rays, FP32 atomics and the
+    deterministic fixed-point sums."""
+    g = configs.sparse_256()
+    _case("sparse_256 full A vs oracle rays", launch=launch)
+    x = phantom.gaussian(g.volume_shape(1), 20)
+    assert abs(float(x.double().sum()) - float(wfx["s256f_vol_sum"])) <= 1e-6
+    rays = torch.from_numpy(wfx["s256f_rays"]).to(cuda)
+    ref = torch.from_numpy(wfx["s256f_forward"])
+    xd = x.to(cuda)
+    for det in (False, True):
+        sino = ops.forward(xd, g, launch, deterministic=det)
+        assert rel_l2(sino.reshape(-1)[rays], ref) <= 1e-5, f"deterministic={det}"
 
 
 def test_large512_forward_and_adjoint_vs_oracle_fixture(cuda, wfx):
