@@ -382,11 +382,14 @@ def roofline_record(dom, dom_t, dom_b, acct, t_fwd, t_adj, med_fwd, med_adj, log
     in-support sample: the 8 trilinear taps), which the LDS-tiled kernels
     serve from LDS, so it is not a bound and exceeds the HBM peak."""
     from dtvproj import roofline
-    key = "k_adj_brick" if dom.startswith("k_adj") else "k_fwd_brick"
+    # the headline A^T runs the partitioned kernel (k_adj_part); older
+    # summaries hold the unpartitioned one (k_adj_brick)
+    keys = ("k_adj_part", "k_adj_brick") if dom.startswith("k_adj") else ("k_fwd_brick",)
     rec = {}
     if os.path.exists(PMC_FILE):
         with open(PMC_FILE) as f:
-            rec = json.load(f).get(key, {})
+            pmc = json.load(f)
+        rec = next((pmc[k] for k in keys if k in pmc), {})
     traffic = rec.get("traffic_bytes")
     valu = rec.get("sq_insts_valu")
     conflict = rec.get("sq_lds_bank_conflict")

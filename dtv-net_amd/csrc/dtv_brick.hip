@@ -440,7 +440,8 @@ __device__ __forceinline__ bool ray_run(const Geo& g, const View& v, int iu, int
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     // integers below 2^53 throughout, so exact; the results are tile-relative
-    // (0 <= X < 2^38 inside the exact window) and |D| <= 2^32
+    // (-2^32 <= X < 2^38 inside the exact window: negative only for the
+    // floor -1 samples of partitioned bricks) and |D| <= 2^32
     X[c] = integral_to_i64(Pd[c] + (double)ka * Dd[c] - org[c]);
     D[c] = integral_to_i64(Dd[c]);
   }
@@ -662,16 +663,33 @@ __device__ __forceinline__ float one_plus_frac(int64_t x) {
 // relative 32.32 position X, step D) into the LDS tile: int32 fixed point
 // (kFixed, taps pre-scaled) or FP32.  Each split w -> (w - f w, f w) takes
 // f w = fma(1 + f, w, -w), rounded once.
-template <bool kFixed>
+// kGuard (partitioned tiles of bricks on the volume's y = 0 or z = 0 face,
+// which have no row or plane below the brick): a sample with floor -1 on y or
+// z drops its taps on that outside row or plane and adds the others at
+// index 0 (the upper weight moves to the lower slot, the upper slot adds 0).
+template <bool kFixed, bool kGuard = false>
 __device__ __forceinline__ void adj_march(int ka, int kb, int64_t X[3], const int64_t D[3],
                                           float val, int stz, int sty, int* iacc) {
   float* facc = reinterpret_cast<float*>(iacc);
   for (int k = ka; k < kb; ++k) {
-    const int ix = (int)(X[0] >> 32), iy = (int)(X[1] >> 32), iz = (int)(X[2] >> 32);
+    const int ix = (int)(X[0] >> 32);
+    int iy = (int)(X[1] >> 32), iz = (int)(X[2] >> 32);
     const float gx = one_plus_frac(X[0]), gy = one_plus_frac(X[1]), gz = one_plus_frac(X[2]);
-    const float wz1 = fmaf(gz, val, -val), wz0 = val - wz1;
-    const float w10 = fmaf(gy, wz0, -wz0), w00 = wz0 - w10;
-    const float w11 = fmaf(gy, wz1, -wz1), w01 = wz1 - w11;
+    float wz1 = fmaf(gz, val, -val), wz0 = val - wz1;
+    if (kGuard && iz < 0) {
+      wz0 = wz1;
+      wz1 = 0.f;
+      iz = 0;
+    }
+    float w10 = fmaf(gy, wz0, -wz0), w00 = wz0 - w10;
+    float w11 = fmaf(gy, wz1, -wz1), w01 = wz1 - w11;
+    if (kGuard && iy < 0) {
+      w00 = w10;
+      w10 = 0.f;
+      w01 = w11;
+      w11 = 0.f;
+      iy = 0;
+    }
     const float t001 = fmaf(gx, w00, -w00), t000 = w00 - t001;
     const float t011 = fmaf(gx, w10, -w10), t010 = w10 - t011;
     const float t101 = fmaf(gx, w01, -w01), t100 = w01 - t101;
@@ -716,7 +734,8 @@ __host__ __device__ constexpr bool fast_flush_shape(int bx, int by, int bz) {
 // registers and clears them (fixed order per voxel: bitwise reproducible).
 // The lane index is made opaque so the addresses are computed here rather
 // than hoisted out of the view loops and held (or spilled) throughout.
-template <int CBX, int CBY, int CBZ>
+// kPart: partitioned tile (no column, row or plane below the brick).
+template <int CBX, int CBY, int CBZ, bool kPart>
 __device__ __forceinline__ void flush_owned(int* iacc, float (&vacc)[kAdjPerThread], bool fixed,
                                             float qinv, int bx, int by, int nown, int stz,
                                             int sty) {
@@ -727,8 +746,10 @@ __device__ __forceinline__ void flush_owned(int* iacc, float (&vacc)[kAdjPerThre
   if (kFast && blockDim.x == 256) {
     // voxel j of lane tid: row q + R (j % P) of plane j / P, q = tid / bx
     constexpr int R = kFast ? 256 / CBX : 1, P = kFast ? CBY / R : 1;
-    constexpr int SY = adj_tile_sty(CBX), SZ = adj_tile_stz(CBX, CBY);
-    const int base = SZ + (tid / (kFast ? CBX : 1) + 1) * SY + tid % (kFast ? CBX : 1) + 1;
+    constexpr int SY = kPart ? adj_ptile_sty(CBX) : adj_tile_sty(CBX),
+                  SZ = kPart ? adj_ptile_stz(CBX, CBY) : adj_tile_stz(CBX, CBY);
+    constexpr int O = kPart ? 0 : SZ + SY + 1;  // tile index of the brick's first voxel
+    const int base = O + (tid / (kFast ? CBX : 1)) * SY + tid % (kFast ? CBX : 1);
     if (fixed) {
 #pragma unroll
       for (int j = 0; j < kAdjPerThread; ++j) {
@@ -746,15 +767,63 @@ __device__ __forceinline__ void flush_owned(int* iacc, float (&vacc)[kAdjPerThre
     }
     return;
   }
+  const int o = kPart ? 0 : stz + sty + 1;
 #pragma unroll
   for (int j = 0; j < kAdjPerThread; ++j) {
     const int i = tid + j * blockDim.x;
     if (i < nown) {
       const int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
-      const int li = (lz + 1) * stz + (ly + 1) * sty + lx + 1;
+      const int li = lz * stz + ly * sty + lx + o;
       vacc[j] += fixed ? (float)iacc[li] * qinv : facc[li];
       iacc[li] = 0;
     }
+  }
+}
+
+// Upper-halo voxel h of a brick (0 <= h < adj_halo_count) in local
+// coordinates: first the plane lz == bz (x fastest), then the rows ly == by of
+// the planes below it, then the column lx == bx of the remaining rows.
+__host__ __device__ inline void halo_voxel(int h, int bx, int by, int bz, int& lx, int& ly,
+                                           int& lz) {
+  const int A = (bx + 1) * (by + 1), B = (bx + 1) * bz;
+  if (h < A) {
+    lz = bz;
+    ly = h / (bx + 1);
+    lx = h - ly * (bx + 1);
+  } else if (h < A + B) {
+    const int r = h - A;
+    lz = r / (bx + 1);
+    ly = by;
+    lx = r - lz * (bx + 1);
+  } else {
+    const int r = h - A - B;
+    lz = r / by;
+    ly = r - lz * by;
+    lx = bx;
+  }
+}
+
+// Inverse of halo_voxel for a local voxel with lx == bx, ly == by or lz == bz.
+__host__ __device__ inline int halo_slot(int lx, int ly, int lz, int bx, int by, int bz) {
+  const int A = (bx + 1) * (by + 1), B = (bx + 1) * bz;
+  if (lz == bz) return ly * (bx + 1) + lx;
+  if (ly == by) return A + lz * (bx + 1) + lx;
+  return A + B + lz * by + ly;
+}
+
+// Partitioned tiles: adds the group's upper-halo sums into the FP32
+// accumulators hacc (slot h belongs to lane h % blockDim.x throughout, so the
+// per-voxel order is fixed) and clears them.
+__device__ __forceinline__ void flush_halo(int* iacc, float* hacc, bool fixed, float qinv,
+                                           int bx, int by, int bz, int stz, int sty) {
+  const float* facc = reinterpret_cast<const float*>(iacc);
+  const int nh = adj_halo_count(bx, by, bz);
+  for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+    int lx, ly, lz;
+    halo_voxel(h, bx, by, bz, lx, ly, lz);
+    const int li = lz * stz + ly * sty + lx;
+    hacc[h] += fixed ? (float)iacc[li] * qinv : facc[li];
+    iacc[li] = 0;
   }
 }
 
@@ -778,31 +847,53 @@ __device__ __forceinline__ void flush_owned(int* iacc, float (&vacc)[kAdjPerThre
 // view seeing the box from right next to the source (loose bound),
 // accumulates in FP32 LDS atomics in the same buffer instead, so infinities
 // and NaNs propagate and precision holds.
-template <int CBX, int CBY, int CBZ>
+//
+// kPart (partitioned): the brick marches only the samples whose floor cell is
+// its own (floor in [o, o+e-1], or from -1 on the volume's lower faces), so
+// every sample is marched by exactly one brick.  Their taps reach the upper
+// halo (local coordinate b on some axis), whose sums a group flush adds into
+// FP32 accumulators hacc in LDS (after the tile) and which end up in hout for
+// the neighbours; the tile has no row or plane below the brick (taps of
+// floor -1 samples on y or z are dropped by the guarded loop; on x they land
+// in a pad word).
+template <int CBX, int CBY, int CBZ, bool kPart = false>
 __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
                                                float* __restrict__ vb, const Geo& g, int bx_,
                                                int by_, int bz_, int nbx, int nby, int brick,
                                                int part, int nparts, bool add, int* iacc,
-                                               ChunkStage& st, RaySched& rs) {
+                                               ChunkStage& st, RaySched& rs,
+                                               float* __restrict__ hout = nullptr) {
   const int bx = CBX ? CBX : bx_, by = CBY ? CBY : by_, bz = CBZ ? CBZ : bz_;
   const int jx = brick % nbx, jy = (brick / nbx) % nby, jz = brick / (nbx * nby);
   const int x0 = jx * bx, y0 = jy * by, z0 = jz * bz;
   const int ex = min(bx, g.nx - x0), ey = min(by, g.ny - y0), ez = min(bz, g.nz - z0);
-  const int sty = adj_tile_sty(bx), stz = adj_tile_stz(bx, by);
+  const int sty = kPart ? adj_ptile_sty(bx) : adj_tile_sty(bx),
+            stz = kPart ? adj_ptile_stz(bx, by) : adj_tile_stz(bx, by);
+  const int ntile = stz * (kPart ? bz + 1 : bz + 2);
+  const int nhalo = kPart ? adj_halo_count(bx, by, bz) : 0;
+  int* tile = kPart ? iacc + 1 : iacc;  // partitioned: after the leading pad word
+  float* hacc = reinterpret_cast<float*>(tile + ntile);
   const int nown = bx * by * bz;
   __syncthreads();  // the previous item of a persistent workgroup is done with LDS
-  for (int i = threadIdx.x; i < stz * (bz + 2); i += blockDim.x) iacc[i] = 0;
+  for (int i = threadIdx.x; i < (kPart ? 1 : 0) + ntile + nhalo; i += blockDim.x) iacc[i] = 0;
   float vacc[kAdjPerThread];
 #pragma unroll
   for (int j = 0; j < kAdjPerThread; ++j) vacc[j] = 0.f;
-  // samples with floor(p) in [x0-1, x0+ex-1] touch owned voxels
-  const int flo[3] = {x0 - 1, y0 - 1, z0 - 1}, fhi[3] = {x0 + ex - 1, y0 + ey - 1, z0 + ez - 1};
-  const double lo[3] = {(double)(x0 - 1), (double)(y0 - 1), (double)(z0 - 1)};
+  // samples with floor(p) in [x0-1, x0+ex-1] touch owned voxels; partitioned,
+  // those with floor(p) in [x0, x0+ex-1] (from -1 on the first brick of an axis)
+  const int flo[3] = {kPart && jx ? x0 : x0 - 1, kPart && jy ? y0 : y0 - 1,
+                      kPart && jz ? z0 : z0 - 1};
+  const int fhi[3] = {x0 + ex - 1, y0 + ey - 1, z0 + ez - 1};
+  const double lo[3] = {(double)flo[0], (double)flo[1], (double)flo[2]};
   const double hi[3] = {(double)(x0 + ex), (double)(y0 + ey), (double)(z0 + ez)};
-  const float blo[3] = {x0 - 1 - kLineSlack, y0 - 1 - kLineSlack, z0 - 1 - kLineSlack};
+  const float blo[3] = {flo[0] - kLineSlack, flo[1] - kLineSlack, flo[2] - kLineSlack};
   const float bhi[3] = {x0 + ex + kLineSlack, y0 + ey + kLineSlack, z0 + ez + kLineSlack};
-  // tile origin = the halo corner (x0-1, y0-1, z0-1)
-  const double org[3] = {(x0 - 1) * kFix, (y0 - 1) * kFix, (z0 - 1) * kFix};
+  // tile origin: the halo corner (x0-1, y0-1, z0-1); partitioned, the
+  // brick's first voxel
+  const double org[3] = {(kPart ? x0 : x0 - 1) * kFix, (kPart ? y0 : y0 - 1) * kFix,
+                         (kPart ? z0 : z0 - 1) * kFix};
+  // partitioned bricks on the y = 0 or z = 0 face take the guarded loop
+  const bool guard = kPart && (jy == 0 || jz == 0);
   const float gscale = (float)g.scale;
 
   int nlive = g.n_views;
@@ -927,12 +1018,19 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
             int ka, kb;
             int64_t X[3], D[3];
             if (!ray_run(g, v, iu, iv, flo, fhi, org, ka, kb, X, D)) continue;
-            // the group's accumulation mode is uniform: one loop per mode, so
-            // the sample loop carries no branch on it
-            if (fixed)
-              adj_march<true>(ka, kb, X, D, val, stz, sty, iacc);
-            else
-              adj_march<false>(ka, kb, X, D, val, stz, sty, iacc);
+            // the group's accumulation mode (and the brick's guard) is
+            // uniform: one loop per mode, so the sample loop carries no
+            // branch on it
+            if (kPart && guard) {
+              if (fixed)
+                adj_march<true, true>(ka, kb, X, D, val, stz, sty, tile);
+              else
+                adj_march<false, true>(ka, kb, X, D, val, stz, sty, tile);
+            } else if (fixed) {
+              adj_march<true>(ka, kb, X, D, val, stz, sty, tile);
+            } else {
+              adj_march<false>(ka, kb, X, D, val, stz, sty, tile);
+            }
           }
         }
       }
@@ -940,10 +1038,12 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
       __syncthreads();
       // flush the group's owned voxels into the lane's FP32 registers (fixed
       // group order: the result is bitwise reproducible) and clear them; the
-      // halo keeps whatever it gathered and is never read.  The lane index is
+      // halo keeps whatever it gathered and is never read (partitioned: the
+      // upper halo goes to its FP32 accumulators likewise).  The lane index is
       // made opaque so the 32 LDS addresses are recomputed here rather than
       // hoisted out of the view loops and held (or spilled) throughout.
-      flush_owned<CBX, CBY, CBZ>(iacc, vacc, fixed, qinv, bx, by, nown, stz, sty);
+      flush_owned<CBX, CBY, CBZ, kPart>(tile, vacc, fixed, qinv, bx, by, nown, stz, sty);
+      if (kPart) flush_halo(tile, hacc, fixed, qinv, bx, by, bz, stz, sty);
       __syncthreads();  // cleared before the next group's atomics
     }
   }
@@ -960,6 +1060,10 @@ __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
           unsafeAtomicAdd(o, vacc[j]);  // the output was zero-filled
       }
     }
+  }
+  if (kPart) {
+    // each lane stores the halo slots it accumulated (no barrier needed)
+    for (int h = threadIdx.x; h < nhalo; h += blockDim.x) hout[h] = hacc[h];
   }
 }
 
@@ -982,6 +1086,134 @@ __global__ __launch_bounds__(256, 3) void k_adj_brick(const float* __restrict__ 
   adj_brick_body<CBX, CBY, CBZ>(sino + (size_t)b * g.n_views * g.nv * g.nu, out, g, bx, by, bz,
                                 nbx, nby, blockIdx.x, blockIdx.z, gridDim.z,
                                 !ws && gridDim.z > 1, iacc, st, rs);
+}
+
+// Partitioned A^T (adj_brick_body with kPart).  Owned sums: with `into`
+// set, every part stores (one part) or adds (two, into the zero-filled
+// output: order-independent) into vol; else part 0 stores into vol and part
+// p >= 1 into wsv[p - 1][batch][voxels].  Every part stores its upper-halo
+// sums into halo[part][batch][brick][adj_halo_count].
+template <int CBX, int CBY, int CBZ>
+__global__ __launch_bounds__(256, 3) void k_adj_part(const float* __restrict__ sino,
+                                                  float* __restrict__ vol,
+                                                  float* __restrict__ wsv,
+                                                  float* __restrict__ halo, Geo g, int bx,
+                                                  int by, int bz, int nbx, int nby, int into) {
+  extern __shared__ int iacc[];
+  __shared__ ChunkStage st;
+  __shared__ RaySched rs;
+  if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
+  const int b = blockIdx.y, p = blockIdx.z;
+  const size_t nvox = (size_t)g.nx * g.ny * g.nz;
+  float* out = into || !p ? vol + (size_t)b * nvox : wsv + ((size_t)(p - 1) * gridDim.y + b) * nvox;
+  const size_t nh = adj_halo_count(CBX ? CBX : bx, CBY ? CBY : by, CBZ ? CBZ : bz);
+  float* hout = halo + (((size_t)p * gridDim.y + b) * gridDim.x + blockIdx.x) * nh;
+  adj_brick_body<CBX, CBY, CBZ, true>(sino + (size_t)b * g.n_views * g.nv * g.nu, out, g, bx,
+                                      by, bz, nbx, nby, blockIdx.x, p, gridDim.z,
+                                      into && gridDim.z > 1, iacc, st, rs, hout);
+}
+
+// Adds to s the upper-halo sums that the lower neighbours of brick (jx, jy,
+// jz) hold for its voxel (lx, ly, lz) (fx, fy, fz: the voxel is on the
+// brick's lower x, y, z face and that neighbour exists): offsets d = 1..7
+// (bit 0 x, bit 1 y, bit 2 z), each summed over the parts in part order, added
+// in the order of d (one fixed order per voxel: bitwise reproducible).  All
+// loads of a part are issued before they are added.
+__device__ __forceinline__ float add_halos(float s, const float* __restrict__ halo, int b,
+                                           int batch, int parts, int jx, int jy, int jz, int lx,
+                                           int ly, int lz, bool fx, bool fy, bool fz, int bx,
+                                           int by, int bz, int nbx, int nby, int nbz) {
+  const size_t nh = adj_halo_count(bx, by, bz);
+  const size_t nb = (size_t)nbx * nby * nbz;
+  size_t off[7];
+  bool ok[7];
+  float t[7];
+#pragma unroll
+  for (int d = 1; d < 8; ++d) {
+    const int dx = d & 1, dy = (d >> 1) & 1, dz = d >> 2;
+    ok[d - 1] = (!dx || fx) && (!dy || fy) && (!dz || fz);
+    const size_t src = ((size_t)(jz - dz) * nby + (jy - dy)) * nbx + (jx - dx);
+    off[d - 1] =
+        ok[d - 1] ? src * nh + halo_slot(lx + dx * bx, ly + dy * by, lz + dz * bz, bx, by, bz)
+                  : 0;
+    t[d - 1] = 0.f;
+  }
+  for (int p = 0; p < parts; ++p) {
+    const float* hp = halo + ((size_t)p * batch + b) * nb * nh;
+    float h[7];
+#pragma unroll
+    for (int d = 0; d < 7; ++d) h[d] = ok[d] ? hp[off[d]] : 0.f;
+#pragma unroll
+    for (int d = 0; d < 7; ++d) t[d] += h[d];
+  }
+#pragma unroll
+  for (int d = 0; d < 7; ++d)
+    if (ok[d]) s += t[d];
+  return s;
+}
+
+// Completes the partitioned A^T in place when vol holds part 0's owned sums
+// and wsv the other parts': every voxel gets the parts in part order, then
+// add_halos.  One lane per voxel of an x row, blockIdx.y = y, blockIdx.z =
+// z + nz (b - b0).
+__global__ void k_sum_halo(float* __restrict__ vol, const float* __restrict__ wsv,
+                           const float* __restrict__ halo, Geo g, int bx, int by, int bz,
+                           int nbx, int nby, int nbz, int parts, int batch, int b0) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z % g.nz, b = b0 + (int)blockIdx.z / g.nz;
+  if (x >= g.nx) return;
+  const int jx = x / bx, jy = y / by, jz = z / bz;
+  const int lx = x - jx * bx, ly = y - jy * by, lz = z - jz * bz;
+  const bool fx = lx == 0 && jx > 0, fy = ly == 0 && jy > 0, fz = lz == 0 && jz > 0;
+  const size_t nvox = (size_t)g.nx * g.ny * g.nz;
+  const size_t v = (size_t)b * nvox + ((size_t)z * g.ny + y) * g.nx + x;
+  float s = vol[v];
+  for (int p = 1; p < parts; ++p) s += wsv[(size_t)(p - 1) * batch * nvox + v];
+  if (fx || fy || fz)
+    s = add_halos(s, halo, b, batch, parts, jx, jy, jz, lx, ly, lz, fx, fy, fz, bx, by, bz, nbx,
+                  nby, nbz);
+  vol[v] = s;
+}
+
+// Completes the partitioned A^T in place when vol already holds every part's
+// owned sums: only the voxels on a brick's lower faces change, one lane each
+// (the z faces, then the y faces off the z faces, then the x faces off both;
+// blockIdx.y = batch entry).
+__global__ void k_halo_faces(float* __restrict__ vol, const float* __restrict__ halo, Geo g,
+                             int bx, int by, int bz, int nbx, int nby, int nbz, int parts,
+                             int batch) {
+  const int b = blockIdx.y;
+  const long long nzf = (long long)(nbz - 1) * g.ny * g.nx;
+  const long long nyf = (long long)(nby - 1) * g.nz * g.nx;
+  const long long nxf = (long long)(nbx - 1) * g.nz * g.ny;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int x, y, z;
+  if (i < nzf) {
+    const int pi = (int)(i / g.nx), q = (int)(i - (long long)pi * g.nx);
+    x = q;
+    y = pi % g.ny;
+    z = (pi / g.ny + 1) * bz;
+  } else if ((i -= nzf) < nyf) {
+    const int pi = (int)(i / g.nx), q = (int)(i - (long long)pi * g.nx);
+    x = q;
+    z = pi % g.nz;
+    y = (pi / g.nz + 1) * by;
+    if (z % bz == 0 && z > 0) return;  // on a z face
+  } else if ((i -= nyf) < nxf) {
+    const int pi = (int)(i / g.ny), q = (int)(i - (long long)pi * g.ny);
+    y = q;
+    z = pi % g.nz;
+    x = (pi / g.nz + 1) * bx;
+    if ((z % bz == 0 && z > 0) || (y % by == 0 && y > 0)) return;  // on a z or y face
+  } else {
+    return;
+  }
+  const int jx = x / bx, jy = y / by, jz = z / bz;
+  const int lx = x - jx * bx, ly = y - jy * by, lz = z - jz * bz;
+  const bool fx = lx == 0 && jx > 0, fy = ly == 0 && jy > 0, fz = lz == 0 && jz > 0;
+  const size_t v = (size_t)b * g.nx * g.ny * g.nz + ((size_t)z * g.ny + y) * g.nx + x;
+  vol[v] = add_halos(vol[v], halo, b, batch, parts, jx, jy, jz, lx, ly, lz, fx, fy, fz, bx, by,
+                     bz, nbx, nby, nbz);
 }
 
 // out[i] = ws[0][i] + ws[1][i] + ... in part order (FP32, one fixed order: the
@@ -1147,6 +1379,52 @@ int launch_adj_brick(const float* sino, float* vol, float* ws, int batch, const 
     hipLaunchKernelGGL(k_sum_parts, dim3(blocks), dim3(256), 0, st, ws, vol, n, parts);
   }
   return hipGetLastError() == hipSuccess ? 0 : 4;
+}
+
+size_t adj_part_ws_bytes(int batch, const Geo& g, int bx, int by, int bz, int parts) {
+  const size_t nvox = (size_t)g.nx * g.ny * g.nz;
+  const size_t nb = (size_t)((g.nx + bx - 1) / bx) * ((g.ny + by - 1) / by) * ((g.nz + bz - 1) / bz);
+  return ((size_t)parts * batch * nb * adj_halo_count(bx, by, bz) +
+          (size_t)(parts - 1) * batch * nvox) *
+         sizeof(float);
+}
+
+int launch_adj_part(const float* sino, float* vol, float* ws, int batch, const Geo& g, int bx,
+                    int by, int bz, int block, int parts, bool into, hipStream_t st) {
+  const int nbx = (g.nx + bx - 1) / bx, nby = (g.ny + by - 1) / by, nbz = (g.nz + bz - 1) / bz;
+  if (bx * by * bz > kAdjPerThread * block || parts < 1 || (into && parts > 2)) return 1;
+  const size_t lds = adj_ptile_bytes(bx, by, bz);
+  if (lds > kMaxTileBytes) return 1;
+  auto kern = &k_adj_part<0, 0, 0>;
+#define DTV_PICK(a, b, c) \
+  if (bx == a && by == b && bz == c) kern = &k_adj_part<a, b, c>;
+  DTV_BRICK_SHAPES(DTV_PICK)
+#undef DTV_PICK
+  const size_t nb = (size_t)nbx * nby * nbz;
+  float* halo = ws;
+  float* wsv = ws + (size_t)parts * batch * nb * adj_halo_count(bx, by, bz);
+  hipLaunchKernelGGL(kern, dim3(nbx * nby * nbz, batch, parts), dim3(block), lds, st, sino, vol,
+                     wsv, halo, g, bx, by, bz, nbx, nby, into ? 1 : 0);
+  if (hipGetLastError() != hipSuccess) return 4;
+  if (!into && parts > 1) {
+    const int tb = std::min(256, (g.nx + 63) / 64 * 64);
+    const int per = std::max(1, 65535 / g.nz);  // batch entries per launch (grid z <= 65535)
+    for (int b0 = 0; b0 < batch; b0 += per) {
+      const int n = std::min(per, batch - b0);
+      hipLaunchKernelGGL(k_sum_halo, dim3((g.nx + tb - 1) / tb, g.ny, g.nz * n), dim3(tb), 0, st,
+                         vol, wsv, halo, g, bx, by, bz, nbx, nby, nbz, parts, batch, b0);
+      if (hipGetLastError() != hipSuccess) return 4;
+    }
+  } else {
+    const long long nf = (long long)(nbz - 1) * g.ny * g.nx + (long long)(nby - 1) * g.nz * g.nx +
+                         (long long)(nbx - 1) * g.nz * g.ny;
+    if (nf > 0) {
+      hipLaunchKernelGGL(k_halo_faces, dim3((unsigned)((nf + 255) / 256), batch), dim3(256), 0,
+                         st, vol, halo, g, bx, by, bz, nbx, nby, nbz, parts, batch);
+      if (hipGetLastError() != hipSuccess) return 4;
+    }
+  }
+  return 0;
 }
 
 }  // namespace dtv

@@ -38,11 +38,28 @@ __host__ __device__ constexpr int adj_tile_sty(int bx) { return bx + 2 + DTV_ADJ
 __host__ __device__ constexpr int adj_tile_stz(int bx, int by) {
   return adj_tile_sty(bx) * (by + 2) + DTV_ADJ_STZ_PAD;
 }
+// Partitioned A^T tiles (each sample handled by the brick holding its floor
+// cell): x, y and z from the brick's first voxel to the upper halo, rows of
+// bx + 2 words (the last a pad word, which also takes the taps at x = -1 of
+// the next row's floor -1 samples; a leading pad word takes the first row's),
+// z stride odd; then one FP32 accumulator per upper-halo voxel.
+__host__ __device__ constexpr int adj_ptile_sty(int bx) { return bx + 2; }
+__host__ __device__ constexpr int adj_ptile_stz(int bx, int by) {
+  return (adj_ptile_sty(bx) * (by + 1)) | 1;
+}
+// Upper-halo voxels of a brick: the tile voxels with lx == bx, ly == by or
+// lz == bz (local coordinates 0..b per axis).
+__host__ __device__ constexpr int adj_halo_count(int bx, int by, int bz) {
+  return (bx + 1) * (by + 1) * (bz + 1) - bx * by * bz;
+}
 inline size_t fwd_tile_bytes(int bx, int by, int bz) {
   return (size_t)fwd_tile_stz(bx, by) * (bz + 1) * 4;
 }
 inline size_t adj_tile_bytes(int bx, int by, int bz) {
   return (size_t)adj_tile_stz(bx, by) * (bz + 2) * 4;
+}
+inline size_t adj_ptile_bytes(int bx, int by, int bz) {
+  return (1 + (size_t)adj_ptile_stz(bx, by) * (bz + 1) + adj_halo_count(bx, by, bz)) * 4;
 }
 constexpr size_t kMaxTileBytes = 56 * 1024;
 
@@ -63,5 +80,17 @@ int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int
                      int bz, int block, int parts, hipStream_t st, void* fix_ws = nullptr);
 int launch_adj_brick(const float* sino, float* vol, float* ws, int batch, const Geo& g, int bx,
                      int by, int bz, int block, int parts, hipStream_t st);
+// Partitioned A^T: every sample is marched by the one brick holding its floor
+// cell; a brick's taps on its upper halo are summed per voxel in FP32 and
+// stored in ws, and a second kernel adds them into the neighbours' voxels in
+// one fixed order (bitwise reproducible).  ws: adj_part_ws_bytes (every
+// part's halo sums, then the partial volumes of parts 1..).  into (at most
+// two parts): the parts' owned sums go straight into vol, which the caller
+// zero-filled for two (added with FP32 atomics: order-independent for two
+// terms); else part 0 stores into vol, the others into ws, and the second
+// kernel adds them in part order.
+size_t adj_part_ws_bytes(int batch, const Geo& g, int bx, int by, int bz, int parts);
+int launch_adj_part(const float* sino, float* vol, float* ws, int batch, const Geo& g, int bx,
+                    int by, int bz, int block, int parts, bool into, hipStream_t st);
 
 }  // namespace dtv

@@ -634,16 +634,43 @@ int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipS
   return run_adjoint(sino, vol, batch, g, st, default_launch(true, g, batch), nullptr, 0);
 }
 
-// Bytes of the A^T workspace for `parts` view parts: each part's partial volumes.
-size_t adjoint_ws_bytes(int32_t batch, const Geo& g, int parts) {
-  return parts > 1 ? (size_t)parts * batch * g.nx * g.ny * g.nz * sizeof(float) : 0;
+// Whether the workspace path partitions the samples over the bricks (its
+// tile, with the upper-halo accumulators, fits the LDS limit).
+bool adj_partitioned(const Launch& L) {
+  return L.algo == DTV_ALGO_BRICK && L.bx * L.by * L.bz <= 32 * L.block &&
+         adj_ptile_bytes(L.bx, L.by, L.bz) <= kMaxTileBytes;
+}
+
+// Bytes of the A^T workspace: partitioned, the partial volumes of parts 1..
+// and every part's upper-halo sums; else each part's partial volumes (none
+// for one part).
+size_t adjoint_ws_bytes(int32_t batch, const Geo& g, const Launch& L) {
+  if (L.algo != DTV_ALGO_BRICK) return 0;
+  if (adj_partitioned(L)) return adj_part_ws_bytes(batch, g, L.bx, L.by, L.bz, L.parts);
+  return L.parts > 1 ? (size_t)L.parts * batch * g.nx * g.ny * g.nz * sizeof(float) : 0;
 }
 
 int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipStream_t st,
                 const Launch& L, float* ws, size_t ws_bytes) {
+  if (ws && adj_partitioned(L)) {
+    // each sample marched by one brick; halos and parts summed in a fixed order
+    if (ws_bytes < adjoint_ws_bytes(batch, g, L)) return DTV_ERR_INVALID_ARGUMENT;
+    if (g.ny > 65535) return DTV_ERR_TOO_LARGE;
+    // one part stores, two add into zeros when the output takes the hardware
+    // atomics; else the parts go through the workspace
+    bool into = L.parts == 1;
+    if (L.parts == 2 && check_atomic_target(vol) == DTV_OK) {
+      const int rc = run_zero(vol, (size_t)batch * g.nx * g.ny * g.nz, st);
+      if (rc) return rc;
+      into = true;
+    }
+    const int rc =
+        launch_adj_part(sino, vol, ws, batch, g, L.bx, L.by, L.bz, L.block, L.parts, into, st);
+    return rc == 0 ? DTV_OK : (rc == 1 ? DTV_ERR_INVALID_ARGUMENT : DTV_ERR_HIP);
+  }
   if (L.algo == DTV_ALGO_BRICK && ws && L.parts > 1) {
     // every part stores its partial volumes; a second kernel adds them in order
-    if (ws_bytes < adjoint_ws_bytes(batch, g, L.parts)) return DTV_ERR_INVALID_ARGUMENT;
+    if (ws_bytes < adjoint_ws_bytes(batch, g, L)) return DTV_ERR_INVALID_ARGUMENT;
     return launch_adj_brick(sino, vol, ws, batch, g, L.bx, L.by, L.bz, L.block, L.parts, st);
   }
   if (L.algo == DTV_ALGO_BRICK) {
@@ -926,10 +953,10 @@ int dtv_cone_adjoint_ws(const float* sino, float* volume, int32_t batch,
   Launch L;
   rc = resolve_launch(launch, L, true, g, batch, true);
   if (rc) return rc;
-  if (L.algo == DTV_ALGO_BRICK && L.parts > 1) {
-    if (!workspace) return DTV_ERR_NULL_POINTER;
-    if (workspace_bytes < adjoint_ws_bytes(batch, g, L.parts)) return DTV_ERR_INVALID_ARGUMENT;
-  }
+  // a NULL workspace is accepted for one part (the unpartitioned kernel)
+  if (!workspace && L.algo == DTV_ALGO_BRICK && L.parts > 1) return DTV_ERR_NULL_POINTER;
+  if (workspace && workspace_bytes < adjoint_ws_bytes(batch, g, L))
+    return DTV_ERR_INVALID_ARGUMENT;
   return run_adjoint(sino, volume, batch, g, (hipStream_t)stream, L, workspace, workspace_bytes);
 }
 
@@ -950,7 +977,7 @@ int dtv_cone_adjoint_plan(int32_t batch, const int32_t volume_size[3],
   plan->brick[2] = L.bz;
   plan->block = L.block;
   plan->view_parts = L.algo == DTV_ALGO_BRICK ? L.parts : 1;
-  *workspace_bytes = L.algo == DTV_ALGO_BRICK ? adjoint_ws_bytes(batch, g, L.parts) : 0;
+  *workspace_bytes = adjoint_ws_bytes(batch, g, L);
   return DTV_OK;
 }
 

@@ -132,12 +132,14 @@ def adjoint_plan(geom: ConeGeometry, batch: int = 1, launch=None):
 def adjoint(sino: torch.Tensor, geom: ConeGeometry, launch=None) -> torch.Tensor:
     """Matched A^T: exact transpose of :func:`forward`.
 
-    Each brick's views are split over several workgroups whose partial
-    volumes go to a workspace from torch's allocator and are added up in a
-    fixed order (``dtv_cone_adjoint_ws``): bitwise reproducible for any
-    split.  A ``launch`` with ``view_parts`` 1 or 2 and no other wish for a
-    workspace runs ``dtv_cone_adjoint_ex`` instead (two parts add into the
-    output with atomics)."""
+    Runs ``dtv_cone_adjoint_ws`` with a workspace from torch's allocator:
+    each sample is marched by the one brick holding its floor cell, the
+    bricks' upper-halo sums (and the partial volumes of a split beyond two
+    parts) are added in a fixed order, so the result is bitwise reproducible
+    for any split.  A ``launch`` with ``view_parts`` 1 or 2 runs
+    ``dtv_cone_adjoint_ex`` instead: the unpartitioned bricks, which the
+    multi-instance launches share (two parts add into the output with
+    atomics)."""
     _check_f32(sino, "sino")
     B = _batch_of(sino, _sino_numel(geom), "sino")
     out = torch.empty(geom.volume_shape(B), dtype=torch.float32, device=sino.device)

@@ -166,15 +166,23 @@ int dtv_cone_adjoint_ex(const float* sino, float* volume, int32_t batch,
                         const double* views, int32_t n_views, const dtv_scheme* scheme,
                         const dtv_launch* launch, void* stream);
 
-/* Matched A^T with each brick's views split over view_parts workgroups that
- * store their partial volumes in a caller-provided workspace; a second kernel
- * adds the parts up in part order, so the result is bitwise reproducible for
- * any number of parts (up to 16), needs no atomics and no zero fill.  The
- * workspace holds view_parts x batch x voxels floats (device memory; see
- * dtv_cone_adjoint_plan).  launch == NULL or view_parts == 0 takes the
- * library's split for the volume size.  No reference counterpart: it serves
- * callers that can hand over scratch memory (the Python operators pass a
- * torch allocation); JITStandardGeometry.backward-shaped callers keep
+/* Matched A^T with the samples partitioned over the bricks: each sample is
+ * marched by the one brick holding its floor cell (the dtv_cone_adjoint
+ * bricks also march, and discard, the samples that only touch their halo,
+ * about 17% more), a brick's taps on its upper-halo voxels are summed per
+ * voxel and stored in the workspace, and a second kernel adds them into the
+ * neighbouring bricks' voxels in one fixed order.  Each brick's views are
+ * split over view_parts workgroups: one part stores into the volume, two add
+ * into it (zero-filled by the call; FP32 addition of two terms does not depend
+ * on their order), more store their partial volumes in the workspace and the
+ * second kernel adds them in part order; so the result is bitwise
+ * reproducible for any split (up to 16 parts).  The workspace (device memory,
+ * dtv_cone_adjoint_plan gives its size) holds every part's halo sums and the
+ * partial volumes of parts beyond the first.  A NULL workspace is taken for
+ * one part (the unpartitioned kernel).  launch == NULL or view_parts == 0
+ * takes the library's split for the volume size.  No reference counterpart:
+ * it serves callers that can hand over scratch memory (the Python operators
+ * pass a torch allocation); JITStandardGeometry.backward-shaped callers keep
  * dtv_*_adjoint. */
 int dtv_cone_adjoint_ws(const float* sino, float* volume, int32_t batch,
                         const int32_t volume_size[3], const int32_t detector_size[2],
@@ -184,7 +192,9 @@ int dtv_cone_adjoint_ws(const float* sino, float* volume, int32_t batch,
 
 /* The launch dtv_cone_adjoint_ws takes for these sizes and `launch` (NULL =
  * defaults): kernel, brick, lanes and view parts in *plan, and the workspace
- * bytes it needs (0 for one part).  No GPU work. */
+ * bytes it needs (4 x (parts x batch x bricks x halo voxels per brick +
+ * (parts - 1) x batch x voxels), halo voxels per brick (bx+1)(by+1)(bz+1) -
+ * bx by bz; 0 for the ray-driven kernel).  No GPU work. */
 int dtv_cone_adjoint_plan(int32_t batch, const int32_t volume_size[3],
                           const int32_t detector_size[2], int32_t n_views,
                           const dtv_launch* launch, dtv_launch* plan, size_t* workspace_bytes);

@@ -20,6 +20,12 @@
 // style detectors with a source off the mid-plane, and sources next to the
 // volume (depth ratios up to 2 across a brick, corners behind the source);
 // bricks from 8^3 to 32x32x8 at seeded positions, edge bricks included.
+// The partitioned adjoint (each sample marched only by the brick holding its
+// floor cell: floor in [o, o + e - 1], from -1 on the first brick of an axis)
+// stages the box [o, o + e] (o - 1 on a first brick) and reads back its
+// upper halo too, so there the bound must hold for every tile voxel in
+// [o, o + e] inside the volume, over the samples of that window; each brick is
+// checked both ways.
 // Prints "ok <checks> <inf_views> <max_ratio>" or the first failure (exit 1).
 // Built and run by tests/test_bound_native.py.
 #include <cmath>
@@ -96,16 +102,23 @@ static int fails = 0;
 static long checks = 0, inf_views = 0;
 static double max_ratio = 0.0;
 
-static void check_brick(const Case& cs, int bx, int by, int bz, int jx, int jy, int jz) {
+static void check_brick(const Case& cs, int bx, int by, int bz, int jx, int jy, int jz,
+                        bool part) {
   const Geo& g = cs.g;
   const int x0 = jx * bx, y0 = jy * by, z0 = jz * bz;
   if (x0 >= g.nx || y0 >= g.ny || z0 >= g.nz) return;
   const int ex = std::min(bx, g.nx - x0), ey = std::min(by, g.ny - y0),
             ez = std::min(bz, g.nz - z0);
-  const int flo[3] = {x0 - 1, y0 - 1, z0 - 1}, fhi[3] = {x0 + ex - 1, y0 + ey - 1, z0 + ez - 1};
-  const double lo[3] = {(double)(x0 - 1), (double)(y0 - 1), (double)(z0 - 1)};
+  const int flo[3] = {part && jx ? x0 : x0 - 1, part && jy ? y0 : y0 - 1,
+                      part && jz ? z0 : z0 - 1};
+  const int fhi[3] = {x0 + ex - 1, y0 + ey - 1, z0 + ez - 1};
+  const double lo[3] = {(double)flo[0], (double)flo[1], (double)flo[2]};
   const double hi[3] = {(double)(x0 + ex), (double)(y0 + ey), (double)(z0 + ez)};
-  std::vector<double> acc((size_t)ex * ey * ez);
+  // voxels whose sums are read back: owned, plus the upper halo inside the
+  // volume when partitioned
+  const int ax = part ? std::min(ex + 1, g.nx - x0) : ex, ay = part ? std::min(ey + 1, g.ny - y0) : ey,
+            az = part ? std::min(ez + 1, g.nz - z0) : ez;
+  std::vector<double> acc((size_t)ax * ay * az);
   for (const View& v : cs.views) {
     ViewProj vp;
     view_proj(v, vp);
@@ -157,13 +170,13 @@ static void check_brick(const Case& cs, int bx, int by, int bz, int jx, int jy, 
           for (int t = 0; t < 8; ++t) {
             const int64_t ix = f[0] + (t & 1), iy = f[1] + ((t >> 1) & 1),
                           iz = f[2] + ((t >> 2) & 1);
-            if (ix < x0 || ix >= x0 + ex || iy < y0 || iy >= y0 + ey || iz < z0 ||
-                iz >= z0 + ez)
+            if (ix < x0 || ix >= x0 + ax || iy < y0 || iy >= y0 + ay || iz < z0 ||
+                iz >= z0 + az)
               continue;
             const double w = ((t & 1) ? fr[0] : 1.0 - fr[0]) *
                              (((t >> 1) & 1) ? fr[1] : 1.0 - fr[1]) *
                              (((t >> 2) & 1) ? fr[2] : 1.0 - fr[2]);
-            acc[((size_t)(iz - z0) * ey + (iy - y0)) * ex + (ix - x0)] += w;
+            acc[((size_t)(iz - z0) * ay + (iy - y0)) * ax + (ix - x0)] += w;
           }
         }
       }
@@ -177,8 +190,8 @@ static void check_brick(const Case& cs, int bx, int by, int bz, int jx, int jy, 
     ++checks;
     if (m > 0.0) max_ratio = std::max(max_ratio, m / (double)wb);
     if (m > (double)wb) {
-      std::printf("FAIL bound: brick (%d,%d,%d)+(%d,%d,%d): max sum w %.6f > bound %.6f\n",
-                  x0, y0, z0, ex, ey, ez, m, (double)wb);
+      std::printf("FAIL bound%s: brick (%d,%d,%d)+(%d,%d,%d): max sum w %.6f > bound %.6f\n",
+                  part ? " (partitioned)" : "", x0, y0, z0, ex, ey, ez, m, (double)wb);
       ++fails;
       return;
     }
@@ -215,19 +228,25 @@ int main() {
     for (const auto& sh : shapes) {
       const int nbx = (cs.g.nx + sh[0] - 1) / sh[0], nby = (cs.g.ny + sh[1] - 1) / sh[1],
                 nbz = (cs.g.nz + sh[2] - 1) / sh[2];
-      // the corner bricks (edge bricks are partial) and two seeded ones
-      for (int q = 0; q < 4; ++q) {
+      // the corner bricks (edge bricks are partial), a middle one and two
+      // seeded ones
+      for (int q = 0; q < 5; ++q) {
         int jx, jy, jz;
         if (q < 2) {
           jx = q ? nbx - 1 : 0;
           jy = q ? nby - 1 : 0;
           jz = q ? nbz - 1 : 0;
+        } else if (q == 2) {
+          jx = nbx / 2;
+          jy = nby / 2;
+          jz = nbz / 2;
         } else {
           jx = (int)(uni(rng) * nbx);
           jy = (int)(uni(rng) * nby);
           jz = (int)(uni(rng) * nbz);
         }
-        check_brick(cs, sh[0], sh[1], sh[2], jx, jy, jz);
+        check_brick(cs, sh[0], sh[1], sh[2], jx, jy, jz, false);
+        check_brick(cs, sh[0], sh[1], sh[2], jx, jy, jz, true);
         if (fails) return 1;
       }
     }

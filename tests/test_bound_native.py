@@ -9,7 +9,9 @@ every ray of the detector with the kernels' own set-up (ray_core, 32.32
 fixed point, exact_window) over circular orbits at magnification 1.1 to 10,
 tilted detectors with an off-plane source and sources next to the volume,
 on bricks from 8^3 to 32x32x8 (edge bricks included), and checks the bound
-and that no ray outside the rectangle reaches the brick."""
+and that no ray outside the rectangle reaches the brick.  Each brick is also
+checked as the partitioned adjoint stages it (samples whose floor cell is the
+brick's own, the upper-halo voxels read back too)."""
 import os
 import shutil
 import subprocess

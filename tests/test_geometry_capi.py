@@ -143,7 +143,8 @@ def test_capi_rejects_bad_arguments_without_gpu():
     args = (ctypes.c_void_p(16), ctypes.c_void_p(16), 1, _capi.int3((8, 8, 8)), _capi.int2((8, 8)),
             vp, 4, ctypes.byref(sc), ctypes.byref(L))
     assert lib.dtv_cone_adjoint_ws(*args, None, 0, None) == 2
-    assert lib.dtv_cone_adjoint_ws(*args, ctypes.c_void_p(64), 3 * 512 * 4 - 1, None) == 1
+    need = _part_ws_bytes((8, 8, 8), (16, 16, 8), 1, 3)
+    assert lib.dtv_cone_adjoint_ws(*args, ctypes.c_void_p(64), need - 1, None) == 1
     L.view_parts = 17
     assert lib.dtv_cone_adjoint_ws(*args, ctypes.c_void_p(64), 1 << 20, None) == 1
     # the deterministic forward: a workspace of 256 + 8 bytes per ray, 256-byte aligned
@@ -237,23 +238,40 @@ def test_pair_bytes_accounting():
     assert b["pair"] == b["forward"] + b["adjoint"]
 
 
+def _part_ws_bytes(vol, brick, batch, parts):
+    """Workspace of the partitioned A^T: the partial volumes of parts 1..,
+    then each part's upper-halo sums, (b+1)^3 - b^3 floats per brick (for
+    b = (bx, by, bz) per axis)."""
+    X, Y, Z = vol
+    bx, by, bz = brick
+    nb = -(-X // bx) * -(-Y // by) * -(-Z // bz)
+    halo = (bx + 1) * (by + 1) * (bz + 1) - bx * by * bz
+    return ((parts - 1) * batch * X * Y * Z + parts * batch * nb * halo) * 4
+
+
 def test_adjoint_plan_splits_by_volume_size():
     """The workspace split the library picks (no GPU work): two parts at
     256^3 / 60 views, none at 512x512x384, many on small volumes (at most 16),
-    and the workspace holds parts x batch x voxels floats."""
+    and the workspace holds the partial volumes of parts 1.. and every part's
+    upper-halo sums."""
     from dtvproj import configs
     cases = [(configs.sparse_256(), 1, (32, 16, 16), 2), (configs.large_512(), 1, (32, 16, 16), 1),
              (configs.fista_64(), 1, (16, 16, 16), 16), (configs.unroll_128(), 4, (32, 16, 16), 4)]
     for g, batch, brick, parts in cases:
         plan, nbytes = ops.adjoint_plan(g, batch)
-        X, Y, Z = g.volume_size
         assert plan.algorithm == 2 and tuple(plan.brick) == brick and plan.view_parts == parts
-        assert nbytes == (parts * batch * X * Y * Z * 4 if parts > 1 else 0)
+        assert nbytes == _part_ws_bytes(g.volume_size, brick, batch, parts)
+    # halo count of a 32x16x16 brick: the z = 16 plane 33 x 17, the y = 16
+    # rows of the planes below 33 x 16, the x = 32 column of the rest 16 x 16
+    assert _part_ws_bytes((32, 16, 16), (32, 16, 16), 1, 1) == (33 * 17 + 33 * 16 + 16 * 16) * 4
     # a fixed launch keeps its shape and split
     plan, nbytes = ops.adjoint_plan(configs.fista_64(), 1,
                                     dict(algorithm=2, brick=(8, 8, 8), block=64, view_parts=5))
     assert tuple(plan.brick) == (8, 8, 8) and plan.block == 64 and plan.view_parts == 5
-    assert nbytes == 5 * 64 ** 3 * 4
+    assert nbytes == _part_ws_bytes((64, 64, 64), (8, 8, 8), 1, 5)
+    # the ray-driven kernel needs none
+    plan, nbytes = ops.adjoint_plan(configs.fista_64(), 1, dict(algorithm=1))
+    assert plan.algorithm == 1 and nbytes == 0
 
 
 def test_forward_plan_workspace():

@@ -159,14 +159,14 @@ def test_auto_parts_split_the_costly_bricks():
 
 
 def _adjoint_ws(y, g, cuda, brick, parts):
-    """A^T through dtv_cone_adjoint_ws with an explicit split."""
+    """A^T through dtv_cone_adjoint_ws with an explicit split (workspace size
+    from dtv_cone_adjoint_plan)."""
     import ctypes
-    import numpy as np
     from dtvproj import _capi
     B = y.shape[0]
     out = torch.empty(g.volume_shape(B), device=cuda)
     L = _capi.launch_struct(algorithm=2, brick=brick, block=256, view_parts=parts)
-    nb = parts * B * int(np.prod(g.volume_size)) * 4 if parts > 1 else 0
+    _, nb = ops.adjoint_plan(g, B, dict(algorithm=2, brick=brick, block=256, view_parts=parts))
     ws = torch.empty(max(nb // 4, 1), device=cuda)
     sc = _capi.scheme_struct(g.scheme)
     rc = _capi.lib().dtv_cone_adjoint_ws(
