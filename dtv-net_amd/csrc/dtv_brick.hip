@@ -1304,6 +1304,137 @@ __global__ __launch_bounds__(256, 3) void k_adj_brick_multi(const dtv_instance* 
   }
 }
 
+// Partitioned matched A^T over uneven instances (adj_brick_body with kPart):
+// the owned sums go into the instance's volume (stored, or added by two parts
+// into zeros), the upper-halo sums of item (brick, part) to halo +
+// hoff[instance] + (part x bricks + brick) x adj_halo_count.
+template <int CBX, int CBY, int CBZ>
+__global__ __launch_bounds__(256, 3) void k_adj_part_multi(const dtv_instance* __restrict__ inst,
+                                                        const dtv_work_item* __restrict__ work,
+                                                        int n_work, int* counter,
+                                                        const long long* __restrict__ hoff,
+                                                        float* __restrict__ halo) {
+  extern __shared__ int iacc[];
+  __shared__ ChunkStage st;
+  __shared__ RaySched rs;
+  __shared__ int slot;
+  if (threadIdx.x < kSchedBins) rs.hist[threadIdx.x] = 0u;
+  for (int done = 0;; ++done) {
+    const int w = next_item(counter, &slot, done);
+    if (w >= n_work) break;
+    const dtv_work_item it = work[w];
+    const dtv_instance in = inst[it.instance];
+    const Geo g = instance_geo(in);
+    const int nb = in.nbx * in.nby * in.nbz;
+    const size_t nh = adj_halo_count(CBX ? CBX : in.bx, CBY ? CBY : in.by, CBZ ? CBZ : in.bz);
+    float* hout = halo + hoff[it.instance] + (size_t)it.brick * nh;
+    adj_brick_body<CBX, CBY, CBZ, true>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby,
+                                        it.brick % nb, it.brick / nb, max(1, in.view_parts),
+                                        in.view_parts > 1, iacc, st, rs, hout);
+  }
+}
+
+// Lanes on a brick's lower faces of an instance (k_halo_faces' enumeration).
+__host__ __device__ inline long long face_lanes(int nx, int ny, int nz, int nbx, int nby,
+                                                int nbz) {
+  return (long long)(nbz - 1) * ny * nx + (long long)(nby - 1) * nz * nx +
+         (long long)(nbx - 1) * nz * ny;
+}
+
+// Workspace header of the partitioned multi-instance A^T, from the device
+// table (so a captured graph recomputes it): hoff[i] = floats of halo sums
+// before instance i (view parts x bricks x adj_halo_count each), foff[i] =
+// face-kernel workgroups before instance i (whole workgroups per instance).
+// One workgroup: each lane sums a contiguous run of instances, lane 0 scans.
+__global__ void k_multi_offsets(const dtv_instance* __restrict__ inst, int n,
+                                long long* __restrict__ hoff, long long* __restrict__ foff) {
+  __shared__ long long th[256], tf[256];
+  const int t = threadIdx.x, per = (n + 255) / 256;
+  const int a = min(n, t * per), e = min(n, a + per);
+  long long sh = 0, sf = 0;
+  for (int i = a; i < e; ++i) {
+    const dtv_instance in = inst[i];
+    sh += (long long)max(1, in.view_parts) * in.nbx * in.nby * in.nbz *
+          adj_halo_count(in.bx, in.by, in.bz);
+    sf += (face_lanes(in.nx, in.ny, in.nz, in.nbx, in.nby, in.nbz) + 255) / 256;
+  }
+  th[t] = sh;
+  tf[t] = sf;
+  __syncthreads();
+  if (t == 0) {
+    long long ch = 0, cf = 0;
+    for (int k = 0; k < 256; ++k) {
+      const long long h = th[k], f = tf[k];
+      th[k] = ch;
+      tf[k] = cf;
+      ch += h;
+      cf += f;
+    }
+    hoff[n] = ch;
+    foff[n] = cf;
+  }
+  __syncthreads();
+  sh = th[t];
+  sf = tf[t];
+  for (int i = a; i < e; ++i) {
+    const dtv_instance in = inst[i];
+    hoff[i] = sh;
+    foff[i] = sf;
+    sh += (long long)max(1, in.view_parts) * in.nbx * in.nby * in.nbz *
+          adj_halo_count(in.bx, in.by, in.bz);
+    sf += (face_lanes(in.nx, in.ny, in.nz, in.nbx, in.nby, in.nbz) + 255) / 256;
+  }
+}
+
+// k_halo_faces over every instance of a multi launch: workgroup w belongs to
+// the instance i with foff[i] <= w < foff[i + 1] (found by bisection); its
+// lanes enumerate that instance's face voxels as k_halo_faces does.
+__global__ void k_halo_faces_multi(const dtv_instance* __restrict__ inst, int n,
+                                   const long long* __restrict__ hoff,
+                                   const long long* __restrict__ foff,
+                                   const float* __restrict__ halo) {
+  const long long w = blockIdx.x;
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (foff[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  const dtv_instance in = inst[lo];
+  const int bx = in.bx, by = in.by, bz = in.bz, nx = in.nx, ny = in.ny, nz = in.nz;
+  const int nbx = in.nbx, nby = in.nby, nbz = in.nbz;
+  const long long nzf = (long long)(nbz - 1) * ny * nx, nyf = (long long)(nby - 1) * nz * nx,
+                  nxf = (long long)(nbx - 1) * nz * ny;
+  long long i = (w - foff[lo]) * blockDim.x + threadIdx.x;
+  int x, y, z;
+  if (i < nzf) {
+    const int pi = (int)(i / nx), q = (int)(i - (long long)pi * nx);
+    x = q;
+    y = pi % ny;
+    z = (pi / ny + 1) * bz;
+  } else if ((i -= nzf) < nyf) {
+    const int pi = (int)(i / nx), q = (int)(i - (long long)pi * nx);
+    x = q;
+    z = pi % nz;
+    y = (pi / nz + 1) * by;
+    if (z % bz == 0 && z > 0) return;
+  } else if ((i -= nyf) < nxf) {
+    const int pi = (int)(i / ny), q = (int)(i - (long long)pi * ny);
+    y = q;
+    z = pi % nz;
+    x = (pi / nz + 1) * bx;
+    if ((z % bz == 0 && z > 0) || (y % by == 0 && y > 0)) return;
+  } else {
+    return;
+  }
+  const int jx = x / bx, jy = y / by, jz = z / bz;
+  const int lx = x - jx * bx, ly = y - jy * by, lz = z - jz * bz;
+  const bool fx = lx == 0 && jx > 0, fy = ly == 0 && jy > 0, fz = lz == 0 && jz > 0;
+  float* o = in.out + ((size_t)z * ny + y) * nx + x;
+  *o = add_halos(*o, halo + hoff[lo], 0, 1, max(1, in.view_parts), jx, jy, jz, lx, ly, lz, fx,
+                 fy, fz, bx, by, bz, nbx, nby, nbz);
+}
+
 // Adjoint instances whose views are split: out = 0 (the parts add into it),
 // blockIdx.y = instance.
 __global__ void k_zero_split_multi(const dtv_instance* __restrict__ inst) {
@@ -1432,7 +1563,8 @@ int launch_adj_part(const float* sino, float* vol, float* ws, int batch, const G
 namespace {
 
 int validate_multi(const dtv_instance* ih, int32_t n_inst, const dtv_work_item* wh,
-                   int32_t n_work, int32_t block, bool adjoint, size_t& lds) {
+                   int32_t n_work, int32_t block, bool adjoint, size_t& lds,
+                   bool part = false) {
   if (!ih || !wh) return DTV_ERR_NULL_POINTER;
   if (n_inst < 1 || n_work < 1 || n_inst > 65535) return DTV_ERR_INVALID_ARGUMENT;
   if (block < 64 || block > 256 || block % 64) return DTV_ERR_INVALID_ARGUMENT;
@@ -1455,7 +1587,7 @@ int validate_multi(const dtv_instance* ih, int32_t n_inst, const dtv_work_item* 
     size_t l;
     if (adjoint) {
       if (in.bx * in.by * in.bz > kAdjPerThread * block) return DTV_ERR_INVALID_ARGUMENT;
-      l = adj_tile_bytes(in.bx, in.by, in.bz);
+      l = part ? adj_ptile_bytes(in.bx, in.by, in.bz) : adj_tile_bytes(in.bx, in.by, in.bz);
     } else {
       l = fwd_tile_bytes(in.bx, in.by, in.bz);
     }
@@ -1491,12 +1623,31 @@ unsigned multi_grid(const void* kern, int block, size_t lds, bool persistent,
   return (unsigned)(n > 0 ? n : 1);
 }
 
+// Workspace of the partitioned multi-instance A^T: a header of hoff[n + 1]
+// and foff[n + 1] (int64, k_multi_offsets) rounded up to 256 bytes, then the
+// halo sums.
+size_t multi_ws_header(int32_t n_inst) { return ((size_t)16 * (n_inst + 1) + 255) / 256 * 256; }
+size_t multi_ws_bytes(const dtv_instance* ih, int32_t n_inst) {
+  size_t h = 0;
+  for (int i = 0; i < n_inst; ++i)
+    h += (size_t)std::max(1, ih[i].view_parts) * ih[i].nbx * ih[i].nby * ih[i].nbz *
+         adj_halo_count(ih[i].bx, ih[i].by, ih[i].bz);
+  return multi_ws_header(n_inst) + h * sizeof(float);
+}
+
 int launch_multi(bool adjoint, const dtv_instance* ih, const dtv_instance* id, int32_t n_inst,
                  const dtv_work_item* wh, const dtv_work_item* wd, int32_t n_work, int32_t block,
-                 int32_t* counter, int32_t persistent_blocks, hipStream_t st) {
+                 int32_t* counter, int32_t persistent_blocks, hipStream_t st,
+                 void* ws = nullptr, size_t ws_bytes = 0) {
+  const bool part = adjoint && ws;
   size_t lds = 0;
-  int rc = validate_multi(ih, n_inst, wh, n_work, block, adjoint, lds);
+  int rc = validate_multi(ih, n_inst, wh, n_work, block, adjoint, lds, part);
   if (rc) return rc;
+  if (part) {
+    if ((uintptr_t)ws % 256 || ws_bytes < multi_ws_bytes(ih, n_inst))
+      return DTV_ERR_INVALID_ARGUMENT;
+    if (lds > kMaxTileBytes) return DTV_ERR_INVALID_ARGUMENT;
+  }
   if (!id || !wd) return DTV_ERR_NULL_POINTER;
   if (!adjoint) {
     for (int i = 0; i < n_inst; ++i) {
@@ -1537,7 +1688,30 @@ int launch_multi(bool adjoint, const dtv_instance* ih, const dtv_instance* id, i
   for (int i = 1; i < n_inst; ++i)
     same = same && ih[i].bx == ih[0].bx && ih[i].by == ih[0].by && ih[i].bz == ih[0].bz;
   const int bx = ih[0].bx, by = ih[0].by, bz = ih[0].bz;
-  if (adjoint) {
+  if (part) {
+    long long* hoff = static_cast<long long*>(ws);
+    long long* foff = hoff + (n_inst + 1);
+    float* halo = reinterpret_cast<float*>(static_cast<char*>(ws) + multi_ws_header(n_inst));
+    hipLaunchKernelGGL(k_multi_offsets, dim3(1), dim3(256), 0, st, id, n_inst, hoff, foff);
+    if (hipGetLastError() != hipSuccess) return DTV_ERR_HIP;
+    auto kern = &k_adj_part_multi<0, 0, 0>;
+#define DTV_PICK(a, b, c) \
+    if (same && bx == a && by == b && bz == c) kern = &k_adj_part_multi<a, b, c>;
+    DTV_BRICK_SHAPES(DTV_PICK)
+#undef DTV_PICK
+    const dim3 grid(multi_grid(reinterpret_cast<const void*>(kern), block, lds, persistent,
+                               persistent_blocks, n_work));
+    hipLaunchKernelGGL(kern, grid, dim3(block), lds, st, id, wd, n_work, ctr, hoff, halo);
+    if (hipGetLastError() != hipSuccess) return DTV_ERR_HIP;
+    long long groups = 0;
+    for (int i = 0; i < n_inst; ++i)
+      groups += (face_lanes(ih[i].nx, ih[i].ny, ih[i].nz, ih[i].nbx, ih[i].nby, ih[i].nbz) + 255) /
+                256;
+    if (groups > 0x7fffffffll) return DTV_ERR_TOO_LARGE;
+    if (groups > 0)
+      hipLaunchKernelGGL(k_halo_faces_multi, dim3((unsigned)groups), dim3(256), 0, st, id, n_inst,
+                         hoff, foff, halo);
+  } else if (adjoint) {
     auto kern = &k_adj_brick_multi<0, 0, 0>;
 #define DTV_PICK(a, b, c) \
     if (same && bx == a && by == b && bz == c) kern = &k_adj_brick_multi<a, b, c>;
@@ -1577,6 +1751,31 @@ int dtv_cone_adjoint_multi(const dtv_instance* inst_host, const dtv_instance* in
                            int32_t* counter, int32_t persistent_blocks, void* stream) {
   return launch_multi(true, inst_host, inst_dev, n_inst, work_host, work_dev, n_work, block,
                       counter, persistent_blocks, (hipStream_t)stream);
+}
+
+int dtv_cone_adjoint_multi_ws(const dtv_instance* inst_host, const dtv_instance* inst_dev,
+                              int32_t n_inst, const dtv_work_item* work_host,
+                              const dtv_work_item* work_dev, int32_t n_work, int32_t block,
+                              int32_t* counter, int32_t persistent_blocks, void* workspace,
+                              size_t workspace_bytes, void* stream) {
+  if (!workspace) return DTV_ERR_NULL_POINTER;
+  return launch_multi(true, inst_host, inst_dev, n_inst, work_host, work_dev, n_work, block,
+                      counter, persistent_blocks, (hipStream_t)stream, workspace,
+                      workspace_bytes);
+}
+
+int dtv_cone_adjoint_multi_ws_bytes(const dtv_instance* inst_host, int32_t n_inst,
+                                    size_t* bytes) {
+  if (!inst_host || !bytes) return DTV_ERR_NULL_POINTER;
+  if (n_inst < 1 || n_inst > 65535) return DTV_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < n_inst; ++i) {
+    const dtv_instance& in = inst_host[i];
+    if (in.bx < 2 || in.by < 2 || in.bz < 2 || in.bx > 64 || in.by > 64 || in.bz > 64 ||
+        in.nbx < 1 || in.nby < 1 || in.nbz < 1 || in.view_parts < 0 || in.view_parts > 2)
+      return DTV_ERR_INVALID_ARGUMENT;
+  }
+  *bytes = multi_ws_bytes(inst_host, n_inst);
+  return DTV_OK;
 }
 
 }  // extern "C"

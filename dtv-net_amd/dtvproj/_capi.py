@@ -83,6 +83,10 @@ _SIGS = {
                                c_void_p, c_int32, c_void_p],
     "dtv_cone_adjoint_multi": [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_int32,
                                c_void_p, c_int32, c_void_p],
+    "dtv_cone_adjoint_multi_ws": [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int32,
+                                  c_int32, c_void_p, c_int32, c_void_p, ctypes.c_size_t,
+                                  c_void_p],
+    "dtv_cone_adjoint_multi_ws_bytes": [c_void_p, c_int32, ctypes.POINTER(ctypes.c_size_t)],
     "dtv_tv_fista_step": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32_p,
                           ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                           c_int32, c_int32, c_void_p],

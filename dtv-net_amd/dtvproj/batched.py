@@ -53,6 +53,7 @@ class _Bound:
         self.dev_work = torch.frombuffer(bytearray(host_work), dtype=torch.uint8).to(device)
         self.n_inst = len(host_inst)
         self.n_work = len(host_work)
+        self.ws = None  # partitioned A^T: halo workspace, allocated at bind time
 
 
 class MultiProjector:
@@ -71,10 +72,15 @@ class MultiProjector:
     over work items (forward at most 16, adjoint at most 2), a single int for
     every instance, or ``"auto"`` (default): bricks whose views would cost more
     than a sixth of a workgroup slot's share of the whole launch are split, so
-    the launch has enough similar-sized items to balance (:func:`auto_parts`)."""
+    the launch has enough similar-sized items to balance (:func:`auto_parts`).
+    ``partition`` (default True): A^T marches each sample in the one brick
+    holding its floor cell and adds the bricks' upper-halo sums in a second
+    kernel (``dtv_cone_adjoint_multi_ws``, equal to ``ops.adjoint`` per
+    instance for the same brick and split); False runs the unpartitioned
+    ``dtv_cone_adjoint_multi``."""
 
     def __init__(self, geoms, bricks=None, block=256, persistent_blocks="auto", device=None,
-                 view_parts="auto"):
+                 view_parts="auto", partition=True):
         self.geoms = list(geoms)
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.block = block
@@ -100,6 +106,7 @@ class MultiProjector:
         elif isinstance(view_parts, int):
             view_parts = [(view_parts, view_parts)] * len(self.geoms)
         self.view_parts = [(int(f), int(a)) for f, a in view_parts]
+        self.partition = bool(partition)
         self.counter = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.views = [g.device_views(self.device) for g in self.geoms]
 
@@ -145,17 +152,31 @@ class MultiProjector:
         return self._table(False, volumes, sinos, inits)
 
     def bind_adjoint(self, sinos, volumes):
-        return self._table(True, sinos, volumes, None)
+        b = self._table(True, sinos, volumes, None)
+        if self.partition:
+            nbytes = ctypes.c_size_t(0)
+            rc = _capi.lib().dtv_cone_adjoint_multi_ws_bytes(ctypes.addressof(b.host_inst),
+                                                             b.n_inst, ctypes.byref(nbytes))
+            _capi.check(rc, "dtv_cone_adjoint_multi_ws_bytes")
+            b.ws = torch.empty(nbytes.value, dtype=torch.uint8, device=self.device)
+        return b
 
     def run(self, bound, adjoint: bool):
         L = _capi.lib()
-        fn = L.dtv_cone_adjoint_multi if adjoint else L.dtv_cone_forward_multi
+        args = (ctypes.addressof(bound.host_inst), bound.dev_inst.data_ptr(), bound.n_inst,
+                ctypes.addressof(bound.host_work), bound.dev_work.data_ptr(), bound.n_work,
+                self.block, self.counter.data_ptr() if self.persistent_blocks else None,
+                self.persistent_blocks)
         with torch.cuda.device(self.device):
-            rc = fn(ctypes.addressof(bound.host_inst), bound.dev_inst.data_ptr(), bound.n_inst,
-                    ctypes.addressof(bound.host_work), bound.dev_work.data_ptr(), bound.n_work,
-                    self.block, self.counter.data_ptr() if self.persistent_blocks else None,
-                    self.persistent_blocks, ops._stream(self.device))
-        _capi.check(rc, "dtv_cone_adjoint_multi" if adjoint else "dtv_cone_forward_multi")
+            if adjoint and bound.ws is not None:
+                name = "dtv_cone_adjoint_multi_ws"
+                rc = L.dtv_cone_adjoint_multi_ws(*args, bound.ws.data_ptr(), bound.ws.numel(),
+                                                 ops._stream(self.device))
+            else:
+                name = "dtv_cone_adjoint_multi" if adjoint else "dtv_cone_forward_multi"
+                fn = L.dtv_cone_adjoint_multi if adjoint else L.dtv_cone_forward_multi
+                rc = fn(*args, ops._stream(self.device))
+        _capi.check(rc, name)
 
     def forward(self, volumes, inits=None):
         """Eager convenience: returns new sinograms (A v_i - init_i)."""

@@ -129,17 +129,19 @@ def adjoint_plan(geom: ConeGeometry, batch: int = 1, launch=None):
     return plan, int(nbytes.value)
 
 
-def adjoint(sino: torch.Tensor, geom: ConeGeometry, launch=None) -> torch.Tensor:
+def adjoint(sino: torch.Tensor, geom: ConeGeometry, launch=None,
+            partition: bool = True) -> torch.Tensor:
     """Matched A^T: exact transpose of :func:`forward`.
 
-    Runs ``dtv_cone_adjoint_ws`` with a workspace from torch's allocator:
-    each sample is marched by the one brick holding its floor cell, the
-    bricks' upper-halo sums (and the partial volumes of a split beyond two
-    parts) are added in a fixed order, so the result is bitwise reproducible
-    for any split.  A ``launch`` with ``view_parts`` 1 or 2 runs
-    ``dtv_cone_adjoint_ex`` instead: the unpartitioned bricks, which the
-    multi-instance launches share (two parts add into the output with
-    atomics)."""
+    ``partition`` (default): ``dtv_cone_adjoint_ws`` with a workspace from
+    torch's allocator.  Each sample is marched by the one brick holding its
+    floor cell, and the bricks' upper-halo sums (and the partial volumes of a
+    split beyond two parts) are added in a fixed order, so the result is
+    bitwise reproducible for any split.  ``partition=False``:
+    ``dtv_cone_adjoint_ex``, the unpartitioned bricks (each also marches the
+    samples that only touch its halo) that the reference-shaped entry points
+    use; their view split is at most two parts, which add into the output
+    with atomics."""
     _check_f32(sino, "sino")
     B = _batch_of(sino, _sino_numel(geom), "sino")
     out = torch.empty(geom.volume_shape(B), dtype=torch.float32, device=sino.device)
@@ -147,9 +149,8 @@ def adjoint(sino: torch.Tensor, geom: ConeGeometry, launch=None) -> torch.Tensor
     sc = _capi.scheme_struct(geom.scheme)
     lib = _capi.lib()
     L = launch
-    if isinstance(L, dict) and L.get("view_parts", 0) in (1, 2):
-        ws_bytes = 0
-    else:
+    ws_bytes = 0
+    if partition:
         plan, ws_bytes = adjoint_plan(geom, B, L)
         if ws_bytes:
             L = plan

@@ -250,6 +250,28 @@ int dtv_cone_adjoint_multi(const dtv_instance* inst_host, const dtv_instance* in
                            const dtv_work_item* work_dev, int32_t n_work, int32_t block,
                            int32_t* counter, int32_t persistent_blocks, void* stream);
 
+/* dtv_cone_adjoint_multi with the samples partitioned over the bricks (as
+ * dtv_cone_adjoint_ws): each sample is marched by the one brick holding its
+ * floor cell, each item stores its bricks' upper-halo sums in the workspace,
+ * and a second kernel adds them into the neighbouring bricks' voxels in one
+ * fixed order (bitwise reproducible; per instance equal to dtv_cone_adjoint_ws
+ * with the same brick and view parts).  The workspace (device memory, 256-byte
+ * aligned, dtv_cone_adjoint_multi_ws_bytes of the same host table) holds a
+ * header of per-instance offsets, which the call computes on the device from
+ * inst_dev (graph-capturable), and the halo sums.  No reference counterpart
+ * (see dtv_cone_adjoint_multi). */
+int dtv_cone_adjoint_multi_ws(const dtv_instance* inst_host, const dtv_instance* inst_dev,
+                              int32_t n_inst, const dtv_work_item* work_host,
+                              const dtv_work_item* work_dev, int32_t n_work, int32_t block,
+                              int32_t* counter, int32_t persistent_blocks, void* workspace,
+                              size_t workspace_bytes, void* stream);
+
+/* Workspace bytes of dtv_cone_adjoint_multi_ws for this instance table
+ * (256-byte header per 16 instances, then 4 x view parts x bricks x
+ * ((bx+1)(by+1)(bz+1) - bx by bz) per instance).  No GPU work. */
+int dtv_cone_adjoint_multi_ws_bytes(const dtv_instance* inst_host, int32_t n_inst,
+                                    size_t* bytes);
+
 /* ---- Fused reconstruction update ---------------------------------------- */
 
 /* One instance of the fused FISTA / smoothed-TV step (device pointers). */

@@ -284,3 +284,38 @@ def test_forward_plan_workspace():
     assert nbytes == 256 + 8 * 2 * 60 * 384 * 384
     plan, nbytes = ops.forward_plan(g, 1, dict(algorithm=1))
     assert plan.algorithm == 1 and nbytes == 0
+
+
+def test_multi_adjoint_workspace_bytes_and_checks():
+    """The partitioned multi-instance A^T's workspace: a 256-byte header per
+    16 instances (per-instance offsets, computed on the device), then each
+    instance's view parts x bricks x halo floats; the entry point refuses a
+    missing, small or misaligned workspace before any GPU work."""
+    from dtvproj.batched import Instance, WorkItem
+    host = (Instance * 2)()
+    shapes = [((45, 38, 29), (32, 16, 16), 2), ((20, 16, 12), (7, 5, 3), 1)]
+    want = 256
+    for e, (vs, b, parts) in zip(host, shapes):
+        e.nx, e.ny, e.nz = vs
+        e.nu, e.nv, e.n_views = 16, 16, 4
+        e.axis_mode = 3
+        e.bx, e.by, e.bz = b
+        e.nbx, e.nby, e.nbz = (-(-vs[0] // b[0]), -(-vs[1] // b[1]), -(-vs[2] // b[2]))
+        e.view_parts = parts
+        e.in_ = e.out = e.views = 16
+        halo = (b[0] + 1) * (b[1] + 1) * (b[2] + 1) - b[0] * b[1] * b[2]
+        want += 4 * parts * e.nbx * e.nby * e.nbz * halo
+    lib = _capi.lib()
+    n = ctypes.c_size_t(0)
+    assert lib.dtv_cone_adjoint_multi_ws_bytes(ctypes.addressof(host), 2, ctypes.byref(n)) == 0
+    assert n.value == want
+    assert lib.dtv_cone_adjoint_multi_ws_bytes(None, 2, ctypes.byref(n)) == 2
+    assert lib.dtv_cone_adjoint_multi_ws_bytes(ctypes.addressof(host), 0, ctypes.byref(n)) == 1
+    work = (WorkItem * 1)()
+    args = (ctypes.addressof(host), ctypes.c_void_p(256), 2, ctypes.addressof(work),
+            ctypes.c_void_p(256), 1, 256, None, 0)
+    assert lib.dtv_cone_adjoint_multi_ws(*args, None, 0, None) == 2
+    assert lib.dtv_cone_adjoint_multi_ws(*args, ctypes.c_void_p(256), want - 1, None) == 1
+    assert lib.dtv_cone_adjoint_multi_ws(*args, ctypes.c_void_p(128), want, None) == 1
+    host[0].view_parts = 3   # the multi-instance A^T splits a brick at most in two
+    assert lib.dtv_cone_adjoint_multi_ws_bytes(ctypes.addressof(host), 2, ctypes.byref(n)) == 1

@@ -26,12 +26,15 @@ def _small_uneven():
 BRICKS = [((32, 16, 16), (32, 16, 16)), ((16, 16, 8), (8, 8, 8)), ((7, 5, 3), (16, 16, 16))]
 
 
+@pytest.mark.parametrize("partition", [True, False])
 @pytest.mark.parametrize("persistent", [0, 37])
-def test_multi_matches_single_instance_calls(cuda, persistent):
+def test_multi_matches_single_instance_calls(cuda, persistent, partition):
+    """Each instance of a multi launch equals its single call with the same
+    brick shape (A to FP32 rounding; A^T, partitioned or not, bitwise)."""
     geoms = _small_uneven()
-    _case("multi_vs_single", persistent=persistent)
+    _case("multi_vs_single", persistent=persistent, partition=partition)
     mp = batched.MultiProjector(geoms, bricks=BRICKS, block=256, persistent_blocks=persistent,
-                                view_parts=1)
+                                view_parts=1, partition=partition)
     vols = [phantom.gaussian(g.volume_shape(1), 40 + i).to(cuda) for i, g in enumerate(geoms)]
     inits = [phantom.gaussian(g.sino_shape(1), 50 + i).to(cuda) for i, g in enumerate(geoms)]
     sinos = mp.forward(vols, inits)
@@ -43,12 +46,13 @@ def test_multi_matches_single_instance_calls(cuda, persistent):
     for i, g in enumerate(geoms):
         # one workgroup per brick, as in the multi-instance launch
         ref = ops.adjoint(ys[i], g, dict(algorithm=2, brick=BRICKS[i][1], block=256,
-                                         view_parts=1))
+                                         view_parts=1), partition=partition)
         assert torch.equal(outs[i], ref)   # integer accumulation: bitwise equal
 
 
+@pytest.mark.parametrize("partition", [True, False])
 @pytest.mark.parametrize("persistent", [0, 37])
-def test_multi_with_a_culled_many_view_instance(cuda, persistent):
+def test_multi_with_a_culled_many_view_instance(cuda, persistent, partition):
     """A 147-view Beijing instance (more than four chunks: each brick culls the
     views it cannot reach) beside a small circular one, in one launch and in
     persistent workgroups that take several (instance, brick) items in turn:
@@ -58,9 +62,10 @@ def test_multi_with_a_culled_many_view_instance(cuda, persistent):
     geoms = [G.beijing(P, (40, 36, 16), (8, 24)),
              G.circular(G.uniform_angles(11), (40, 40, 24), 1.0, (60, 30), 1.6, 200.0, 400.0)]
     bricks = [((16, 16, 8), (16, 16, 8)), ((24, 24, 14), (32, 16, 16))]
-    _case("multi_culled", persistent=persistent, views=[g.n_views for g in geoms])
+    _case("multi_culled", persistent=persistent, views=[g.n_views for g in geoms],
+          partition=partition)
     mp = batched.MultiProjector(geoms, bricks=bricks, block=256, persistent_blocks=persistent,
-                                view_parts=1)
+                                view_parts=1, partition=partition)
     vols = [phantom.gaussian(g.volume_shape(1), 70 + i).to(cuda) for i, g in enumerate(geoms)]
     sinos = mp.forward(vols)
     for i, g in enumerate(geoms):
@@ -70,7 +75,7 @@ def test_multi_with_a_culled_many_view_instance(cuda, persistent):
     outs = mp.adjoint(ys)
     for i, g in enumerate(geoms):
         ref = ops.adjoint(ys[i], g, dict(algorithm=2, brick=bricks[i][1], block=256,
-                                         view_parts=1))
+                                         view_parts=1), partition=partition)
         assert torch.equal(outs[i], ref)
 
 
@@ -86,8 +91,9 @@ def _oracle_pair(g):
     return (lambda x: cone.circ_forward(x, *args)), (lambda y: cone.circ_adjoint(y, *args))
 
 
+@pytest.mark.parametrize("partition", [True, False])
 @pytest.mark.parametrize("persistent", [0, 37])
-def test_multi_instance_kernels_vs_fp64_oracle(cuda, persistent):
+def test_multi_instance_kernels_vs_fp64_oracle(cuda, persistent, partition):
     """dtv_cone_forward_multi / dtv_cone_adjoint_multi on four uneven
     instances (three circular orbits of different size, voxel, pitch and view
     count, and a 147-view Beijing scan that the bricks cull), one workgroup
@@ -101,7 +107,7 @@ def test_multi_instance_kernels_vs_fp64_oracle(cuda, persistent):
     _case("multi vs oracle", persistent=persistent, views=[g.n_views for g in geoms])
     # bricks split over several items (A up to 5 parts, A^T 2) beside whole ones
     mp = batched.MultiProjector(geoms, bricks=bricks, block=256, persistent_blocks=persistent,
-                                view_parts=[(3, 2), (1, 1), (5, 2), (4, 2)])
+                                view_parts=[(3, 2), (1, 1), (5, 2), (4, 2)], partition=partition)
     vols = [phantom.gaussian(g.volume_shape(1), 140 + i) for i, g in enumerate(geoms)]
     inits = [phantom.gaussian(g.sino_shape(1), 150 + i) for i, g in enumerate(geoms)]
     ys = [phantom.gaussian(g.sino_shape(1), 160 + i) for i, g in enumerate(geoms)]

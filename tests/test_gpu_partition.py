@@ -67,7 +67,7 @@ def test_partitioned_adjoint_meets_oracle(cuda, brick):
     y = phantom.gaussian(g.sino_shape(2), 61)
     ref = cone.circ_adjoint(y, *args)
     yd = y.to(cuda)
-    classic = ops.adjoint(yd, g, _ws_launch(brick, 1))   # unpartitioned (no workspace)
+    classic = ops.adjoint(yd, g, _ws_launch(brick, 1), partition=False)
     assert rel_l2(classic, ref) <= TOL
     for p in (1, 2, 5, 16):
         _case("partitioned_adjoint", brick=brick, parts=p)
@@ -106,7 +106,8 @@ def test_partitioned_adjoint_standard_and_beijing(cuda):
     _case("standard_default")
     out = ops.adjoint(y.to(cuda), g)
     assert rel_l2(out, ref) <= TOL
-    assert rel_l2(out, ops.adjoint(y.to(cuda), g, _ws_launch((32, 16, 16), 1))) <= 2e-6
+    assert rel_l2(out, ops.adjoint(y.to(cuda), g, _ws_launch((32, 16, 16), 1),
+                                   partition=False)) <= 2e-6
     vs, ds = (40, 36, 16), (8, 24)
     P = G.beijing_default_vectors(n_angles=7, planes=21, sub_detector=ds, sid=70.0, sdd=114.0)
     gb = G.beijing(P, vs, ds)
@@ -129,7 +130,8 @@ def test_partitioned_adjoint_near_source(cuda):
         _case("near_source_partitioned", brick=brick)
         out = _adjoint_ws(y.to(cuda), g, brick, 3)
         assert rel_l2(out, ref) <= TOL
-        assert rel_l2(out, ops.adjoint(y.to(cuda), g, _ws_launch(brick, 1))) <= 5e-6
+        assert rel_l2(out, ops.adjoint(y.to(cuda), g, _ws_launch(brick, 1),
+                                       partition=False)) <= 5e-6
 
 
 def test_partitioned_adjoint_non_finite_values(cuda):
@@ -144,7 +146,7 @@ def test_partitioned_adjoint_non_finite_values(cuda):
     yd = y.to(cuda)
     for brick in ((16, 16, 8), (32, 16, 16)):
         _case("non_finite_partitioned", brick=brick)
-        classic = ops.adjoint(yd, g, _ws_launch(brick, 1)).cpu()
+        classic = ops.adjoint(yd, g, _ws_launch(brick, 1), partition=False).cpu()
         out = _adjoint_ws(yd, g, brick, 2).cpu()
         bad = ~torch.isfinite(classic)
         assert int(bad.sum()) > 0

@@ -46,16 +46,19 @@ def test_forward_parts_meet_oracle(cuda, brick):
         assert rel_l2(out, one) <= 1e-6
 
 
+@pytest.mark.parametrize("partition", [False, True])
 @pytest.mark.parametrize("brick", [(16, 16, 8), (32, 16, 16), (8, 8, 8)])
-def test_adjoint_two_parts_meet_oracle_and_repeat(cuda, brick):
+def test_adjoint_two_parts_meet_oracle_and_repeat(cuda, brick, partition):
+    """Two parts adding into the zero-filled volume, through the
+    unpartitioned and the partitioned bricks."""
     g, args = _circ()
     y = phantom.gaussian(g.sino_shape(2), 42)
     ref = cone.circ_adjoint(y, *args)
-    _case("adjoint_parts", brick=brick)
+    _case("adjoint_parts", brick=brick, partition=partition)
     L2 = dict(algorithm=2, brick=brick, block=256, view_parts=2)
-    one = ops.adjoint(y.to(cuda), g, dict(L2, view_parts=1))
-    out = ops.adjoint(y.to(cuda), g, L2)
-    again = ops.adjoint(y.to(cuda), g, L2)
+    one = ops.adjoint(y.to(cuda), g, dict(L2, view_parts=1), partition=partition)
+    out = ops.adjoint(y.to(cuda), g, L2, partition=partition)
+    again = ops.adjoint(y.to(cuda), g, L2, partition=partition)
     assert rel_l2(out, ref) <= TOL
     assert rel_l2(one, ref) <= TOL
     assert rel_l2(out, one) <= 2e-6

@@ -136,9 +136,10 @@ def test_unroll128_batch4_vs_oracle_fixture(cuda, wfx):
     vox = torch.from_numpy(wfx["u128_voxels"].astype(np.int64))
     flat = ((vox[:, 2] * Y + vox[:, 1]) * X + vox[:, 0]).to(cuda)
     aref = torch.from_numpy(wfx["u128_adjoint"])
-    for launch in (None, dict(algorithm=2, brick=(16, 16, 16), block=256, view_parts=2)):
-        vol = ops.adjoint(yb, g, launch)
-        assert rel_l2(vol[2].reshape(-1)[flat], aref) <= 1e-5, f"launch={launch}"
+    two = dict(algorithm=2, brick=(16, 16, 16), block=256, view_parts=2)
+    for launch, part in ((None, True), (two, True), (two, False)):
+        vol = ops.adjoint(yb, g, launch, partition=part)
+        assert rel_l2(vol[2].reshape(-1)[flat], aref) <= 1e-5, f"launch={launch} {part}"
 
 
 def test_uneven16_batched_fista_tv(cuda, wfx):
