@@ -853,9 +853,11 @@ __device__ __forceinline__ void flush_halo(int* iacc, float* hacc, bool fixed, f
 // every sample is marched by exactly one brick.  Their taps reach the upper
 // halo (local coordinate b on some axis), whose sums a group flush adds into
 // FP32 accumulators hacc in LDS (after the tile) and which end up in hout for
-// the neighbours; the tile has no row or plane below the brick (taps of
-// floor -1 samples on y or z are dropped by the guarded loop; on x they land
-// in a pad word).
+// the neighbours.  The weight bound staged for the box [floor range, o+e]
+// covers those voxels too: they collect only the brick's own samples.  The
+// tile has no column, row or plane below the brick: taps of floor -1 samples
+// on x land in the previous row's pad word (or the leading one), on y or z
+// the guarded loop drops them.
 template <int CBX, int CBY, int CBZ, bool kPart = false>
 __device__ __forceinline__ void adj_brick_body(const float* __restrict__ sbase,
                                                float* __restrict__ vb, const Geo& g, int bx_,
