@@ -254,9 +254,9 @@ __device__ __forceinline__ double fix_ray_scale(unsigned absmax_bits, const Geo&
 // samples in the brick exactly (exact_window), so its sample loop carries no
 // membership test.
 
-constexpr int kSchedR = 8;       // candidate rays per lane per segment
+constexpr int kSchedR = 8;       // candidate rays per lane per segment (256-lane workgroups)
 constexpr int kSchedBins = 64;   // chord classes, one per sample (the last open-ended)
-constexpr int kSchedCap = kSchedR * 256;
+constexpr int kSchedCap = kSchedR * 256;  // also 4 per lane of a 512-lane workgroup
 
 struct RaySched {
   unsigned hist[kSchedBins];     // hits per class; cleared by the scan
@@ -336,7 +336,8 @@ __device__ __forceinline__ unsigned sched_hit(RaySched& rs, float chord) {
 
 // Counting sort of the segment's hits (slot[j] = class << 16 | rank, or ~0)
 // into rs.list, longest class first; rs.n = number of hits.
-__device__ __forceinline__ void sched_sort(RaySched& rs, const unsigned (&slot)[kSchedR]) {
+template <int kR>
+__device__ __forceinline__ void sched_sort(RaySched& rs, const unsigned (&slot)[kR]) {
   __syncthreads();
   if (threadIdx.x < kSchedBins) {
     const int l = threadIdx.x;
@@ -354,7 +355,7 @@ __device__ __forceinline__ void sched_sort(RaySched& rs, const unsigned (&slot)[
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kSchedR; ++j)
+  for (int j = 0; j < kR; ++j)
     if (slot[j] != ~0u)
       rs.list[rs.offs[slot[j] >> 16] + (slot[j] & 0xffffu)] =
           (unsigned short)(threadIdx.x + j * blockDim.x);
@@ -469,7 +470,7 @@ __device__ __forceinline__ void stage_fp64_view(const Geo& g, int a, int slot,
 // CBX/CBY/CBZ: the brick shape when known at compile time (0 = take the
 // run-time value); the instantiated shapes turn every index computation into
 // shifts and immediates.
-template <int CBX, int CBY, int CBZ>
+template <int CBX, int CBY, int CBZ, int kR = kSchedR>
 __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
                                                float* __restrict__ sbase, const Geo& g,
                                                int bx_, int by_, int bz_, int nbx, int nby,
@@ -527,10 +528,10 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
 #pragma unroll
       for (int k = 0; k < 12; ++k) fv[k] = uni(st.fview[vi][k]);
       float* srow = sbase + (size_t)a * g.nv * g.nu;
-      for (int seg = 0; seg < rm.n; seg += kSchedR * blockDim.x) {
-        unsigned slot[kSchedR];
+      for (int seg = 0; seg < rm.n; seg += kR * blockDim.x) {
+        unsigned slot[kR];
 #pragma unroll
-        for (int j = 0; j < kSchedR; ++j) {
+        for (int j = 0; j < kR; ++j) {
           // one candidate at a time: interleaving the unrolled candidates would
           // only raise register pressure
           asm volatile("" ::: "memory");
@@ -593,8 +594,14 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
 // fix != nullptr: the per-(ray, brick) sums go into int64 fixed point at the
 // scale fix_ray_scale derives from hdr (max |volume|), unless the volume holds
 // a non-finite value, in which case they go into sino as FP32 atomics.
+// Workgroups of up to kFwdBlock lanes, at least six waves per SIMD (80
+// VGPRs): a tile's LDS lets three workgroups share a CU, so 512-lane ones keep
+// twice the waves of 256-lane ones in flight (the spills this costs are all
+// outside the sample loop); pass 1 then takes 4 candidates per lane, so one
+// segment still fills the schedule list.
+constexpr int kFwdBlock = 512;
 template <int CBX, int CBY, int CBZ>
-__global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol,
+__global__ __launch_bounds__(kFwdBlock, 6) void k_fwd_brick(const float* __restrict__ vol,
                                                    float* __restrict__ sino,
                                                    long long* __restrict__ fix,
                                                    const unsigned* __restrict__ hdr, Geo g,
@@ -614,9 +621,9 @@ __global__ __launch_bounds__(256) void k_fwd_brick(const float* __restrict__ vol
       fix = nullptr;
     }
   }
-  fwd_brick_body<CBX, CBY, CBZ>(vol + (size_t)b * g.nx * g.ny * g.nz,
-                 sino + (size_t)b * g.n_views * g.nv * g.nu, g, bx, by, bz, nbx, nby,
-                 blockIdx.x, blockIdx.z, gridDim.z, tile, st, rs, fix, qs);
+  fwd_brick_body<CBX, CBY, CBZ, kSchedCap / kFwdBlock>(
+      vol + (size_t)b * g.nx * g.ny * g.nz, sino + (size_t)b * g.n_views * g.nv * g.nu, g, bx,
+      by, bz, nbx, nby, blockIdx.x, blockIdx.z, gridDim.z, tile, st, rs, fix, qs);
 }
 
 // max |v| over the volume as FP32 bits (non-negative floats order like their
@@ -1262,8 +1269,9 @@ __device__ __forceinline__ int next_item(int* counter, int* slot, int done) {
   return *slot;
 }
 
+// Workgroups of up to kFwdBlock lanes, as k_fwd_brick.
 template <int CBX, int CBY, int CBZ>
-__global__ __launch_bounds__(256) void k_fwd_brick_multi(const dtv_instance* __restrict__ inst,
+__global__ __launch_bounds__(kFwdBlock, 6) void k_fwd_brick_multi(const dtv_instance* __restrict__ inst,
                                                          const dtv_work_item* __restrict__ work,
                                                          int n_work, int* counter) {
   extern __shared__ float tile[];
@@ -1278,9 +1286,9 @@ __global__ __launch_bounds__(256) void k_fwd_brick_multi(const dtv_instance* __r
     const dtv_instance in = inst[it.instance];
     const Geo g = instance_geo(in);
     const int nb = in.nbx * in.nby * in.nbz;
-    fwd_brick_body<CBX, CBY, CBZ>(in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby,
-                                  it.brick % nb, it.brick / nb, max(1, in.view_parts), tile, st,
-                                  rs);
+    fwd_brick_body<CBX, CBY, CBZ, kSchedCap / kFwdBlock>(
+        in.in, in.out, g, in.bx, in.by, in.bz, in.nbx, in.nby, it.brick % nb, it.brick / nb,
+        max(1, in.view_parts), tile, st, rs);
   }
 }
 
@@ -1465,6 +1473,7 @@ namespace dtv {
 
 int launch_fwd_brick(const float* vol, float* sino, int batch, const Geo& g, int bx, int by,
                      int bz, int block, int parts, hipStream_t st, void* fix_ws) {
+  if (block > kFwdBlock) return 1;
   const int nbx = (g.nx + 1 + bx - 1) / bx, nby = (g.ny + 1 + by - 1) / by,
             nbz = (g.nz + 1 + bz - 1) / bz;
   const size_t lds = fwd_tile_bytes(bx, by, bz);
@@ -1569,7 +1578,9 @@ int validate_multi(const dtv_instance* ih, int32_t n_inst, const dtv_work_item* 
                    bool part = false) {
   if (!ih || !wh) return DTV_ERR_NULL_POINTER;
   if (n_inst < 1 || n_work < 1 || n_inst > 65535) return DTV_ERR_INVALID_ARGUMENT;
-  if (block < 64 || block > 256 || block % 64) return DTV_ERR_INVALID_ARGUMENT;
+  // A's multi kernel takes workgroups of up to 512 lanes, A^T's up to 256
+  if (block < 64 || block > (adjoint ? 256 : kFwdBlock) || block % 64)
+    return DTV_ERR_INVALID_ARGUMENT;
   lds = 0;
   size_t max_out = 0;
   for (int i = 0; i < n_inst; ++i) {

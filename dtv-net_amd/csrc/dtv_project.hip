@@ -554,6 +554,7 @@ Launch default_launch(bool adjoint, const Geo& g, int batch, bool ws = false) {
     L.algo = DTV_ALGO_RAY;
     return L;
   }
+  L.block = 512;
   L.parts = default_parts(false, g, batch, L.bx, L.by, L.bz);
   return L;
 }
@@ -579,7 +580,9 @@ int resolve_launch(const dtv_launch* l, Launch& out, bool adjoint, const Geo& g,
     if (l->algorithm == DTV_ALGO_DEFAULT) out.algo = DTV_ALGO_BRICK;
   }
   if (l->block) {
-    if (l->block < 64 || l->block > 256 || l->block % 64) return DTV_ERR_INVALID_ARGUMENT;
+    // A's brick kernel takes workgroups of up to 512 lanes, A^T's up to 256
+    if (l->block < 64 || l->block > (adjoint ? 256 : 512) || l->block % 64)
+      return DTV_ERR_INVALID_ARGUMENT;
     out.block = l->block;
   }
   if (l->view_parts) {

@@ -19,7 +19,11 @@ import torch
 from . import ops
 from .geometry import ConeGeometry
 
-# (bx, by, bz, lanes); the adjoint needs bx*by*bz <= 32 * lanes
+# (bx, by, bz, lanes); the adjoint needs bx*by*bz <= 32 * lanes.  A is timed
+# on each shape once, at FORWARD_LANES: its brick kernel is built for
+# 512-lane workgroups (six waves per SIMD), smaller ones run it below its
+# register budget's occupancy.
+FORWARD_LANES = 512
 CANDIDATES = [
     (32, 16, 16, 256), (26, 26, 14, 256), (24, 24, 14, 256), (32, 32, 8, 256), (16, 16, 32, 256),
     (16, 16, 16, 256),
@@ -93,11 +97,14 @@ def tune(geom: ConeGeometry, device=None, reps: int = 3, candidates=CANDIDATES,
     x = torch.rand(geom.volume_shape(1), generator=gen).to(device)
     y = torch.rand(geom.sino_shape(1), generator=gen).to(device)
     times = {"forward": {}, "adjoint": {}}
+    shapes = set()
     for c in candidates:
-        for p in FORWARD_PARTS:
-            cf = tuple(c[:4]) + (p,)
-            L = launch_of(cf)
-            times["forward"][str(cf)] = _time(lambda: ops.forward(x, geom, L), reps)
+        if tuple(c[:3]) not in shapes:
+            shapes.add(tuple(c[:3]))
+            for p in FORWARD_PARTS:
+                cf = tuple(c[:3]) + (FORWARD_LANES, p)
+                L = launch_of(cf)
+                times["forward"][str(cf)] = _time(lambda: ops.forward(x, geom, L), reps)
         # the adjoint's lanes own at most 32 voxels each
         if c[0] * c[1] * c[2] <= 32 * c[3]:
             for p in ADJOINT_PARTS:
@@ -106,7 +113,7 @@ def tune(geom: ConeGeometry, device=None, reps: int = 3, candidates=CANDIDATES,
                 times["adjoint"][str(ca)] = _time(lambda: ops.adjoint(y, geom, L), reps)
         if verbose:
             print(f"tune {c}: " + " ".join(f"{k} {v:.3f}" for d in times.values()
-                                           for k, v in d.items() if k.startswith(str(c[:4])[:-1])),
+                                           for k, v in d.items() if k.startswith(str(c[:3])[:-1])),
                   flush=True)
     best_f = min(times["forward"], key=times["forward"].get)
     best_a = min(times["adjoint"], key=times["adjoint"].get)

@@ -80,10 +80,11 @@ class MultiProjector:
     ``dtv_cone_adjoint_multi``."""
 
     def __init__(self, geoms, bricks=None, block=256, persistent_blocks="auto", device=None,
-                 view_parts="auto", partition=True):
+                 view_parts="auto", partition=True, forward_block=512):
         self.geoms = list(geoms)
         self.device = device or torch.device("cuda", torch.cuda.current_device())
-        self.block = block
+        self.block = block                  # A^T lanes per workgroup (at most 256)
+        self.forward_block = forward_block  # A lanes per workgroup (at most 512)
         if persistent_blocks == "auto":
             # the library caps a persistent grid at the workgroups the device
             # holds at once for the kernel it launches (LDS, registers)
@@ -165,7 +166,8 @@ class MultiProjector:
         L = _capi.lib()
         args = (ctypes.addressof(bound.host_inst), bound.dev_inst.data_ptr(), bound.n_inst,
                 ctypes.addressof(bound.host_work), bound.dev_work.data_ptr(), bound.n_work,
-                self.block, self.counter.data_ptr() if self.persistent_blocks else None,
+                self.block if adjoint else self.forward_block,
+                self.counter.data_ptr() if self.persistent_blocks else None,
                 self.persistent_blocks)
         with torch.cuda.device(self.device):
             if adjoint and bound.ws is not None:
