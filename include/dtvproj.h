@@ -46,7 +46,8 @@ typedef struct dtv_scheme {
 typedef struct dtv_launch {
   int32_t algorithm;   /* 0: default, 1: ray-driven (gather / atomic scatter), 2: brick (LDS) */
   int32_t brick[3];    /* brick edge in voxels (x, y, z) for algorithm 2 */
-  int32_t block;       /* threads per workgroup for algorithm 2 (64..256) */
+  int32_t block;       /* threads per workgroup for algorithm 2, a multiple of 64: A 64..512
+                          (default 512), A^T 64..256 (default 256) */
   int32_t view_parts;  /* algorithm 2: workgroups sharing one brick's views (0 = default;
                           A: 1..16; A^T: 1 or 2, with the output zero-filled and added to,
                           so it must be coarse-grained device memory) */
@@ -239,7 +240,8 @@ typedef struct dtv_work_item {
  * counter != NULL (a device int) and persistent_blocks > 0, that many
  * workgroups (at most as many as the device holds at once for the kernel)
  * pull items from a queue (dynamic balance); otherwise one workgroup per
- * item, in list order.  No reference counterpart: the
+ * item, in list order.  block: lanes per workgroup, a multiple of 64, A up
+ * to 512, A^T up to 256.  No reference counterpart: the
  * reference launches per view and per batch item (forwardKernel.cu:115-141). */
 int dtv_cone_forward_multi(const dtv_instance* inst_host, const dtv_instance* inst_dev,
                            int32_t n_inst, const dtv_work_item* work_host,
