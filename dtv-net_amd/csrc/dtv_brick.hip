@@ -649,7 +649,6 @@ __global__ void k_fix_to_float(const long long* __restrict__ fix, float* __restr
     sino[i] = (float)((double)fix[i] * inv);
 }
 
-constexpr int kAdjPerThread = 32;  // owned voxels per lane (bx*by*bz <= 32*block)
 
 // Nearest integer, halves rounded up (v_cvt_rpi_i32_f32: one instruction
 // where the compiler would emit a round and a convert); deterministic, which

@@ -62,6 +62,8 @@ inline size_t adj_ptile_bytes(int bx, int by, int bz) {
   return (1 + (size_t)adj_ptile_stz(bx, by) * (bz + 1) + adj_halo_count(bx, by, bz)) * 4;
 }
 constexpr size_t kMaxTileBytes = 56 * 1024;
+// A^T brick kernels: owned voxels per lane (bx*by*bz <= kAdjPerThread * lanes)
+constexpr int kAdjPerThread = 32;
 
 // DTV_OK when p is coarse-grained device memory (hipMalloc), which the
 // kernels' hardware FP32 global atomics need; DTV_ERR_INVALID_ARGUMENT for

@@ -640,7 +640,7 @@ int run_adjoint(const float* sino, float* vol, int32_t batch, const Geo& g, hipS
 // Whether the workspace path partitions the samples over the bricks (its
 // tile, with the upper-halo accumulators, fits the LDS limit).
 bool adj_partitioned(const Launch& L) {
-  return L.algo == DTV_ALGO_BRICK && L.bx * L.by * L.bz <= 32 * L.block &&
+  return L.algo == DTV_ALGO_BRICK && L.bx * L.by * L.bz <= kAdjPerThread * L.block &&
          adj_ptile_bytes(L.bx, L.by, L.bz) <= kMaxTileBytes;
 }
 
