@@ -454,49 +454,106 @@ def circ_adjoint_at(sino, angles, volume_size, voxel_mm, detector_size, pitch_mm
         u1 = (torch.ceil(iu.max(1).values).to(torch.int64) + 1).clamp(max=U - 1)
         v0 = (torch.floor(iv.min(1).values).to(torch.int64) - 1).clamp(min=0)
         v1 = (torch.ceil(iv.max(1).values).to(torch.int64) + 1).clamp(max=V - 1)
-        wu = int((u1 - u0 + 1).clamp(min=0).max())
-        wv = int((v1 - v0 + 1).clamp(min=0).max())
-        if wu <= 0 or wv <= 0:
-            continue
-        ou = torch.arange(wu).view(1, 1, wu)
-        ov = torch.arange(wv).view(1, wv, 1)
-        uu = u0.view(K, 1, 1) + ou
-        vv = v0.view(K, 1, 1) + ov
-        ok = (uu <= u1.view(K, 1, 1)) & (vv <= v1.view(K, 1, 1))
-        kk = torch.arange(K).view(K, 1, 1).expand(K, wv, wu)[ok]
-        r = (j * V + vv.expand(K, wv, wu)[ok]) * U + uu.expand(K, wv, wu)[ok]
-        src, d = rays.src[r], rays.d[r]
-        a_start, a_end = rays.a_start[r], rays.a_end[r]
-        v = vf[kk]
-        lo = torch.full_like(a_start, -math.inf)
-        hi = torch.full_like(a_start, math.inf)
-        for ax in range(3):
-            dd = d[:, ax]
-            nz = dd != 0
-            safe = torch.where(nz, dd, torch.ones_like(dd))
-            e0 = ((v[:, ax] - 1 - rays.off[ax]) * rays.div_s - src[:, ax]) / safe
-            e1 = ((v[:, ax] + 1 - rays.off[ax]) * rays.div_s - src[:, ax]) / safe
-            lo = torch.where(nz, torch.maximum(lo, torch.minimum(e0, e1)), lo)
-            hi = torch.where(nz, torch.minimum(hi, torch.maximum(e0, e1)), hi)
-        live = hi > lo
-        if not bool(live.any()):
-            continue
-        kk, r, src, d, a_start, a_end, v = (t[live] for t in (kk, r, src, d, a_start, a_end, v))
-        k_lo = torch.floor((lo[live] - a_start) / rays.step) - 1
-        k_hi = torch.ceil((hi[live] - a_start) / rays.step) + 1
-        span = int((k_hi - k_lo + 1).max())
-        if span > 64:
-            raise ValueError("unexpectedly long sample run at one voxel")
-        acc = torch.zeros(kk.shape[0], dtype=dtype)
-        for t in range(span):
-            k = k_lo + t
-            alpha = a_start + k * rays.step
-            p = (src + alpha[:, None] * d) / rays.div_s + rays.off
-            w = torch.clamp(1 - (p - v).abs(), min=0).prod(1)
-            keep = (k >= 0) & (k <= k_hi) & (alpha < a_end)
-            acc += torch.where(keep, w, torch.zeros_like(w))
-        for b in range(B):
-            out[b].index_add_(0, kk, acc * sflat[b][r])
+        _adjoint_at_candidates(sflat, rays, vf, j, U, V, u0, u1, v0, v1, out)
+    return out * rays.scale
+
+
+def _adjoint_at_candidates(sflat, rays, vf, j, U, V, u0, u1, v0, v1, out):
+    """Adds into out [B, K] the adjoint contributions of view j's candidate
+    rays, pixels [u0, u1] x [v0, v1] per voxel (of vf [K, 3]): along each ray
+    only the k whose alpha lies in the three slabs |p_ax - v_ax| < 1 are
+    visited, each adding the trilinear weight times the ray's value (the
+    enumeration of ``_march``)."""
+    K = vf.shape[0]
+    B = sflat.shape[0]
+    dtype = vf.dtype
+    wu = int((u1 - u0 + 1).clamp(min=0).max())
+    wv = int((v1 - v0 + 1).clamp(min=0).max())
+    if wu <= 0 or wv <= 0:
+        return
+    ou = torch.arange(wu).view(1, 1, wu)
+    ov = torch.arange(wv).view(1, wv, 1)
+    uu = u0.view(K, 1, 1) + ou
+    vv = v0.view(K, 1, 1) + ov
+    ok = (uu <= u1.view(K, 1, 1)) & (vv <= v1.view(K, 1, 1))
+    kk = torch.arange(K).view(K, 1, 1).expand(K, wv, wu)[ok]
+    r = (j * V + vv.expand(K, wv, wu)[ok]) * U + uu.expand(K, wv, wu)[ok]
+    src, d = rays.src[r], rays.d[r]
+    a_start, a_end = rays.a_start[r], rays.a_end[r]
+    v = vf[kk]
+    lo = torch.full_like(a_start, -math.inf)
+    hi = torch.full_like(a_start, math.inf)
+    for ax in range(3):
+        dd = d[:, ax]
+        nz = dd != 0
+        safe = torch.where(nz, dd, torch.ones_like(dd))
+        e0 = ((v[:, ax] - 1 - rays.off[ax]) * rays.div_s - src[:, ax]) / safe
+        e1 = ((v[:, ax] + 1 - rays.off[ax]) * rays.div_s - src[:, ax]) / safe
+        lo = torch.where(nz, torch.maximum(lo, torch.minimum(e0, e1)), lo)
+        hi = torch.where(nz, torch.minimum(hi, torch.maximum(e0, e1)), hi)
+    live = hi > lo
+    if not bool(live.any()):
+        return
+    kk, r, src, d, a_start, a_end, v = (t[live] for t in (kk, r, src, d, a_start, a_end, v))
+    k_lo = torch.floor((lo[live] - a_start) / rays.step) - 1
+    k_hi = torch.ceil((hi[live] - a_start) / rays.step) + 1
+    span = int((k_hi - k_lo + 1).max())
+    if span > 64:
+        raise ValueError("unexpectedly long sample run at one voxel")
+    acc = torch.zeros(kk.shape[0], dtype=dtype)
+    for t in range(span):
+        k = k_lo + t
+        alpha = a_start + k * rays.step
+        p = (src + alpha[:, None] * d) / rays.div_s + rays.off
+        w = torch.clamp(1 - (p - v).abs(), min=0).prod(1)
+        keep = (k >= 0) & (k <= k_hi) & (alpha < a_end)
+        acc += torch.where(keep, w, torch.zeros_like(w))
+    for b in range(B):
+        out[b].index_add_(0, kk, acc * sflat[b][r])
+
+
+def table_adjoint_at(sino, views, volume_size, detector_size, axis_mode, box_lo, box_hi,
+                     margin, scale, voxels, dtype=F64):
+    """``table_adjoint`` (canonical view table [n, 12] = S, D0, U, V in the
+    voxel frame; the Standard, circular and Beijing schemes all reduce to
+    it) evaluated at the voxels [K, 3] of (ix, iy, iz) only; returns [B, K].
+    Per view the rays that can reach a voxel are the pixels whose centres
+    fall in the bounding rectangle, widened by one pixel, of the perspective
+    images of its support cube's 8 corners (a convex box in front of the
+    source maps inside the hull of its corners' images); then the same
+    per-candidate enumeration as ``circ_adjoint_at``."""
+    U, V = (int(v) for v in detector_size)
+    rays = table_rays(views, detector_size, axis_mode, box_lo, box_hi, margin, scale, dtype)
+    T = torch.as_tensor(np.asarray(views, dtype=np.float64)).to(dtype)
+    vox = torch.as_tensor(np.asarray(voxels), dtype=torch.int64).view(-1, 3)
+    K = vox.shape[0]
+    B = sino.shape[0]
+    sflat = sino.to(dtype).reshape(B, -1)
+    out = torch.zeros(B, K, dtype=dtype)
+    vf = vox.to(dtype)
+    corners = torch.stack([vf + torch.tensor(c, dtype=dtype)
+                           for c in ((a, b, e) for a in (-1, 1) for b in (-1, 1)
+                                     for e in (-1, 1))], 1)          # [K, 8, 3] voxel frame
+    for j in range(T.shape[0]):
+        S, D0, Uv, Vv = T[j, 0:3], T[j, 3:6], T[j, 6:9], T[j, 9:12]
+        n = torch.linalg.cross(Uv, Vv)
+        h = torch.dot(D0 - S, n)          # the detector plane is (x - S).n = h
+        w = corners - S                   # [K, 8, 3]
+        wn = (w * n).sum(-1)
+        if not bool((wn * h > 0).all()):
+            raise ValueError("a voxel's support reaches behind the source")
+        P = S + w * (h / wn).unsqueeze(-1) - D0   # on the detector plane, from D0
+        # P = iu U + iv V: solve the 2x2 normal equations
+        uu, uv_, vv_ = torch.dot(Uv, Uv), torch.dot(Uv, Vv), torch.dot(Vv, Vv)
+        det = uu * vv_ - uv_ * uv_
+        pu, pv = (P * Uv).sum(-1), (P * Vv).sum(-1)
+        iu = (vv_ * pu - uv_ * pv) / det
+        iv = (uu * pv - uv_ * pu) / det
+        u0 = (torch.floor(iu.min(1).values).to(torch.int64) - 1).clamp(min=0)
+        u1 = (torch.ceil(iu.max(1).values).to(torch.int64) + 1).clamp(max=U - 1)
+        v0 = (torch.floor(iv.min(1).values).to(torch.int64) - 1).clamp(min=0)
+        v1 = (torch.ceil(iv.max(1).values).to(torch.int64) + 1).clamp(max=V - 1)
+        _adjoint_at_candidates(sflat, rays, vf, j, U, V, u0, u1, v0, v1, out)
     return out * rays.scale
 
 

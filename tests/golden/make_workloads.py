@@ -34,9 +34,16 @@ full-size workloads, sampled at seeded voxels so the file stays small.
   at 2048 seeded rays and A^T of the N(0,1) sinogram of seed 31 at 2048
   seeded voxels (1792 uniform, 256 on the faces).
 
+* ``bj_*`` -- the 3024-view Beijing scan of the benchmark's loop (256x256x64,
+  144 angles x 21 planes of 80x128 sub-detectors, SID 708 / SDD 1143, pitch
+  from ``dtvproj.geometry.beijing_default_vectors``): the matched A^T of the
+  N(0,1) sinogram of seed 41 at 1024 seeded voxels (896 uniform, 128 on the
+  faces), evaluated by ``oracle.cone.table_adjoint_at`` over all 3024 views,
+  with the sinogram's FP64 sum and sum of squares.
+
 Takes several minutes on 8 cores.  Run from the repository root:
     python tests/golden/make_workloads.py          # everything
-    python tests/golden/make_workloads.py l512 u128 s256f   # (re)write only these arrays
+    python tests/golden/make_workloads.py l512 u128 s256f bj   # (re)write only these arrays
 """
 import math
 import os
@@ -68,6 +75,40 @@ def angles(n):
 L512 = dict(vol=(512, 512, 384), voxel=0.4, det=(768, 576), pitch=0.6, n_views=120,
             sod=700.0, sdd=1200.0, vol_seed=20, sino_seed=21, n_rays=4096, n_uniform=3584,
             n_face=512, ray_seed=2023, vox_seed=2024)
+
+
+BJ = dict(vol=(256, 256, 64), det=(80, 128), angles=144, planes=21, sid=708.0, sdd=1143.0,
+          sino_seed=41, n_uniform=896, n_face=128, vox_seed=2041)
+
+
+def bj_geometry():
+    """The Beijing scan of the benchmark's loop, built as the bench builds it."""
+    from dtvproj import geometry as G
+    c = BJ
+    X, Y, Z = c["vol"]
+    U, V = c["det"]
+    mag = c["sdd"] / c["sid"]
+    pitch = (1.2 * np.hypot(X, Y) * mag / (U * c["planes"]), 1.2 * Z * mag / V)
+    vec = G.beijing_default_vectors(c["angles"], c["planes"], (U, V), c["sid"], c["sdd"], pitch)
+    return G.beijing(vec, (X, Y, Z), (U, V))
+
+
+def bj_voxels():
+    c = BJ
+    return face_and_uniform_voxels(c["vol"], c["n_uniform"], c["n_face"], c["vox_seed"])
+
+
+def build_bj(log=print):
+    c = BJ
+    g = bj_geometry()
+    y = phantom.gaussian(g.sino_shape(1), c["sino_seed"]).double()
+    vox = bj_voxels()
+    sc = g.scheme
+    adj = cone.table_adjoint_at(y, g.views, g.volume_size, g.detector_size, sc.axis_mode,
+                                sc.box_lo, sc.box_hi, sc.margin, sc.scale, vox)
+    log(f"bj: {g.n_views} views, {vox.shape[0]} voxels")
+    return dict(bj_voxels=vox, bj_adjoint=adj[0].numpy(), bj_sino_sum=np.float64(y.sum()),
+                bj_sino_sumsq=np.float64((y * y).sum()))
 
 
 def face_and_uniform_voxels(vol, n_uniform, n_face, seed):
@@ -239,7 +280,7 @@ def build_u16(log=print):
 if __name__ == "__main__":
     torch.set_num_threads(os.cpu_count() or 1)
     log = lambda s: print(s, flush=True)  # noqa: E731
-    parts = dict(l512=build_l512, u128=build_u128, s256f=build_s256f)
+    parts = dict(l512=build_l512, u128=build_u128, s256f=build_s256f, bj=build_bj)
     if sys.argv[1:]:
         with np.load(OUT) as f:
             fx = {k: f[k] for k in f.files}
@@ -252,5 +293,6 @@ if __name__ == "__main__":
         fx.update(build_l512(log))
         fx.update(build_u128(log))
         fx.update(build_s256f(log))
+        fx.update(build_bj(log))
     np.savez_compressed(OUT, **fx)
     print(f"wrote {OUT}: {len(fx)} arrays, {os.path.getsize(OUT)} bytes")

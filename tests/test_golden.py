@@ -231,3 +231,25 @@ def test_workload_fixture_sparse256_forward_reproduces_on_a_subset():
     got = cone.circ_forward_at(x, m.angles(c["n_views"]), c["vol"], c["voxel"], c["det"],
                                c["pitch"], c["sod"], c["sdd"], rays[pick])[0].numpy()
     assert np.abs(got - ref[pick]).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.slow
+def test_workload_fixture_beijing_reproduces_on_a_subset():
+    """The oracle still gives the stored 3024-view Beijing A^T at 32 of the
+    stored voxels (the sinogram regenerated from its seed, checked by sum)."""
+    from oracle import cone
+    from dtvproj import phantom
+    m = _wgen()
+    with np.load(WFIX) as f:
+        fx = {k: f[k] for k in f.files if k.startswith("bj_")}
+    assert np.array_equal(fx["bj_voxels"], m.bj_voxels())
+    g = m.bj_geometry()
+    assert g.n_views == 3024
+    y = phantom.gaussian(g.sino_shape(1), m.BJ["sino_seed"]).double()
+    assert abs(float(y.sum()) - float(fx["bj_sino_sum"])) <= 1e-12 * float(fx["bj_sino_sumsq"])
+    pick = np.r_[0:16, len(fx["bj_voxels"]) - 16:len(fx["bj_voxels"])]
+    sc = g.scheme
+    got = cone.table_adjoint_at(y, g.views, g.volume_size, g.detector_size, sc.axis_mode,
+                                sc.box_lo, sc.box_hi, sc.margin, sc.scale,
+                                fx["bj_voxels"][pick])[0].numpy()
+    assert np.abs(got - fx["bj_adjoint"][pick]).max() <= 1e-12 * np.abs(fx["bj_adjoint"]).max()

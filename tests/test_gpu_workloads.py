@@ -10,6 +10,9 @@
   deterministic forward and a split A^T);
 * the 4 x 128^3 / 90-view unroll geometry: A at 2048 rays and A^T at 2048
   voxels of the seeded item of a batch of four;
+* the 3024-view Beijing scan (256x256x64, 144 angles x 21 planes of 80x128):
+  the matched A^T over all views at 1024 seeded voxels, partitioned (view
+  parts through the workspace) and unpartitioned;
 * the 16-instance uneven FISTA-TV workload: 20 iterations batched in one
   launch per operator, eager and hipGraph replay, against the plain
   per-volume loop (rel L2 <= 1e-6), and its smallest-work instance against
@@ -22,7 +25,7 @@ import torch
 
 from conftest import rel_l2
 
-from dtvproj import batched, configs, ops, phantom, unroll
+from dtvproj import batched, configs, geometry as G, ops, phantom, unroll
 
 pytestmark = pytest.mark.gpu
 
@@ -181,3 +184,28 @@ def test_uneven16_batched_fista_tv(cuda, wfx):
     assert rmse / peak <= 1e-4
     assert abs(float(x.sum()) - float(wfx["u16_sum"])) <= 1e-4 * float(wfx["u16_sum"])
     assert abs(float(x.abs().max()) - peak) <= 1e-4 * peak
+
+
+def test_beijing_scan_adjoint_vs_oracle_fixture(cuda, wfx):
+    """The matched A^T over all 3024 views of the Beijing scan (each brick
+    culls the views it cannot reach; the default splits the rest over eight
+    parts through the workspace) against the FP64 oracle at 1024 seeded
+    voxels, 128 of them on the volume's faces."""
+    X, Y, Z, U, V = 256, 256, 64, 80, 128
+    mag = 1143.0 / 708.0
+    pitch = (1.2 * np.hypot(X, Y) * mag / (U * 21), 1.2 * Z * mag / V)
+    g = G.beijing(G.beijing_default_vectors(144, 21, (U, V), 708.0, 1143.0, pitch), (X, Y, Z),
+                  (U, V))
+    y = phantom.gaussian(g.sino_shape(1), 41)
+    # the same input (sums agree up to the summation order of the CPU threads)
+    assert abs(float(y.double().sum()) - float(wfx["bj_sino_sum"])) <= 1e-12 * float(
+        wfx["bj_sino_sumsq"])
+    vox = wfx["bj_voxels"].astype(np.int64)
+    flat = torch.from_numpy((vox[:, 2] * Y + vox[:, 1]) * X + vox[:, 0]).to(cuda)
+    ref = torch.from_numpy(wfx["bj_adjoint"])
+    yd = y.to(cuda)
+    for part in (True, False):
+        _case("beijing 3024-view A^T vs oracle voxels", partition=part)
+        got = ops.adjoint(yd, g, partition=part).reshape(-1)[flat]
+        assert rel_l2(got, ref) <= 1e-5, f"partition={part}"
+        assert rel_l2(got[-128:], ref[-128:]) <= 1e-5, f"faces, partition={part}"

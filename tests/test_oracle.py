@@ -275,6 +275,38 @@ def test_matched_adjoint_at_selected_voxels(case):
     assert float((sub - ref).abs().max()) <= 1e-13 * float(ref.abs().max())
 
 
+def _table_cases():
+    """View tables of three schemes: a Beijing scan (tilted sub-detectors,
+    source off the mid-plane), the Standard scheme and a circular orbit with
+    the source close to the volume."""
+    vs = (11, 9, 7)
+    P = G.beijing_default_vectors(2, 21, (10, 6), 30.0, 50.0, (0.6, 0.7))
+    bj = G.beijing(P, vs, (10, 6))
+    st = G.standard(G.uniform_angles(5) + 3.0, vs, (13, 8), 40.0, 80.0)
+    ci = G.circular(G.uniform_angles(6) + 1.0, vs, 1.0, (15, 9), 1.1, 14.0, 30.0)
+    return [("beijing", bj), ("standard", st), ("circular", ci)]
+
+
+@pytest.mark.parametrize("name,g", _table_cases(), ids=lambda v: v if isinstance(v, str) else "")
+def test_table_adjoint_at_selected_voxels(name, g):
+    """``table_adjoint_at`` (the per-voxel evaluation for any view table, used
+    for the full-size Beijing fixture) equals the full ``table_adjoint`` at
+    every voxel of the volume."""
+    X, Y, Z = g.volume_size
+    U, V = g.detector_size
+    sc = g.scheme
+    y = phantom.gaussian((2, 1, g.n_views, V, U), 8, dtype=torch.float64)
+    args = (g.views, g.volume_size, g.detector_size, sc.axis_mode, sc.box_lo, sc.box_hi,
+            sc.margin, sc.scale)
+    full = cone.table_adjoint(y, *args).reshape(2, -1)
+    vox = np.stack(np.meshgrid(np.arange(X), np.arange(Y), np.arange(Z), indexing="ij"),
+                   -1).reshape(-1, 3)
+    sub = cone.table_adjoint_at(y, *args, voxels=vox)
+    ref = full[:, (vox[:, 2] * Y + vox[:, 1]) * X + vox[:, 0]]
+    assert float(ref.abs().max()) > 0
+    assert float((sub - ref).abs().max()) <= 1e-13 * float(ref.abs().max())
+
+
 @pytest.mark.parametrize("case", [
     ((8, 8, 8), 1.0, (12, 10), 1.2, 6, 40.0, 80.0),
     ((9, 7, 5), 1.1, (13, 9), 1.0, 5, 30.0, 120.0),
