@@ -41,9 +41,16 @@ full-size workloads, sampled at seeded voxels so the file stays small.
   faces), evaluated by ``oracle.cone.table_adjoint_at`` over all 3024 views,
   with the sinogram's FP64 sum and sum of squares.
 
+* ``std_*`` -- the reference's Standard options geometry (256x256x64, 48
+  views over 360 degrees, 288x28 detector, SID 1000 / SDD 2000 detector
+  pixels): the matched A^T of the N(0,1) sinogram of seed 42 over all views at
+  1024 seeded voxels (896 uniform, 128 on the faces), by
+  ``oracle.cone.table_adjoint_at`` on the scheme's view table, with the
+  sinogram's FP64 sum and sum of squares.
+
 Takes several minutes on 8 cores.  Run from the repository root:
     python tests/golden/make_workloads.py          # everything
-    python tests/golden/make_workloads.py l512 u128 s256f bj   # (re)write only these arrays
+    python tests/golden/make_workloads.py l512 u128 s256f bj std   # (re)write only these arrays
 """
 import math
 import os
@@ -109,6 +116,35 @@ def build_bj(log=print):
     log(f"bj: {g.n_views} views, {vox.shape[0]} voxels")
     return dict(bj_voxels=vox, bj_adjoint=adj[0].numpy(), bj_sino_sum=np.float64(y.sum()),
                 bj_sino_sumsq=np.float64((y * y).sum()))
+
+
+STD = dict(vol=(256, 256, 64), det=(288, 28), n_views=48, sid=1000.0, sdd=2000.0,
+           sino_seed=42, n_uniform=896, n_face=128, vox_seed=2042)
+
+
+def std_geometry():
+    """The reference's Standard options geometry (options.yaml defaults)."""
+    from dtvproj import geometry as G
+    c = STD
+    return G.standard(G.uniform_angles(c["n_views"]), c["vol"], c["det"], c["sid"], c["sdd"])
+
+
+def std_voxels():
+    c = STD
+    return face_and_uniform_voxels(c["vol"], c["n_uniform"], c["n_face"], c["vox_seed"])
+
+
+def build_std(log=print):
+    c = STD
+    g = std_geometry()
+    y = phantom.gaussian(g.sino_shape(1), c["sino_seed"]).double()
+    vox = std_voxels()
+    sc = g.scheme
+    adj = cone.table_adjoint_at(y, g.views, g.volume_size, g.detector_size, sc.axis_mode,
+                                sc.box_lo, sc.box_hi, sc.margin, sc.scale, vox)
+    log(f"std: {g.n_views} views, {vox.shape[0]} voxels")
+    return dict(std_voxels=vox, std_adjoint=adj[0].numpy(), std_sino_sum=np.float64(y.sum()),
+                std_sino_sumsq=np.float64((y * y).sum()))
 
 
 def face_and_uniform_voxels(vol, n_uniform, n_face, seed):
@@ -280,7 +316,7 @@ def build_u16(log=print):
 if __name__ == "__main__":
     torch.set_num_threads(os.cpu_count() or 1)
     log = lambda s: print(s, flush=True)  # noqa: E731
-    parts = dict(l512=build_l512, u128=build_u128, s256f=build_s256f, bj=build_bj)
+    parts = dict(l512=build_l512, u128=build_u128, s256f=build_s256f, bj=build_bj, std=build_std)
     if sys.argv[1:]:
         with np.load(OUT) as f:
             fx = {k: f[k] for k in f.files}
@@ -294,5 +330,6 @@ if __name__ == "__main__":
         fx.update(build_u128(log))
         fx.update(build_s256f(log))
         fx.update(build_bj(log))
+        fx.update(build_std(log))
     np.savez_compressed(OUT, **fx)
     print(f"wrote {OUT}: {len(fx)} arrays, {os.path.getsize(OUT)} bytes")

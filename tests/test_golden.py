@@ -253,3 +253,23 @@ def test_workload_fixture_beijing_reproduces_on_a_subset():
                                 sc.box_lo, sc.box_hi, sc.margin, sc.scale,
                                 fx["bj_voxels"][pick])[0].numpy()
     assert np.abs(got - fx["bj_adjoint"][pick]).max() <= 1e-12 * np.abs(fx["bj_adjoint"]).max()
+
+
+def test_workload_fixture_standard_options_reproduces_on_a_subset():
+    """The oracle still gives the stored Standard-options A^T (all 48 views)
+    at 32 of the stored voxels."""
+    from oracle import cone
+    from dtvproj import phantom
+    m = _wgen()
+    with np.load(WFIX) as f:
+        fx = {k: f[k] for k in f.files if k.startswith("std_")}
+    assert np.array_equal(fx["std_voxels"], m.std_voxels())
+    g = m.std_geometry()
+    y = phantom.gaussian(g.sino_shape(1), m.STD["sino_seed"]).double()
+    assert abs(float(y.sum()) - float(fx["std_sino_sum"])) <= 1e-12 * float(fx["std_sino_sumsq"])
+    pick = np.r_[0:16, len(fx["std_voxels"]) - 16:len(fx["std_voxels"])]
+    sc = g.scheme
+    got = cone.table_adjoint_at(y, g.views, g.volume_size, g.detector_size, sc.axis_mode,
+                                sc.box_lo, sc.box_hi, sc.margin, sc.scale,
+                                fx["std_voxels"][pick])[0].numpy()
+    assert np.abs(got - fx["std_adjoint"][pick]).max() <= 1e-12 * np.abs(fx["std_adjoint"]).max()

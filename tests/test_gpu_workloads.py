@@ -13,6 +13,8 @@
 * the 3024-view Beijing scan (256x256x64, 144 angles x 21 planes of 80x128):
   the matched A^T over all views at 1024 seeded voxels, partitioned (view
   parts through the workspace) and unpartitioned;
+* the reference's Standard options geometry (256x256x64, 48 views of
+  288x28): the matched A^T over all views at 1024 seeded voxels;
 * the 16-instance uneven FISTA-TV workload: 20 iterations batched in one
   launch per operator, eager and hipGraph replay, against the plain
   per-volume loop (rel L2 <= 1e-6), and its smallest-work instance against
@@ -209,3 +211,25 @@ def test_beijing_scan_adjoint_vs_oracle_fixture(cuda, wfx):
         got = ops.adjoint(yd, g, partition=part).reshape(-1)[flat]
         assert rel_l2(got, ref) <= 1e-5, f"partition={part}"
         assert rel_l2(got[-128:], ref[-128:]) <= 1e-5, f"faces, partition={part}"
+
+
+def test_standard_options_adjoint_vs_oracle_fixture(cuda, wfx):
+    """The matched A^T over all 48 views of the reference's Standard options
+    geometry (options.yaml: 256x256x64, 288x28 detector, SID 1000 / SDD 2000
+    pixels), partitioned and unpartitioned, against the FP64 oracle at 1024
+    seeded voxels; the voxels no ray reaches (the detector's 28 rows see a
+    slab of the volume) are exact zeros."""
+    g = G.standard(G.uniform_angles(48), (256, 256, 64), (288, 28), 1000.0, 2000.0)
+    X, Y, Z = g.volume_size
+    y = phantom.gaussian(g.sino_shape(1), 42)
+    assert abs(float(y.double().sum()) - float(wfx["std_sino_sum"])) <= 1e-12 * float(
+        wfx["std_sino_sumsq"])
+    vox = wfx["std_voxels"].astype(np.int64)
+    flat = torch.from_numpy((vox[:, 2] * Y + vox[:, 1]) * X + vox[:, 0]).to(cuda)
+    ref = torch.from_numpy(wfx["std_adjoint"])
+    yd = y.to(cuda)
+    for part in (True, False):
+        _case("standard options A^T vs oracle voxels", partition=part)
+        got = ops.adjoint(yd, g, partition=part).reshape(-1)[flat].cpu()
+        assert rel_l2(got, ref) <= 1e-5, f"partition={part}"
+        assert bool((got[ref == 0] == 0).all()), f"zeros, partition={part}"
