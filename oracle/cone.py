@@ -566,6 +566,24 @@ def table_forward(vol, views, volume_size, detector_size, axis_mode, box_lo, box
     return _shape_out(out, vol.shape[0], len(views), V, U)
 
 
+def table_forward_at(vol, views, volume_size, detector_size, axis_mode, box_lo, box_hi,
+                     margin, scale, rays_idx, dtype=F64):
+    """``table_forward`` evaluated at the rays ``rays_idx`` (flat indices in
+    sinogram order [view, v, u]) only; returns [B, K].  Rays are built for the
+    views that hold chosen rays only (the same construction per view), then
+    the same sample enumeration."""
+    X, Y, Z = volume_size
+    U, V = (int(v) for v in detector_size)
+    idx = np.asarray(rays_idx, dtype=np.int64).reshape(-1)
+    per = U * V
+    vsel, pos = np.unique(idx // per, return_inverse=True)
+    T = np.asarray(views, dtype=np.float64)[vsel]
+    rays = table_rays(T, detector_size, axis_mode, box_lo, box_hi, margin, scale, dtype)
+    sub_idx = torch.as_tensor(pos.reshape(-1) * per + idx % per, dtype=torch.int64)
+    sub = Rays(rays.src[sub_idx], rays.d[sub_idx], rays.a_start[sub_idx],
+               rays.a_end[sub_idx], rays.step, rays.div_s, rays.off, rays.scale)
+    return forward(vol.to(dtype), sub, (X, Y, Z))
+
 def table_adjoint(sino, views, volume_size, detector_size, axis_mode, box_lo, box_hi,
                   margin, scale, dtype=F64):
     X, Y, Z = volume_size

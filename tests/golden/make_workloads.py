@@ -48,9 +48,14 @@ full-size workloads, sampled at seeded voxels so the file stays small.
   ``oracle.cone.table_adjoint_at`` on the scheme's view table, with the
   sinogram's FP64 sum and sum of squares.
 
+* ``bjf_*`` / ``stdf_*`` -- A over all views of the same Beijing scan and
+  Standard options geometry, applied to the N(0,1) volume of seed 40, at
+  4096 seeded rays (``oracle.cone.table_forward_at``), with the volume's FP64
+  sum and sum of squares.
+
 Takes several minutes on 8 cores.  Run from the repository root:
     python tests/golden/make_workloads.py          # everything
-    python tests/golden/make_workloads.py l512 u128 s256f bj std   # (re)write only these arrays
+    python tests/golden/make_workloads.py l512 u128 s256f bj std fwd   # (re)write only these arrays
 """
 import math
 import os
@@ -145,6 +150,30 @@ def build_std(log=print):
     log(f"std: {g.n_views} views, {vox.shape[0]} voxels")
     return dict(std_voxels=vox, std_adjoint=adj[0].numpy(), std_sino_sum=np.float64(y.sum()),
                 std_sino_sumsq=np.float64((y * y).sum()))
+
+
+def table_rays_pick(g, n, seed):
+    U, V = g.detector_size
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(g.n_views * U * V, size=n, replace=False)).astype(np.int64)
+
+
+def build_table_forwards(log=print):
+    """A of the seed-40 N(0,1) volume at 4096 seeded rays over all views of
+    the Beijing scan and of the Standard options geometry."""
+    out = {}
+    for key, g, seed in (("bjf", bj_geometry(), 2043), ("stdf", std_geometry(), 2044)):
+        X, Y, Z = g.volume_size
+        x = phantom.gaussian((1, 1, Z, Y, X), 40).double()
+        rays = table_rays_pick(g, 4096, seed)
+        sc = g.scheme
+        fwd = cone.table_forward_at(x, g.views, g.volume_size, g.detector_size, sc.axis_mode,
+                                    sc.box_lo, sc.box_hi, sc.margin, sc.scale, rays)
+        log(f"{key}: {g.n_views} views, {rays.shape[0]} rays")
+        out.update({f"{key}_rays": rays, f"{key}_forward": fwd[0].numpy(),
+                    f"{key}_vol_sum": np.float64(x.sum()),
+                    f"{key}_vol_sumsq": np.float64((x * x).sum())})
+    return out
 
 
 def face_and_uniform_voxels(vol, n_uniform, n_face, seed):
@@ -316,7 +345,8 @@ def build_u16(log=print):
 if __name__ == "__main__":
     torch.set_num_threads(os.cpu_count() or 1)
     log = lambda s: print(s, flush=True)  # noqa: E731
-    parts = dict(l512=build_l512, u128=build_u128, s256f=build_s256f, bj=build_bj, std=build_std)
+    parts = dict(l512=build_l512, u128=build_u128, s256f=build_s256f, bj=build_bj, std=build_std,
+                 fwd=build_table_forwards)
     if sys.argv[1:]:
         with np.load(OUT) as f:
             fx = {k: f[k] for k in f.files}
@@ -331,5 +361,6 @@ if __name__ == "__main__":
         fx.update(build_s256f(log))
         fx.update(build_bj(log))
         fx.update(build_std(log))
+        fx.update(build_table_forwards(log))
     np.savez_compressed(OUT, **fx)
     print(f"wrote {OUT}: {len(fx)} arrays, {os.path.getsize(OUT)} bytes")

@@ -273,3 +273,24 @@ def test_workload_fixture_standard_options_reproduces_on_a_subset():
                                 sc.box_lo, sc.box_hi, sc.margin, sc.scale,
                                 fx["std_voxels"][pick])[0].numpy()
     assert np.abs(got - fx["std_adjoint"][pick]).max() <= 1e-12 * np.abs(fx["std_adjoint"]).max()
+
+
+def test_workload_fixture_table_forwards_reproduce_on_a_subset():
+    """The oracle still gives the stored A of the Beijing scan and of the
+    Standard options geometry at 32 of the stored rays each, bitwise."""
+    from oracle import cone
+    from dtvproj import phantom
+    m = _wgen()
+    with np.load(WFIX) as f:
+        fx = {k: f[k] for k in f.files if k.startswith(("bjf_", "stdf_"))}
+    for key, g, seed in (("bjf", m.bj_geometry(), 2043), ("stdf", m.std_geometry(), 2044)):
+        assert np.array_equal(fx[f"{key}_rays"], m.table_rays_pick(g, 4096, seed))
+        X, Y, Z = g.volume_size
+        x = phantom.gaussian((1, 1, Z, Y, X), 40).double()
+        assert abs(float(x.sum()) - float(fx[f"{key}_vol_sum"])) <= 1e-12 * float(
+            fx[f"{key}_vol_sumsq"])
+        pick = fx[f"{key}_rays"][np.r_[0:16, 4080:4096]]
+        sc = g.scheme
+        got = cone.table_forward_at(x, g.views, g.volume_size, g.detector_size, sc.axis_mode,
+                                    sc.box_lo, sc.box_hi, sc.margin, sc.scale, pick)[0].numpy()
+        assert np.array_equal(got, fx[f"{key}_forward"][np.r_[0:16, 4080:4096]]), key

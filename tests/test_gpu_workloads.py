@@ -15,6 +15,8 @@
   parts through the workspace) and unpartitioned;
 * the reference's Standard options geometry (256x256x64, 48 views of
   288x28): the matched A^T over all views at 1024 seeded voxels;
+* A over all views of both: 4096 seeded rays each, FP32 atomics and the
+  deterministic fixed-point sums;
 * the 16-instance uneven FISTA-TV workload: 20 iterations batched in one
   launch per operator, eager and hipGraph replay, against the plain
   per-volume loop (rel L2 <= 1e-6), and its smallest-work instance against
@@ -233,3 +235,31 @@ def test_standard_options_adjoint_vs_oracle_fixture(cuda, wfx):
         got = ops.adjoint(yd, g, partition=part).reshape(-1)[flat].cpu()
         assert rel_l2(got, ref) <= 1e-5, f"partition={part}"
         assert bool((got[ref == 0] == 0).all()), f"zeros, partition={part}"
+
+
+@pytest.mark.parametrize("key", ["bjf", "stdf"])
+def test_table_geometry_forward_vs_oracle_fixture(cuda, wfx, key):
+    """A over all views of the 3024-view Beijing scan and of the Standard
+    options geometry (default launch: 512-lane workgroups, view parts), with
+    FP32 atomics and with the deterministic fixed-point sums, against the
+    FP64 oracle at 4096 seeded rays."""
+    if key == "bjf":
+        X, Y, Z, U, V = 256, 256, 64, 80, 128
+        mag = 1143.0 / 708.0
+        pitch = (1.2 * np.hypot(X, Y) * mag / (U * 21), 1.2 * Z * mag / V)
+        g = G.beijing(G.beijing_default_vectors(144, 21, (U, V), 708.0, 1143.0, pitch),
+                      (X, Y, Z), (U, V))
+    else:
+        g = G.standard(G.uniform_angles(48), (256, 256, 64), (288, 28), 1000.0, 2000.0)
+    X, Y, Z = g.volume_size
+    x = phantom.gaussian((1, 1, Z, Y, X), 40)
+    assert abs(float(x.double().sum()) - float(wfx[f"{key}_vol_sum"])) <= 1e-12 * float(
+        wfx[f"{key}_vol_sumsq"])
+    rays = torch.from_numpy(wfx[f"{key}_rays"]).to(cuda)
+    ref = torch.from_numpy(wfx[f"{key}_forward"])
+    xd = x.to(cuda)
+    for det in (False, True):
+        _case("table-geometry A vs oracle rays", key=key, deterministic=det)
+        got = ops.forward(xd, g, deterministic=det).reshape(-1)[rays].cpu()
+        assert rel_l2(got, ref) <= 1e-5, f"deterministic={det}"
+        assert bool((got[ref == 0] == 0).all())

@@ -307,6 +307,22 @@ def test_table_adjoint_at_selected_voxels(name, g):
     assert float((sub - ref).abs().max()) <= 1e-13 * float(ref.abs().max())
 
 
+@pytest.mark.parametrize("name,g", _table_cases(), ids=lambda v: v if isinstance(v, str) else "")
+def test_table_forward_at_selected_rays(name, g):
+    """``table_forward_at`` (rays built for the chosen rays' views only)
+    equals the full ``table_forward`` at the chosen rays, bitwise."""
+    X, Y, Z = g.volume_size
+    U, V = g.detector_size
+    sc = g.scheme
+    args = (g.views, g.volume_size, g.detector_size, sc.axis_mode, sc.box_lo, sc.box_hi,
+            sc.margin, sc.scale)
+    x = phantom.gaussian((2, 1, Z, Y, X), 9, dtype=torch.float64)
+    full = cone.table_forward(x, *args).reshape(2, -1)
+    idx = np.random.default_rng(4).choice(g.n_views * U * V, size=60, replace=False)
+    sub = cone.table_forward_at(x, *args, rays_idx=idx)
+    assert torch.equal(sub, full[:, torch.from_numpy(idx)])
+
+
 @pytest.mark.parametrize("case", [
     ((8, 8, 8), 1.0, (12, 10), 1.2, 6, 40.0, 80.0),
     ((9, 7, 5), 1.1, (13, 9), 1.0, 5, 30.0, 120.0),
