@@ -191,9 +191,18 @@ __device__ __forceinline__ void qvec_vals(const Vol3& v, bool hx, bool hy, bool 
 // (volumes below 2^31 voxels) from the volume's base; q and f of two
 // consecutive slices use alternate LDS buffers, so one barrier per slice
 // orders them.
+// Pointers read from a descriptor table in memory are generic (flat) to the
+// compiler, and flat loads count against the LDS wait counter as well, so
+// every LDS access in the march would also wait for the prefetched slice: the
+// march and the FISTA output address global memory explicitly.
+typedef __attribute__((address_space(1))) float gfloat;
+__device__ __forceinline__ const gfloat* glob(const float* p) { return (const gfloat*)p; }
+__device__ __forceinline__ gfloat* glob(float* p) { return (gfloat*)p; }
+
 template <bool kAsd, typename Out>
-__device__ __forceinline__ void tv_march(const Vol3& v, const float* __restrict__ f, float eps,
+__device__ __forceinline__ void tv_march(const Vol3& v, const float* __restrict__ f_, float eps,
                                          int b, Out& out) {
+  const gfloat* __restrict__ f = glob(f_);
   __shared__ float sq[2][2][kMY][kMX];
   __shared__ float sf[2][kMY][kMX];
   const int txy = march_tiles_x(v) * march_tiles_y(v);
@@ -239,7 +248,12 @@ __device__ __forceinline__ void tv_march(const Vol3& v, const float* __restrict_
   };
   if (in_xy && z0 < z1) fetch(z0, fxm, fym, fz, pre);
   __syncthreads();  // the first slice's f in sf
-  for (int z = z0; z < z1; ++z) {
+  // fully unrolled, so the prefetched values change registers by renaming
+  // rather than by copies (a copy waits for its load within the iteration)
+#pragma unroll
+  for (int k = 0; k < kMZ; ++k) {
+    const int z = z0 + k;
+    if (z >= z1) break;
     const unsigned i = (unsigned)z * s2 + row;
     const bool hy = hy_at(i), hz = z != v.nz - 1;
     float nfx = 0.f, nfy = 0.f, nfz = 0.f;
@@ -292,12 +306,14 @@ struct FistaOut {
   float* ynew;
   float step, lam, beta;
   int nonneg;
-  __device__ __forceinline__ Pre load(unsigned i) const { return Pre{g[i], xprev[i]}; }
+  __device__ __forceinline__ Pre load(unsigned i) const {
+    return Pre{glob(g)[i], glob(xprev)[i]};
+  }
   __device__ __forceinline__ void store(unsigned i, float yv, float gtv, const Pre& p) const {
     float x = yv - step * (p.g + lam * gtv);
     if (nonneg) x = fmaxf(x, 0.f);
-    xnew[i] = x;
-    ynew[i] = x + beta * (x - p.xp);
+    glob(xnew)[i] = x;
+    glob(ynew)[i] = x + beta * (x - p.xp);
   }
 };
 
@@ -428,7 +444,10 @@ __device__ __forceinline__ void tv_march4(const Vol3& v, const float* __restrict
     if (out_lane) gp = out.load(i);
   };
   if (in_xy && z0 < z1) fetch(z0, yn, zn, xa, pre);
-  for (int z = z0; z < z1; ++z) {
+#pragma unroll
+  for (int k = 0; k < kMZ; ++k) {
+    const int z = z0 + k;
+    if (z >= z1) break;
     const unsigned i = (unsigned)z * s2 + row;
     const bool hy = in_xy && hy_at(i), hz = z != v.nz - 1;
     float4 nyn = zero4, nzn = zero4;
