@@ -470,6 +470,8 @@ __device__ __forceinline__ void stage_fp64_view(const Geo& g, int a, int slot,
 // CBX/CBY/CBZ: the brick shape when known at compile time (0 = take the
 // run-time value); the instantiated shapes turn every index computation into
 // shifts and immediates.
+typedef __attribute__((address_space(3))) float lds_f32;
+
 template <int CBX, int CBY, int CBZ, int kR = kSchedR>
 __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
                                                float* __restrict__ sbase, const Geo& g,
@@ -503,6 +505,7 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
   const float bhi[3] = {ox + ex + kLineSlack, oy + ey + kLineSlack, oz + ez + kLineSlack};
   const double org[3] = {ox * kFix, oy * kFix, oz * kFix};
   const float scale = (float)g.scale;
+  const unsigned tbase = (unsigned)(size_t)(lds_f32*)tile;  // LDS byte address
 
   int nlive = g.n_views;
   if (culls_views(g)) {
@@ -563,11 +566,24 @@ __device__ __forceinline__ void fwd_brick_body(const float* __restrict__ vb,
             const float fz = (float)(uint32_t)X[2] * kInvFix;
             // tile indices are small and non-negative (the exact window
             // keeps them in the tile): full-rate 24-bit multiplies
-            const float* t0 =
-                tile + (__umul24((unsigned)iz, (unsigned)stz) + __umul24((unsigned)iy, (unsigned)sty) + ix);
+            // LDS byte address of the lower corner: one shift-add and two
+            // multiply-adds, the strides in scalar registers (left to
+            // itself the compiler re-associates this into six instructions
+            // and a constant copy per sample); the upper plane gets its own
+            // base so that both planes' pairs use small DS offsets
+            unsigned ab;
+            asm("v_lshl_add_u32 %0, %1, 2, %2\n\t"
+                "v_mad_u32_u24 %0, %3, %4, %0\n\t"
+                "v_mad_u32_u24 %0, %5, %6, %0"
+                : "=&v"(ab)
+                : "v"(ix), "s"(tbase), "v"(iy), "s"((unsigned)(4 * sty)), "v"(iz),
+                  "s"((unsigned)(4 * stz)));
+            unsigned ab1 = ab + 4u * (unsigned)stz;
+            asm volatile("" : "+v"(ab1));
+            const lds_f32* t0 = (const lds_f32*)(size_t)ab;
+            const lds_f32* t1 = (const lds_f32*)(size_t)ab1;
             const float c000 = t0[0], c100 = t0[1], c010 = t0[sty], c110 = t0[sty + 1];
-            const float c001 = t0[stz], c101 = t0[stz + 1], c011 = t0[stz + sty],
-                        c111 = t0[stz + sty + 1];
+            const float c001 = t1[0], c101 = t1[1], c011 = t1[sty], c111 = t1[sty + 1];
             const float a0 = c000 + fx * (c100 - c000);
             const float a1 = c010 + fx * (c110 - c010);
             const float a2 = c001 + fx * (c101 - c001);
